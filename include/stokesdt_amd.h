@@ -1,0 +1,166 @@
+/*
+ * stokesdt_amd.h -- C ABI of the MI355X (gfx950) Brownian-dynamics step.
+ *
+ * Drop-in boundary for the xing-liu/stokesdt hot path. Every entry point
+ * returns an int status (SDT_OK = 0, negative on error) and leaves a message
+ * for sdt_last_error(). Vectors use the reference conventions: positions are
+ * AoS [x0 y0 z0 x1 ...] (3 x npos doubles), radii npos doubles, blocks of
+ * vectors are column-major 3*npos x nrhs with an explicit leading dimension.
+ * Functions without the _dev suffix take host pointers; _dev variants take
+ * device pointers (hipMalloc'd) and keep the data on the GPU.
+ *
+ * Citations are file:line in the reference source tree.
+ */
+#ifndef STOKESDT_AMD_H_
+#define STOKESDT_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDT_OK 0
+#define SDT_EINVAL (-1)     /* invalid argument */
+#define SDT_ESTATE (-2)     /* call out of order */
+#define SDT_ENOMEM (-3)     /* device allocation failed */
+#define SDT_ECONVERGE (-4)  /* Lanczos did not converge (result still written) */
+#define SDT_EHIP (-5)       /* HIP runtime error */
+#define SDT_ECAP (-6)       /* capacity exceeded */
+
+#define SDT_FP64 0          /* all arithmetic in FP64 */
+#define SDT_FP32 1          /* spread, FFT, influence, interp and real space in
+                               FP32 with FP32 accumulation; vectors, Krylov
+                               basis and small dense algebra in FP64 */
+
+typedef struct sdt_mob_s* sdt_mob_t;
+typedef struct sdt_brwn_s* sdt_brwn_t;
+typedef struct sdt_pairlist_s* sdt_pairlist_t;
+typedef struct sdt_force_s* sdt_force_t;
+typedef struct sdt_bd_s* sdt_bd_t;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* sdt_last_error(void);
+/* Library version string. */
+const char* sdt_version(void);
+
+/* ------------------------------------------------------------------ MobSpme
+ * Replaces MobSpme (src/mob/mob_spme.h:21-125): ctor
+ * MobSpme(npos, rdi, box_size, xi, rmax, dim, porder) src/mob/mob_spme.cc:19-39,
+ * Init :53-132, Update :205-216, MulVector :219-236, RealMulVector :239-249,
+ * RecipMulVector :252-262, dim() src/mob/mob_spme.h:94.
+ * precision: SDT_FP64 or SDT_FP32. The handle owns device state and one HIP
+ * stream; a handle is not re-entrant, distinct handles may run concurrently.
+ */
+int sdt_mob_spme_create(int npos, const double* rdi, double box_size, double xi, double rmax,
+                        int dim, int porder, int precision, sdt_mob_t* out);
+int sdt_mob_spme_init(sdt_mob_t mob);
+int sdt_mob_spme_update(sdt_mob_t mob, const double* pos, const double* rdi);
+int sdt_mob_spme_update_dev(sdt_mob_t mob, const double* d_pos, const double* d_rdi);
+/* v := alpha * M * f + beta * v (src/mob/mob_base.h:52-69) */
+int sdt_mob_spme_mulvector(sdt_mob_t mob, int num_rhs, double alpha, int ldf, const double* f,
+                           double beta, int ldv, double* v);
+int sdt_mob_spme_mulvector_real(sdt_mob_t mob, int num_rhs, double alpha, int ldf,
+                                const double* f, double beta, int ldv, double* v);
+int sdt_mob_spme_mulvector_recip(sdt_mob_t mob, int num_rhs, double alpha, int ldf,
+                                 const double* f, double beta, int ldv, double* v);
+/* mode: 0 full, 1 real, 2 recip; device pointers */
+int sdt_mob_spme_mulvector_dev(sdt_mob_t mob, int mode, int num_rhs, double alpha, int ldf,
+                               const double* d_f, double beta, int ldv, double* d_v);
+int sdt_mob_spme_dim(sdt_mob_t mob, int* dim);
+/* number of real-space pairs (3x3 blocks, self included) of the last update */
+int sdt_mob_spme_real_nnz(sdt_mob_t mob, int64_t* nnz);
+int sdt_mob_spme_destroy(sdt_mob_t mob);
+
+/* One-shot SPME matvec with the semantics of the MATLAB entry point
+ * v = rpy_spme_mex(pos, rdi, L, f, mode, xi, rmax, dim, porder)
+ * (mex/rpy_spme_mex.cc:8-75): alpha = 1, beta = 0, mode "full", "real" or
+ * "recip"; FP64. sdt_rpy_spme_ex adds the precision choice. */
+int sdt_rpy_spme(const double* pos, const double* rdi, int npos, double L, const double* f,
+                 int nrhs, int ldf, const char* mode, double xi, double rmax, int dim, int porder,
+                 double* v, int ldv);
+int sdt_rpy_spme_ex(const double* pos, const double* rdi, int npos, double L, const double* f,
+                    int nrhs, int ldf, const char* mode, double xi, double rmax, int dim,
+                    int porder, int precision, double* v, int ldv);
+
+/* --------------------------------------------------------------- PairList
+ * Replaces PairListR / PairListS (src/pairlist/pair_base.h:30-69, pair_r.h,
+ * pair_s.h): mode 'R' (min-image r <= cutoff) or 'S' (2r/(a_i+a_j) <
+ * cutoff); self pairs included. Init returns the estimated nnz
+ * (src/pairlist/pair_base.cc:34-71), Build the exact nnz (:118-123),
+ * GetPairs fills caller buffers of npos+1 and nnz ints, rows sorted
+ * (:126-140). Sets are bit-identical to the reference's. */
+int sdt_pairlist_create(int npos, const double* rdi, double box_size, double cutoff, char mode,
+                        sdt_pairlist_t* out);
+int sdt_pairlist_init(sdt_pairlist_t pl, int64_t* est_nnz);
+int sdt_pairlist_build(sdt_pairlist_t pl, const double* pos, int64_t* nnz);
+int sdt_pairlist_get_pairs(sdt_pairlist_t pl, int32_t* rowptr, int32_t* colidx);
+int sdt_pairlist_destroy(sdt_pairlist_t pl);
+/* One-shot [ia, ja] = pairlist_mex(pos, rdi, L, cutoff, mode)
+ * (mex/pairlist_mex.cc:9-80): 0-based CSR. If ja_cap < nnz only *nnz is
+ * set and SDT_ECAP is returned (call again with a larger ja). */
+int sdt_pairlist(const double* pos, const double* rdi, int npos, double L, double cutoff,
+                 char mode, int32_t* ia, int32_t* ja, int64_t ja_cap, int64_t* nnz);
+
+/* ------------------------------------------------------------ BrwnLanczos
+ * Replaces BrwnLanczos(dim, max_iters, max_nrhs, tol)
+ * (src/brwn/brwn_lanczos.h:45-46), Init (src/brwn/brwn_lanczos.cc:224-275)
+ * and Compute(mob, nrhs, ldz, z, ldy, y) (:278-304): y ~= M^{1/2} z by block
+ * Lanczos in groups of max_nrhs (max 16) columns. Returns SDT_ECONVERGE if
+ * a group did not converge within max_iters (y is still written, as the
+ * reference does). ldy is honoured. */
+int sdt_brwn_lanczos_create(int dim, int max_iters, int max_nrhs, double tol, sdt_brwn_t* out);
+int sdt_brwn_lanczos_init(sdt_brwn_t br);
+int sdt_brwn_lanczos_compute(sdt_brwn_t br, sdt_mob_t mob, int num_rhs, int ldz, const double* z,
+                             int ldy, double* y);
+int sdt_brwn_lanczos_compute_dev(sdt_brwn_t br, sdt_mob_t mob, int num_rhs, int ldz,
+                                 const double* d_z, int ldy, double* d_y);
+/* iteration index at convergence of each group of the last compute (-1 if
+ * not converged), like the reference's return value */
+int sdt_brwn_lanczos_iters(sdt_brwn_t br, int* iters, int cap, int* ngroups);
+/* 0: no reorthogonalisation (reference); 1: full FP64 reorthogonalisation */
+int sdt_brwn_lanczos_set_reorth(sdt_brwn_t br, int reorth);
+int sdt_brwn_lanczos_destroy(sdt_brwn_t br);
+
+/* ------------------------------------------------------------ StericForce
+ * Replaces StericForce(npos, rdi, box_size, r0, k0)
+ * (src/force/force_steric.cc:56-69), Init (:77-96), ForceBase::Compute
+ * (src/force/force_base.cc:24-28: zero then accumulate) and Accumulate
+ * (src/force/force_steric.cc:99-112). */
+int sdt_steric_create(int npos, const double* rdi, double box_size, double r0, double k0,
+                      sdt_force_t* out);
+int sdt_steric_init(sdt_force_t fo);
+int sdt_steric_compute(sdt_force_t fo, const double* pos, const double* rdi, double* f);
+int sdt_steric_accumulate(sdt_force_t fo, const double* pos, const double* rdi, double* f);
+int sdt_steric_destroy(sdt_force_t fo);
+
+/* --------------------------------------------------------------- BD step
+ * BDSimBox::Advance (src/simbox/simbox_bd.cc:296-366) with SPME mobility,
+ * block Lanczos and steric forces, positions resident on the device. Every
+ * mob_interval steps a Gaussian block is drawn (Philox-4x32-10, stream =
+ * step / mob_interval), the mobility is updated and D = M^{1/2} Z computed;
+ * every step disp = dt M f + sqrt(2 dt) D[:, step % mob_interval] and
+ * pos += disp (kT = 1). */
+int sdt_bd_create(int npos, const double* pos, const double* rdi, double box_size, double xi,
+                  double rmax, int dim, int porder, int precision, int max_iters, int max_nrhs,
+                  double tol, double steric_r0, double steric_k0, double dt, int mob_interval,
+                  uint64_t seed, sdt_bd_t* out);
+int sdt_bd_init(sdt_bd_t bd);
+int sdt_bd_advance(sdt_bd_t bd, int nsteps);
+int sdt_bd_get_positions(sdt_bd_t bd, double* pos);
+int sdt_bd_last_iters(sdt_bd_t bd, int* iters, int cap, int* ngroups);
+int sdt_bd_destroy(sdt_bd_t bd);
+
+/* ----------------------------------------------------------------- misc */
+/* Gaussian block of the BD driver (Philox-4x32-10 + Box-Muller, FP64):
+ * element e of (seed, stream); replaces RndStream::Gaussian
+ * (src/util/rnd_stream.cc:44-65), whose MKL stream cannot be reproduced. */
+int sdt_gaussian(uint64_t seed, uint32_t stream, int64_t n, double* out);
+/* Block until all work of every handle finished. */
+int sdt_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STOKESDT_AMD_H_ */

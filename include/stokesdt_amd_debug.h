@@ -1,0 +1,40 @@
+/*
+ * stokesdt_amd_debug.h -- test and benchmark hooks of libstokesdt_amd.
+ *
+ * These have no counterpart in the reference: they expose single pipeline
+ * stages for parity tests and per-kernel timing with HIP events on the
+ * handle's own stream.
+ */
+#ifndef STOKESDT_AMD_DEBUG_H_
+#define STOKESDT_AMD_DEBUG_H_
+
+#include <stdint.h>
+#include "stokesdt_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* G = A^T B (b x b, row-major) for row-major n x b host arrays, on the
+ * device Gram kernel (FP64 MFMA). */
+int sdt_debug_gram(int64_t n, int b, const double* A, const double* B, double* G);
+/* T^{1/2} E_1 (m x kb, row-major) of a symmetric band matrix T (m x m,
+ * row-major, bandwidth kd): the host routine used by block Lanczos. */
+int sdt_debug_band_sqrt(int m, int kd, const double* T, int kb, double* S);
+/* Spread one force vector (3*npos, reference layout) with an updated
+ * mobility and return the three real meshes [c][z][y][x] (3*dim^3). */
+int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh);
+/* Mean device time (ms, HIP events on the handle's stream) of each stage of
+ * one packed matvec with b right-hand sides over `reps` repetitions:
+ * t[0] spread, t[1] forward x/y FFT, t[2] z FFT + influence + inverse z,
+ * t[3] inverse y/x FFT, t[4] interp, t[5] real space, t[6] whole matvec. */
+int sdt_bench_stage_times(sdt_mob_t mob, int b, int reps, double* t);
+/* Mean device time (ms) of one full BD step sequence: runs `nsteps` steps
+ * bracketed by events on the handle's stream. */
+int sdt_bench_bd_steps(sdt_bd_t bd, int nsteps, double* ms_total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STOKESDT_AMD_DEBUG_H_ */

@@ -1,0 +1,420 @@
+"""stokesdt_amd: MI355X (gfx950) Brownian-dynamics step of StokesDT.
+
+Python mirror of the reference's class interfaces over the C ABI of
+libstokesdt_amd.so (include/stokesdt_amd.h): MobSpme, PairListR/PairListS,
+BrwnLanczos, StericForce, the rpy_spme_mex / pairlist_mex one-shot entry
+points, and a BDSimBox-style step driver. All arithmetic runs on the GPU in
+HIP kernels; if the library is missing every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libstokesdt_amd.so")
+
+FP64 = 0
+FP32 = 1
+
+SDT_OK = 0
+SDT_EINVAL = -1
+SDT_ESTATE = -2
+SDT_ENOMEM = -3
+SDT_ECONVERGE = -4
+SDT_EHIP = -5
+SDT_ECAP = -6
+
+_lib = None
+
+_I = ctypes.c_int
+_D = ctypes.c_double
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+_PI = ctypes.POINTER(ctypes.c_int)
+
+# C signatures of include/stokesdt_amd.h and include/stokesdt_amd_debug.h
+SIGNATURES = {
+    "sdt_last_error": (ctypes.c_char_p, []),
+    "sdt_version": (ctypes.c_char_p, []),
+    "sdt_mob_spme_create": (_I, [_I, _P, _D, _D, _D, _I, _I, _I, ctypes.POINTER(_P)]),
+    "sdt_mob_spme_init": (_I, [_P]),
+    "sdt_mob_spme_update": (_I, [_P, _P, _P]),
+    "sdt_mob_spme_update_dev": (_I, [_P, _P, _P]),
+    "sdt_mob_spme_mulvector": (_I, [_P, _I, _D, _I, _P, _D, _I, _P]),
+    "sdt_mob_spme_mulvector_real": (_I, [_P, _I, _D, _I, _P, _D, _I, _P]),
+    "sdt_mob_spme_mulvector_recip": (_I, [_P, _I, _D, _I, _P, _D, _I, _P]),
+    "sdt_mob_spme_mulvector_dev": (_I, [_P, _I, _I, _D, _I, _P, _D, _I, _P]),
+    "sdt_mob_spme_dim": (_I, [_P, _PI]),
+    "sdt_mob_spme_real_nnz": (_I, [_P, _PI64]),
+    "sdt_mob_spme_destroy": (_I, [_P]),
+    "sdt_rpy_spme": (_I, [_P, _P, _I, _D, _P, _I, _I, ctypes.c_char_p, _D, _D, _I, _I, _P, _I]),
+    "sdt_rpy_spme_ex": (_I, [_P, _P, _I, _D, _P, _I, _I, ctypes.c_char_p, _D, _D, _I, _I, _I, _P,
+                             _I]),
+    "sdt_pairlist_create": (_I, [_I, _P, _D, _D, ctypes.c_char, ctypes.POINTER(_P)]),
+    "sdt_pairlist_init": (_I, [_P, _PI64]),
+    "sdt_pairlist_build": (_I, [_P, _P, _PI64]),
+    "sdt_pairlist_get_pairs": (_I, [_P, _P, _P]),
+    "sdt_pairlist_destroy": (_I, [_P]),
+    "sdt_pairlist": (_I, [_P, _P, _I, _D, _D, ctypes.c_char, _P, _P, _I64, _PI64]),
+    "sdt_brwn_lanczos_create": (_I, [_I, _I, _I, _D, ctypes.POINTER(_P)]),
+    "sdt_brwn_lanczos_init": (_I, [_P]),
+    "sdt_brwn_lanczos_compute": (_I, [_P, _P, _I, _I, _P, _I, _P]),
+    "sdt_brwn_lanczos_compute_dev": (_I, [_P, _P, _I, _I, _P, _I, _P]),
+    "sdt_brwn_lanczos_iters": (_I, [_P, _P, _I, _PI]),
+    "sdt_brwn_lanczos_set_reorth": (_I, [_P, _I]),
+    "sdt_brwn_lanczos_destroy": (_I, [_P]),
+    "sdt_steric_create": (_I, [_I, _P, _D, _D, _D, ctypes.POINTER(_P)]),
+    "sdt_steric_init": (_I, [_P]),
+    "sdt_steric_compute": (_I, [_P, _P, _P, _P]),
+    "sdt_steric_accumulate": (_I, [_P, _P, _P, _P]),
+    "sdt_steric_destroy": (_I, [_P]),
+    "sdt_bd_create": (_I, [_I, _P, _P, _D, _D, _D, _I, _I, _I, _I, _I, _D, _D, _D, _D, _I,
+                           ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "sdt_bd_init": (_I, [_P]),
+    "sdt_bd_advance": (_I, [_P, _I]),
+    "sdt_bd_get_positions": (_I, [_P, _P]),
+    "sdt_bd_last_iters": (_I, [_P, _P, _I, _PI]),
+    "sdt_bd_destroy": (_I, [_P]),
+    "sdt_gaussian": (_I, [ctypes.c_uint64, ctypes.c_uint32, _I64, _P]),
+    "sdt_device_synchronize": (_I, []),
+    "sdt_debug_gram": (_I, [_I64, _I, _P, _P, _P]),
+    "sdt_debug_band_sqrt": (_I, [_I, _I, _P, _I, _P]),
+    "sdt_debug_spme_spread": (_I, [_P, _P, _P]),
+    "sdt_bench_stage_times": (_I, [_P, _I, _I, _P]),
+    "sdt_bench_bd_steps": (_I, [_P, _I, ctypes.POINTER(_D)]),
+}
+
+
+class SdtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("stokesdt_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Loads libstokesdt_amd.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libstokesdt_amd.so is not built: run __graft_entry__.build() "
+                               "or python stokesdt_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, allow=()):
+    if rc != SDT_OK and rc not in allow:
+        raise SdtError(rc, lib().sdt_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _colmajor(a):
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim == 1:
+        a = a.reshape(-1, 1)
+    return np.asfortranarray(a)
+
+
+def version():
+    return lib().sdt_version().decode()
+
+
+class MobSpme:
+    """MobSpme (src/mob/mob_spme.h:21-125) on the GPU.
+
+    MobSpme(npos, rdi, box_size, xi, rmax, dim, porder, precision=FP64);
+    init() -> bool; update(pos, rdi); mul_vector / real_mul_vector /
+    recip_mul_vector(f, alpha, beta, v) with f, v of shape (3 npos, nrhs).
+    """
+
+    def __init__(self, npos, rdi, box_size, xi, rmax, dim, porder, precision=FP64):
+        self.npos = int(npos)
+        self.K = int(dim)
+        self.rdi = _f64(rdi)
+        h = _P()
+        _check(lib().sdt_mob_spme_create(self.npos, _ptr(self.rdi), box_size, xi, rmax, dim,
+                                         porder, precision, ctypes.byref(h)))
+        self.h = h
+
+    def init(self):
+        rc = lib().sdt_mob_spme_init(self.h)
+        if rc != SDT_OK:
+            self.init_error = lib().sdt_last_error().decode()
+            return False
+        return True
+
+    def update(self, pos, rdi=None):
+        pos = _f64(pos)
+        r = self.rdi if rdi is None else _f64(rdi)
+        _check(lib().sdt_mob_spme_update(self.h, _ptr(pos), _ptr(r)))
+
+    def dim(self):
+        d = ctypes.c_int(0)
+        _check(lib().sdt_mob_spme_dim(self.h, ctypes.byref(d)))
+        return d.value
+
+    def real_nnz(self):
+        n = ctypes.c_int64(0)
+        _check(lib().sdt_mob_spme_real_nnz(self.h, ctypes.byref(n)))
+        return n.value
+
+    def _mul(self, fn, f, alpha, beta, v):
+        f = _colmajor(f)
+        n, nrhs = f.shape
+        out = np.zeros((n, nrhs), order="F") if v is None else _colmajor(v).copy(order="F")
+        _check(fn(self.h, nrhs, alpha, n, _ptr(f), beta, n, _ptr(out)))
+        return out
+
+    def mul_vector(self, f, alpha=1.0, beta=0.0, v=None):
+        return self._mul(lib().sdt_mob_spme_mulvector, f, alpha, beta, v)
+
+    def real_mul_vector(self, f, alpha=1.0, beta=0.0, v=None):
+        return self._mul(lib().sdt_mob_spme_mulvector_real, f, alpha, beta, v)
+
+    def recip_mul_vector(self, f, alpha=1.0, beta=0.0, v=None):
+        return self._mul(lib().sdt_mob_spme_mulvector_recip, f, alpha, beta, v)
+
+    def debug_spread(self, f):
+        """Spread of one 3N force vector; returns meshes [3][K][K][K]."""
+        f = _f64(f)
+        K = self.K
+        mesh = np.zeros(3 * K * K * K)
+        _check(lib().sdt_debug_spme_spread(self.h, _ptr(f), _ptr(mesh)))
+        return mesh.reshape(3, K, K, K)
+
+    def stage_times(self, b, reps=5):
+        t = np.zeros(7)
+        _check(lib().sdt_bench_stage_times(self.h, b, reps, _ptr(t)))
+        return t
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_mob_spme_destroy(self.h)
+            self.h = None
+
+
+def rpy_spme(pos, rdi, L, f, mode, xi, rmax, dim, porder, precision=FP64):
+    """v = rpy_spme_mex(pos, rdi, L, f, mode, xi, rmax, dim, porder)
+    (mex/rpy_spme_mex.cc:8-75). pos: (3, npos) or flat AoS; f: (3 npos, nrhs)."""
+    pos = _f64(np.asarray(pos).T.reshape(-1) if np.ndim(pos) == 2 and np.shape(pos)[0] == 3
+               else np.asarray(pos).reshape(-1))
+    rdi = _f64(rdi).reshape(-1)
+    f = _colmajor(f)
+    n, nrhs = f.shape
+    v = np.zeros((n, nrhs), order="F")
+    _check(lib().sdt_rpy_spme_ex(_ptr(pos), _ptr(rdi), rdi.size, L, _ptr(f), nrhs, n,
+                                 mode.encode(), xi, rmax, dim, porder, precision, _ptr(v), n))
+    return v
+
+
+class PairList:
+    """PairListR ('R') / PairListS ('S') (src/pairlist/pair_base.h:30-69)."""
+
+    def __init__(self, npos, rdi, box_size, cutoff, mode):
+        self.npos = int(npos)
+        self.rdi = _f64(rdi)
+        h = _P()
+        _check(lib().sdt_pairlist_create(self.npos, _ptr(self.rdi), box_size, cutoff,
+                                         mode.encode()[:1], ctypes.byref(h)))
+        self.h = h
+
+    def init(self):
+        est = ctypes.c_int64(0)
+        rc = lib().sdt_pairlist_init(self.h, ctypes.byref(est))
+        return est.value if rc == SDT_OK else 0
+
+    def build(self, pos):
+        pos = _f64(pos)
+        n = ctypes.c_int64(0)
+        _check(lib().sdt_pairlist_build(self.h, _ptr(pos), ctypes.byref(n)))
+        self.nnz = n.value
+        return n.value
+
+    def get_pairs(self):
+        ia = np.zeros(self.npos + 1, dtype=np.int32)
+        ja = np.zeros(max(self.nnz, 1), dtype=np.int32)
+        _check(lib().sdt_pairlist_get_pairs(self.h, _ptr(ia), _ptr(ja)))
+        return ia, ja[:self.nnz]
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_pairlist_destroy(self.h)
+            self.h = None
+
+
+def PairListR(npos, rdi, box_size, cutoff):
+    return PairList(npos, rdi, box_size, cutoff, "R")
+
+
+def PairListS(npos, rdi, box_size, cutoff):
+    return PairList(npos, rdi, box_size, cutoff, "S")
+
+
+def pairlist(pos, rdi, L, cutoff, mode):
+    """[ia, ja] = pairlist_mex(pos, rdi, L, cutoff, mode) (mex/pairlist_mex.cc:9-80)."""
+    pos = _f64(np.asarray(pos).T.reshape(-1) if np.ndim(pos) == 2 and np.shape(pos)[0] == 3
+               else np.asarray(pos).reshape(-1))
+    rdi = _f64(rdi).reshape(-1)
+    n = rdi.size
+    ia = np.zeros(n + 1, dtype=np.int32)
+    nnz = ctypes.c_int64(0)
+    cap = 64 * n + 64
+    while True:
+        ja = np.zeros(cap, dtype=np.int32)
+        rc = lib().sdt_pairlist(_ptr(pos), _ptr(rdi), n, L, cutoff, mode.encode()[:1], _ptr(ia),
+                                _ptr(ja), cap, ctypes.byref(nnz))
+        if rc == SDT_ECAP:
+            cap = nnz.value
+            continue
+        _check(rc)
+        return ia, ja[:nnz.value].copy()
+
+
+class BrwnLanczos:
+    """BrwnLanczos(dim, max_iters, max_nrhs, tol) (src/brwn/brwn_lanczos.h:45-46)."""
+
+    def __init__(self, dim, max_iters, max_nrhs, tol):
+        h = _P()
+        _check(lib().sdt_brwn_lanczos_create(dim, max_iters, max_nrhs, tol, ctypes.byref(h)))
+        self.h = h
+        self.dim = dim
+
+    def init(self):
+        return lib().sdt_brwn_lanczos_init(self.h) == SDT_OK
+
+    def set_reorth(self, on):
+        _check(lib().sdt_brwn_lanczos_set_reorth(self.h, 1 if on else 0))
+
+    def compute(self, mob, z):
+        """y ~= M^{1/2} z; z: (dim, nrhs). Returns (y, converged)."""
+        z = _colmajor(z)
+        n, nrhs = z.shape
+        y = np.zeros((n, nrhs), order="F")
+        rc = lib().sdt_brwn_lanczos_compute(self.h, mob.h, nrhs, n, _ptr(z), n, _ptr(y))
+        _check(rc, allow=(SDT_ECONVERGE,))
+        return y, rc == SDT_OK
+
+    def iters(self):
+        out = np.zeros(4096, dtype=np.int32)
+        ng = ctypes.c_int(0)
+        _check(lib().sdt_brwn_lanczos_iters(self.h, _ptr(out), 4096, ctypes.byref(ng)))
+        return out[:ng.value].copy()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_brwn_lanczos_destroy(self.h)
+            self.h = None
+
+
+class StericForce:
+    """StericForce(npos, rdi, box_size, r0, k0) (src/force/force_steric.cc:56-112)."""
+
+    def __init__(self, npos, rdi, box_size, r0, k0):
+        self.npos = int(npos)
+        self.rdi = _f64(rdi)
+        h = _P()
+        _check(lib().sdt_steric_create(self.npos, _ptr(self.rdi), box_size, r0, k0,
+                                       ctypes.byref(h)))
+        self.h = h
+
+    def init(self):
+        return lib().sdt_steric_init(self.h) == SDT_OK
+
+    def compute(self, pos):
+        pos = _f64(pos)
+        f = np.zeros(3 * self.npos)
+        _check(lib().sdt_steric_compute(self.h, _ptr(pos), _ptr(self.rdi), _ptr(f)))
+        return f
+
+    def accumulate(self, pos, f):
+        pos = _f64(pos)
+        f = _f64(f).copy()
+        _check(lib().sdt_steric_accumulate(self.h, _ptr(pos), _ptr(self.rdi), _ptr(f)))
+        return f
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_steric_destroy(self.h)
+            self.h = None
+
+
+class BDSim:
+    """BDSimBox::Advance (src/simbox/simbox_bd.cc:296-366) on the GPU."""
+
+    def __init__(self, pos, rdi, box_size, xi, rmax, dim, porder, precision, max_iters, max_nrhs,
+                 tol, steric_r0, steric_k0, dt, mob_interval, seed):
+        pos = _f64(pos)
+        self.rdi = _f64(rdi)
+        self.npos = self.rdi.size
+        h = _P()
+        _check(lib().sdt_bd_create(self.npos, _ptr(pos), _ptr(self.rdi), box_size, xi, rmax, dim,
+                                   porder, precision, max_iters, max_nrhs, tol, steric_r0,
+                                   steric_k0, dt, mob_interval, seed, ctypes.byref(h)))
+        self.h = h
+
+    def init(self):
+        _check(lib().sdt_bd_init(self.h))
+        return True
+
+    def advance(self, nsteps):
+        _check(lib().sdt_bd_advance(self.h, nsteps))
+
+    def timed_advance(self, nsteps):
+        ms = ctypes.c_double(0.0)
+        _check(lib().sdt_bench_bd_steps(self.h, nsteps, ctypes.byref(ms)))
+        return ms.value
+
+    def positions(self):
+        out = np.zeros(3 * self.npos)
+        _check(lib().sdt_bd_get_positions(self.h, _ptr(out)))
+        return out
+
+    def last_iters(self):
+        out = np.zeros(4096, dtype=np.int32)
+        ng = ctypes.c_int(0)
+        _check(lib().sdt_bd_last_iters(self.h, _ptr(out), 4096, ctypes.byref(ng)))
+        return out[:ng.value].copy()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_bd_destroy(self.h)
+            self.h = None
+
+
+def gaussian(seed, stream, n):
+    out = np.zeros(n)
+    _check(lib().sdt_gaussian(seed, stream, n, _ptr(out)))
+    return out
+
+
+def debug_gram(A, B):
+    A = _f64(A)
+    B = _f64(B)
+    n, b = A.shape
+    G = np.zeros((b, b))
+    _check(lib().sdt_debug_gram(n, b, _ptr(A), _ptr(B), _ptr(G)))
+    return G
+
+
+def band_sqrt(T, kd, kb):
+    """T^{1/2} E_1 for a symmetric band matrix (host routine of block Lanczos)."""
+    T = _f64(T)
+    m = T.shape[0]
+    S = np.zeros((m, kb))
+    _check(lib().sdt_debug_band_sqrt(m, kd, _ptr(T), kb, _ptr(S)))
+    return S

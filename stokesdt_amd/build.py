@@ -1,0 +1,66 @@
+"""Builds stokesdt_amd/libstokesdt_amd.so for gfx950 with hipcc (in-tree)."""
+import concurrent.futures
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libstokesdt_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("SDT_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = [
+    ("spme_kernels.hip", []),
+    ("sort_kernels.hip", []),
+    # exact FP64 pair decisions: no contraction into fused multiply-adds
+    ("pair_kernels.hip", ["-ffp-contract=off"]),
+    ("real_kernels.hip", []),
+    ("block_kernels.hip", []),
+    ("mob_gpu.hip", []),
+    ("capi.hip", []),
+]
+HOST_SOURCES = [("host_linalg.cpp", [])]
+
+HIP_FLAGS = [
+    "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-Wno-unused-result",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(line for line in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in line)
+    if r.returncode != 0:
+        raise RuntimeError("command failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def build(verbose=False, jobs=8):
+    os.makedirs(OBJ, exist_ok=True)
+    cmds = []
+    objs = []
+    for src, extra in HIP_SOURCES:
+        o = os.path.join(OBJ, src + ".o")
+        objs.append(o)
+        cmds.append([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", o])
+    for src, extra in HOST_SOURCES:
+        o = os.path.join(OBJ, src + ".o")
+        objs.append(o)
+        cmds.append(["g++", "-O2", "-fPIC", "-std=c++17"] + extra +
+                    ["-c", os.path.join(CSRC, src), "-o", o])
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,23 @@
+// Tall-skinny FP64 block kernels and vector layout conversion.
+#pragma once
+#include "sdt_internal.hpp"
+
+namespace sdt {
+
+// G (b x b, row-major, device) = A^T B for packed n x b blocks A, B
+void gram(long long n, int b, const double* A, const double* B, double* d_G, DevBuf<double>& part,
+          hipStream_t s);
+// C = beta C + alpha [A_0 .. A_{nblk-1}] S, blocks n x b at stride blk_stride,
+// S row-major (nblk*b) x b on the device
+void tsmm(long long n, int b, int nblk, const double* A, long long blk_stride, const double* d_S,
+          double alpha, double beta, double* C, hipStream_t s);
+// sum of squares (synchronises)
+double sumsq(long long n, const double* x, DevBuf<double>& part, hipStream_t s);
+// reference layout (column-major 3N x b, leading dimension ld, original
+// particle order) <-> packed layout (row-major 3N x b, sorted order)
+void pack(int npos, int b, const int* perm, int ldf, const double* f, double* X, hipStream_t s);
+void unpack(int npos, int b, const int* perm, int ldv, const double* X, double* v, hipStream_t s);
+void axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s);
+void fill(long long n, double v, double* p, hipStream_t s);
+
+}  // namespace sdt

@@ -1,0 +1,531 @@
+// extern "C" boundary of libstokesdt_amd (declared in include/stokesdt_amd.h
+// and include/stokesdt_amd_debug.h).
+#include <cstring>
+#include <string>
+#include "../../include/stokesdt_amd.h"
+#include "../../include/stokesdt_amd_debug.h"
+#include "mob_gpu.hpp"
+
+namespace sdt {
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+const char* last_error() { return g_err.c_str(); }
+}  // namespace sdt
+
+using namespace sdt;
+
+struct sdt_mob_s { MobSpmeGpu* m; };
+struct sdt_brwn_s { BrwnLanczosGpu* b; };
+struct sdt_pairlist_s {
+    PairListGpu pl;
+    std::vector<double> rdi;
+    DevBuf<double> d_pos;
+    hipStream_t s = nullptr;
+    bool inited = false, built = false;
+};
+struct sdt_force_s { StericGpu* f; hipStream_t s; };
+struct sdt_bd_s { BDGpu* bd; };
+
+template <typename F>
+static int guard(F&& fn) {
+    try {
+        fn();
+        return SDT_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return SDT_ENOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SDT_EINVAL;
+    }
+}
+
+#define SDT_CHECK_HANDLE(h)                                              \
+    do {                                                                 \
+        if (!(h)) {                                                      \
+            set_error("null handle");                                    \
+            return SDT_EINVAL;                                           \
+        }                                                                \
+    } while (0)
+
+extern "C" {
+
+const char* sdt_last_error(void) { return last_error(); }
+const char* sdt_version(void) { return "stokesdt_amd 0.1.0 (gfx950)"; }
+
+// ---------------------------------------------------------------- MobSpme --
+int sdt_mob_spme_create(int npos, const double* rdi, double box_size, double xi, double rmax,
+                        int dim, int porder, int precision, sdt_mob_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] {
+        MobSpmeGpu* m = new MobSpmeGpu(npos, rdi, box_size, xi, rmax, dim, porder, precision);
+        *out = new sdt_mob_s{m};
+    });
+}
+int sdt_mob_spme_init(sdt_mob_t mob) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] { mob->m->init(); });
+}
+int sdt_mob_spme_update(sdt_mob_t mob, const double* pos, const double* rdi) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        mob->m->update(pos, rdi, false);
+        SDT_HIP_CHECK(hipStreamSynchronize(mob->m->stream()));
+    });
+}
+int sdt_mob_spme_update_dev(sdt_mob_t mob, const double* d_pos, const double* d_rdi) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        mob->m->update(d_pos, d_rdi, true);
+        SDT_HIP_CHECK(hipStreamSynchronize(mob->m->stream()));
+    });
+}
+int sdt_mob_spme_mulvector(sdt_mob_t mob, int num_rhs, double alpha, int ldf, const double* f,
+                           double beta, int ldv, double* v) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] { mob->m->mul_vector(0, num_rhs, alpha, ldf, f, beta, ldv, v, false); });
+}
+int sdt_mob_spme_mulvector_real(sdt_mob_t mob, int num_rhs, double alpha, int ldf,
+                                const double* f, double beta, int ldv, double* v) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] { mob->m->mul_vector(1, num_rhs, alpha, ldf, f, beta, ldv, v, false); });
+}
+int sdt_mob_spme_mulvector_recip(sdt_mob_t mob, int num_rhs, double alpha, int ldf,
+                                 const double* f, double beta, int ldv, double* v) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] { mob->m->mul_vector(2, num_rhs, alpha, ldf, f, beta, ldv, v, false); });
+}
+int sdt_mob_spme_mulvector_dev(sdt_mob_t mob, int mode, int num_rhs, double alpha, int ldf,
+                               const double* d_f, double beta, int ldv, double* d_v) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] { mob->m->mul_vector(mode, num_rhs, alpha, ldf, d_f, beta, ldv, d_v, true); });
+}
+int sdt_mob_spme_dim(sdt_mob_t mob, int* dim) {
+    SDT_CHECK_HANDLE(mob);
+    if (!dim) { set_error("null output"); return SDT_EINVAL; }
+    *dim = mob->m->dim();
+    return SDT_OK;
+}
+int sdt_mob_spme_real_nnz(sdt_mob_t mob, int64_t* nnz) {
+    SDT_CHECK_HANDLE(mob);
+    if (!nnz) { set_error("null output"); return SDT_EINVAL; }
+    *nnz = mob->m->real_nnz();
+    return SDT_OK;
+}
+int sdt_mob_spme_destroy(sdt_mob_t mob) {
+    if (!mob) return SDT_OK;
+    return guard([&] {
+        delete mob->m;
+        delete mob;
+    });
+}
+
+int sdt_rpy_spme_ex(const double* pos, const double* rdi, int npos, double L, const double* f,
+                    int nrhs, int ldf, const char* mode, double xi, double rmax, int dim,
+                    int porder, int precision, double* v, int ldv) {
+    // argument checks of mex/rpy_spme_mex.cc:12-51
+    if (!pos || !rdi || !f || !v || npos <= 0) { set_error("invalid array arguments"); return SDT_EINVAL; }
+    if (L <= 0. || xi <= 0. || rmax <= 0 || dim < 1) {
+        set_error("Scalar arguments must be positive.");
+        return SDT_EINVAL;
+    }
+    if (porder <= 1) { set_error("porder must be greater than 1."); return SDT_EINVAL; }
+    if (2. * rmax >= L) { set_error("rmax must be less than L/2."); return SDT_EINVAL; }
+    if (!mode) { set_error("invalid argument: mode."); return SDT_EINVAL; }
+    int md;
+    if (strcmp(mode, "full") == 0) md = 0;
+    else if (strcmp(mode, "real") == 0) md = 1;
+    else if (strcmp(mode, "recip") == 0) md = 2;
+    else { set_error("mode must be full, real, or recip."); return SDT_EINVAL; }
+    return guard([&] {
+        MobSpmeGpu m(npos, rdi, L, xi, rmax, dim, porder, precision);
+        m.init();
+        m.update(pos, rdi, false);
+        m.mul_vector(md, nrhs, 1.0, ldf, f, 0.0, ldv, v, false);
+    });
+}
+int sdt_rpy_spme(const double* pos, const double* rdi, int npos, double L, const double* f,
+                 int nrhs, int ldf, const char* mode, double xi, double rmax, int dim, int porder,
+                 double* v, int ldv) {
+    return sdt_rpy_spme_ex(pos, rdi, npos, L, f, nrhs, ldf, mode, xi, rmax, dim, porder, SDT_FP64,
+                           v, ldv);
+}
+
+// --------------------------------------------------------------- PairList --
+int sdt_pairlist_create(int npos, const double* rdi, double box_size, double cutoff, char mode,
+                        sdt_pairlist_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    if (npos <= 0 || !rdi) { set_error("npos must be positive and rdi non-null"); return SDT_EINVAL; }
+    return guard([&] {
+        sdt_pairlist_s* p = new sdt_pairlist_s();
+        p->rdi.assign(rdi, rdi + npos);
+        p->pl.npos = npos;
+        p->pl.box = box_size;
+        p->pl.cutoff = cutoff;
+        p->pl.mode = mode;
+        hipError_t e = hipStreamCreateWithFlags(&p->s, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete p;
+            throw Error{SDT_EHIP, "hipStreamCreate failed"};
+        }
+        *out = p;
+    });
+}
+int sdt_pairlist_init(sdt_pairlist_t pl, int64_t* est_nnz) {
+    SDT_CHECK_HANDLE(pl);
+    return guard([&] {
+        int est = pl->pl.init(pl->pl.npos, pl->rdi.data(), pl->pl.box, pl->pl.cutoff, pl->pl.mode);
+        if (est_nnz) *est_nnz = est;
+        SDT_REQUIRE(est > 0, SDT_EINVAL, "pair list init failed (invalid parameters)");
+        pl->d_pos.reserve(3 * (size_t)pl->pl.npos);
+        pl->inited = true;
+    });
+}
+int sdt_pairlist_build(sdt_pairlist_t pl, const double* pos, int64_t* nnz) {
+    SDT_CHECK_HANDLE(pl);
+    return guard([&] {
+        SDT_REQUIRE(pl->inited, SDT_ESTATE, "build called before init");
+        SDT_REQUIRE(pos != nullptr, SDT_EINVAL, "pos is null");
+        SDT_HIP_CHECK(hipMemcpyAsync(pl->d_pos.get(), pos, sizeof(double) * 3 * pl->pl.npos,
+                                     hipMemcpyHostToDevice, pl->s));
+        int n = pl->pl.build(pl->d_pos.get(), nullptr, nullptr, true, pl->s);
+        SDT_HIP_CHECK(hipStreamSynchronize(pl->s));
+        if (nnz) *nnz = n;
+        pl->built = true;
+    });
+}
+int sdt_pairlist_get_pairs(sdt_pairlist_t pl, int32_t* rowptr, int32_t* colidx) {
+    SDT_CHECK_HANDLE(pl);
+    return guard([&] {
+        SDT_REQUIRE(pl->built, SDT_ESTATE, "get_pairs called before build");
+        SDT_REQUIRE(rowptr && (colidx || pl->pl.nnz == 0), SDT_EINVAL, "null output arrays");
+        SDT_HIP_CHECK(hipMemcpy(rowptr, pl->pl.d_rowptr.get(), sizeof(int) * (pl->pl.npos + 1),
+                                hipMemcpyDeviceToHost));
+        if (pl->pl.nnz > 0)
+            SDT_HIP_CHECK(hipMemcpy(colidx, pl->pl.d_colidx.get(), sizeof(int) * pl->pl.nnz,
+                                    hipMemcpyDeviceToHost));
+    });
+}
+int sdt_pairlist_destroy(sdt_pairlist_t pl) {
+    if (!pl) return SDT_OK;
+    return guard([&] {
+        if (pl->s) (void)hipStreamDestroy(pl->s);
+        delete pl;
+    });
+}
+int sdt_pairlist(const double* pos, const double* rdi, int npos, double L, double cutoff,
+                 char mode, int32_t* ia, int32_t* ja, int64_t ja_cap, int64_t* nnz) {
+    // argument checks of mex/pairlist_mex.cc:13-53
+    if (!pos || !rdi || !ia || npos <= 0) { set_error("invalid array arguments"); return SDT_EINVAL; }
+    if (L <= 0. || cutoff <= 0.) { set_error("Scalar arguments must be positive."); return SDT_EINVAL; }
+    if (2. * cutoff >= L) { set_error("cutoff must be less than L/2."); return SDT_EINVAL; }
+    if (mode != 'R' && mode != 'S') { set_error("mode must be R or S."); return SDT_EINVAL; }
+    sdt_pairlist_t h = nullptr;
+    int rc = sdt_pairlist_create(npos, rdi, L, cutoff, mode, &h);
+    if (rc) return rc;
+    int64_t est = 0, n = 0;
+    rc = sdt_pairlist_init(h, &est);
+    if (!rc) rc = sdt_pairlist_build(h, pos, &n);
+    if (!rc) {
+        if (nnz) *nnz = n;
+        if (n > ja_cap || (n > 0 && !ja)) {
+            set_error("ja capacity is smaller than nnz");
+            rc = SDT_ECAP;
+        } else {
+            rc = sdt_pairlist_get_pairs(h, ia, ja);
+        }
+    }
+    sdt_pairlist_destroy(h);
+    return rc;
+}
+
+// ------------------------------------------------------------ BrwnLanczos --
+int sdt_brwn_lanczos_create(int dim, int max_iters, int max_nrhs, double tol, sdt_brwn_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] { *out = new sdt_brwn_s{new BrwnLanczosGpu(dim, max_iters, max_nrhs, tol)}; });
+}
+int sdt_brwn_lanczos_init(sdt_brwn_t br) {
+    SDT_CHECK_HANDLE(br);
+    return guard([&] { br->b->init(); });
+}
+static int lanczos_status(BrwnLanczosGpu* b) {
+    for (int it : b->iters())
+        if (it < 0) {
+            set_error("Lanczos did not converge for at least one group (result written)");
+            return SDT_ECONVERGE;
+        }
+    return SDT_OK;
+}
+int sdt_brwn_lanczos_compute(sdt_brwn_t br, sdt_mob_t mob, int num_rhs, int ldz, const double* z,
+                             int ldy, double* y) {
+    SDT_CHECK_HANDLE(br);
+    SDT_CHECK_HANDLE(mob);
+    int rc = guard([&] { br->b->compute(mob->m, num_rhs, ldz, z, ldy, y, false); });
+    return rc ? rc : lanczos_status(br->b);
+}
+int sdt_brwn_lanczos_compute_dev(sdt_brwn_t br, sdt_mob_t mob, int num_rhs, int ldz,
+                                 const double* d_z, int ldy, double* d_y) {
+    SDT_CHECK_HANDLE(br);
+    SDT_CHECK_HANDLE(mob);
+    int rc = guard([&] { br->b->compute(mob->m, num_rhs, ldz, d_z, ldy, d_y, true); });
+    return rc ? rc : lanczos_status(br->b);
+}
+int sdt_brwn_lanczos_iters(sdt_brwn_t br, int* iters, int cap, int* ngroups) {
+    SDT_CHECK_HANDLE(br);
+    const std::vector<int>& it = br->b->iters();
+    if (ngroups) *ngroups = (int)it.size();
+    for (int i = 0; i < (int)it.size() && i < cap && iters; i++) iters[i] = it[i];
+    return SDT_OK;
+}
+int sdt_brwn_lanczos_set_reorth(sdt_brwn_t br, int reorth) {
+    SDT_CHECK_HANDLE(br);
+    br->b->reorth = reorth ? 1 : 0;
+    return SDT_OK;
+}
+int sdt_brwn_lanczos_destroy(sdt_brwn_t br) {
+    if (!br) return SDT_OK;
+    return guard([&] {
+        delete br->b;
+        delete br;
+    });
+}
+
+// ---------------------------------------------------------------- Steric ---
+int sdt_steric_create(int npos, const double* rdi, double box_size, double r0, double k0,
+                      sdt_force_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] {
+        StericGpu* f = new StericGpu(npos, rdi, box_size, r0, k0);
+        hipStream_t s;
+        hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete f;
+            throw Error{SDT_EHIP, "hipStreamCreate failed"};
+        }
+        *out = new sdt_force_s{f, s};
+    });
+}
+int sdt_steric_init(sdt_force_t fo) {
+    SDT_CHECK_HANDLE(fo);
+    return guard([&] { fo->f->init(); });
+}
+int sdt_steric_accumulate(sdt_force_t fo, const double* pos, const double* rdi, double* f) {
+    (void)rdi;
+    SDT_CHECK_HANDLE(fo);
+    return guard([&] {
+        SDT_REQUIRE(pos && f, SDT_EINVAL, "null array");
+        fo->f->accumulate(pos, f, false, fo->s);
+    });
+}
+int sdt_steric_compute(sdt_force_t fo, const double* pos, const double* rdi, double* f) {
+    (void)rdi;
+    SDT_CHECK_HANDLE(fo);
+    // ForceBase::Compute zeroes f before accumulating
+    // (src/force/force_base.cc:24-28)
+    return guard([&] {
+        SDT_REQUIRE(pos && f, SDT_EINVAL, "null array");
+        memset(f, 0, sizeof(double) * 3 * (size_t)fo->f->npos());
+        fo->f->accumulate(pos, f, false, fo->s);
+    });
+}
+int sdt_steric_destroy(sdt_force_t fo) {
+    if (!fo) return SDT_OK;
+    return guard([&] {
+        (void)hipStreamDestroy(fo->s);
+        delete fo->f;
+        delete fo;
+    });
+}
+
+// -------------------------------------------------------------------- BD ---
+int sdt_bd_create(int npos, const double* pos, const double* rdi, double box_size, double xi,
+                  double rmax, int dim, int porder, int precision, int max_iters, int max_nrhs,
+                  double tol, double steric_r0, double steric_k0, double dt, int mob_interval,
+                  uint64_t seed, sdt_bd_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] {
+        BDGpu* bd = new BDGpu(npos, pos, rdi, box_size, xi, rmax, dim, porder, precision, max_iters,
+                              max_nrhs, tol, steric_r0, steric_k0, dt, mob_interval, seed);
+        *out = new sdt_bd_s{bd};
+    });
+}
+int sdt_bd_init(sdt_bd_t bd) {
+    SDT_CHECK_HANDLE(bd);
+    return guard([&] { bd->bd->init(); });
+}
+int sdt_bd_advance(sdt_bd_t bd, int nsteps) {
+    SDT_CHECK_HANDLE(bd);
+    return guard([&] { bd->bd->advance(nsteps); });
+}
+int sdt_bd_get_positions(sdt_bd_t bd, double* pos) {
+    SDT_CHECK_HANDLE(bd);
+    return guard([&] { bd->bd->get_positions(pos); });
+}
+int sdt_bd_last_iters(sdt_bd_t bd, int* iters, int cap, int* ngroups) {
+    SDT_CHECK_HANDLE(bd);
+    const std::vector<int>& it = bd->bd->last_iters();
+    if (ngroups) *ngroups = (int)it.size();
+    for (int i = 0; i < (int)it.size() && i < cap && iters; i++) iters[i] = it[i];
+    return SDT_OK;
+}
+int sdt_bd_destroy(sdt_bd_t bd) {
+    if (!bd) return SDT_OK;
+    return guard([&] {
+        delete bd->bd;
+        delete bd;
+    });
+}
+
+// ------------------------------------------------------------------ misc ---
+int sdt_gaussian(uint64_t seed, uint32_t stream, int64_t n, double* out) {
+    if (!out || n < 0) { set_error("invalid arguments"); return SDT_EINVAL; }
+    return guard([&] {
+        DevBuf<double> d;
+        d.reserve((size_t)n);
+        gaussian_fill_device(seed, stream, n, d.get(), 0);
+        SDT_HIP_CHECK(hipMemcpy(out, d.get(), sizeof(double) * n, hipMemcpyDeviceToHost));
+    });
+}
+int sdt_device_synchronize(void) {
+    return guard([&] { SDT_HIP_CHECK(hipDeviceSynchronize()); });
+}
+
+// ----------------------------------------------------------- debug hooks ---
+int sdt_debug_gram(int64_t n, int b, const double* A, const double* B, double* G) {
+    return guard([&] {
+        SDT_REQUIRE(n > 0 && b >= 1 && b <= 16 && A && B && G, SDT_EINVAL, "bad arguments");
+        DevBuf<double> dA, dB, dG, part;
+        dA.reserve((size_t)n * b);
+        dB.reserve((size_t)n * b);
+        dG.reserve((size_t)b * b);
+        SDT_HIP_CHECK(hipMemcpy(dA.get(), A, sizeof(double) * n * b, hipMemcpyHostToDevice));
+        SDT_HIP_CHECK(hipMemcpy(dB.get(), B, sizeof(double) * n * b, hipMemcpyHostToDevice));
+        gram(n, b, dA.get(), dB.get(), dG.get(), part, 0);
+        SDT_HIP_CHECK(hipMemcpy(G, dG.get(), sizeof(double) * b * b, hipMemcpyDeviceToHost));
+    });
+}
+int sdt_debug_band_sqrt(int m, int kd, const double* T, int kb, double* S) {
+    return guard([&] {
+        SDT_REQUIRE(m > 0 && kd >= 0 && kb > 0 && kb <= m && T && S, SDT_EINVAL, "bad arguments");
+        BandSqrt bs;
+        bs.compute(m, kd, T, kb, S);
+    });
+}
+int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        MobSpmeGpu* m = mob->m;
+        SDT_REQUIRE(m->updated(), SDT_ESTATE, "mobility not updated");
+        const int n3 = m->dim();
+        const size_t K3 = (size_t)m->K() * m->K() * m->K();
+        DevBuf<double> df, dX;
+        df.reserve(n3);
+        dX.reserve(n3);
+        hipStream_t s = m->stream();
+        SDT_HIP_CHECK(hipMemcpyAsync(df.get(), f, sizeof(double) * n3, hipMemcpyHostToDevice, s));
+        pack(m->npos(), 1, m->perm(), n3, df.get(), dX.get(), s);
+        std::vector<float> tmp;
+        if (m->precision() == kFP32) {
+            SpmeGpu<float>* e = m->spme32();
+            e->reserve_rhs(1);
+            e->spread(1, dX.get(), s);
+            tmp.resize(3 * K3);
+            SDT_HIP_CHECK(hipMemcpyAsync(tmp.data(), e->d_real.get(), sizeof(float) * 3 * K3,
+                                         hipMemcpyDeviceToHost, s));
+            SDT_HIP_CHECK(hipStreamSynchronize(s));
+            for (size_t i = 0; i < 3 * K3; i++) mesh[i] = tmp[i];
+        } else {
+            SpmeGpu<double>* e = m->spme64();
+            e->reserve_rhs(1);
+            e->spread(1, dX.get(), s);
+            SDT_HIP_CHECK(hipMemcpyAsync(mesh, e->d_real.get(), sizeof(double) * 3 * K3,
+                                         hipMemcpyDeviceToHost, s));
+            SDT_HIP_CHECK(hipStreamSynchronize(s));
+        }
+    });
+}
+
+}  // extern "C"
+
+template <typename T>
+static void stage_times(MobSpmeGpu* m, SpmeGpu<T>* e, int b, int reps, const double* X, double* Y,
+                        double* t) {
+    hipStream_t s = m->stream();
+    e->reserve_rhs(b);
+    hipEvent_t ev[8];
+    for (int i = 0; i < 8; i++) SDT_HIP_CHECK(hipEventCreate(&ev[i]));
+    for (int i = 0; i < 7; i++) t[i] = 0.0;
+    for (int r = 0; r < reps + 1; r++) {
+        SDT_HIP_CHECK(hipEventRecord(ev[0], s));
+        e->spread(b, X, s);
+        SDT_HIP_CHECK(hipEventRecord(ev[1], s));
+        e->fft_forward(b, s);
+        SDT_HIP_CHECK(hipEventRecord(ev[2], s));
+        e->influence(b, s);
+        SDT_HIP_CHECK(hipEventRecord(ev[3], s));
+        e->fft_backward(b, s);
+        SDT_HIP_CHECK(hipEventRecord(ev[4], s));
+        e->interp(b, 1.0, 0.0, Y, s);
+        SDT_HIP_CHECK(hipEventRecord(ev[5], s));
+        m->apply_packed(1, b, 1.0, X, 1.0, Y);
+        SDT_HIP_CHECK(hipEventRecord(ev[6], s));
+        m->apply_packed(0, b, 1.0, X, 0.0, Y);
+        SDT_HIP_CHECK(hipEventRecord(ev[7], s));
+        SDT_HIP_CHECK(hipEventSynchronize(ev[7]));
+        if (r == 0) continue;   // warm-up
+        for (int i = 0; i < 7; i++) {
+            float ms = 0.f;
+            SDT_HIP_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            t[i] += ms / reps;
+        }
+    }
+    for (int i = 0; i < 8; i++) (void)hipEventDestroy(ev[i]);
+}
+
+extern "C" {
+
+int sdt_bench_stage_times(sdt_mob_t mob, int b, int reps, double* t) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        MobSpmeGpu* m = mob->m;
+        SDT_REQUIRE(m->updated(), SDT_ESTATE, "mobility not updated");
+        SDT_REQUIRE(b >= 1 && b <= 16 && reps >= 1 && t, SDT_EINVAL, "bad arguments");
+        const size_t n = (size_t)m->dim() * b;
+        DevBuf<double> dX, dY;
+        dX.reserve(n);
+        dY.reserve(n);
+        gaussian_fill_device(99, 7, (long long)n, dX.get(), m->stream());
+        fill((long long)n, 0.0, dY.get(), m->stream());
+        if (m->precision() == kFP32) stage_times(m, m->spme32(), b, reps, dX.get(), dY.get(), t);
+        else stage_times(m, m->spme64(), b, reps, dX.get(), dY.get(), t);
+    });
+}
+
+int sdt_bench_bd_steps(sdt_bd_t bd, int nsteps, double* ms_total) {
+    SDT_CHECK_HANDLE(bd);
+    return guard([&] {
+        hipStream_t s = bd->bd->mob()->stream();
+        hipEvent_t a, b;
+        SDT_HIP_CHECK(hipEventCreate(&a));
+        SDT_HIP_CHECK(hipEventCreate(&b));
+        SDT_HIP_CHECK(hipEventRecord(a, s));
+        bd->bd->advance(nsteps);
+        SDT_HIP_CHECK(hipEventRecord(b, s));
+        SDT_HIP_CHECK(hipEventSynchronize(b));
+        float ms = 0.f;
+        SDT_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+        if (ms_total) *ms_total = ms;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    });
+}
+
+}  // extern "C"

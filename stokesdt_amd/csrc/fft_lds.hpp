@@ -1,0 +1,134 @@
+// Device-side radix-2/3/4/5 Stockham FFT on pencils held in LDS.
+//
+// Self-sorting decimation-in-frequency form: a stage of radix r on a
+// sub-sequence length `len` with stride `s` (m = len / r) reads
+// src[q + s*(p + j*m)], j < r, applies an r-point DFT and the twiddle
+// w_len^(p*t), and writes dst[q + s*(r*p + t)]. After all stages the result
+// is in natural order. Twiddles come from one table of K roots of unity
+// w_K^j (w_len^(p t) = w_K^((K/len) p t mod K)).
+#pragma once
+#include "sdt_internal.hpp"
+
+namespace sdt {
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return cplx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
+    return cplx<T>{a.x + b.x, a.y + b.y};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
+    return cplx<T>{a.x - b.x, a.y - b.y};
+}
+// multiply by -i*sign... : rot(a) = a * (sign < 0 ? -i : +i)
+template <typename T, int SIGN>
+__device__ __forceinline__ cplx<T> rot90(cplx<T> a) {
+    return SIGN < 0 ? cplx<T>{a.y, -a.x} : cplx<T>{-a.y, a.x};
+}
+
+// r-point DFT in registers, kernel e^{SIGN * 2 pi i j t / r}
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void small_dft(cplx<T>* a) {
+    if constexpr (R == 2) {
+        cplx<T> u = a[0], v = a[1];
+        a[0] = cadd(u, v);
+        a[1] = csub(u, v);
+    } else if constexpr (R == 4) {
+        cplx<T> s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+        cplx<T> s13 = cadd(a[1], a[3]), d13 = rot90<T, SIGN>(csub(a[1], a[3]));
+        a[0] = cadd(s02, s13);
+        a[1] = cadd(d02, d13);
+        a[2] = csub(s02, s13);
+        a[3] = csub(d02, d13);
+    } else if constexpr (R == 3) {
+        const T c1 = T(-0.5);
+        const T s1 = T(SIGN) * T(0.86602540378443864676372317075294);
+        cplx<T> t1 = cadd(a[1], a[2]);
+        cplx<T> t2 = csub(a[1], a[2]);
+        cplx<T> m = cplx<T>{a[0].x + c1 * t1.x, a[0].y + c1 * t1.y};
+        // s1 * i * t2
+        cplx<T> j = cplx<T>{-s1 * t2.y, s1 * t2.x};
+        a[0] = cadd(a[0], t1);
+        a[1] = cadd(m, j);
+        a[2] = csub(m, j);
+    } else if constexpr (R == 5) {
+        const T c1 = T(0.30901699437494742410229341718282);    // cos(2pi/5)
+        const T c2 = T(-0.80901699437494742410229341718282);   // cos(4pi/5)
+        const T s1 = T(SIGN) * T(0.95105651629515357211643933337938);  // sin(2pi/5)
+        const T s2 = T(SIGN) * T(0.58778525229247312916870595463907);  // sin(4pi/5)
+        cplx<T> t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+        cplx<T> d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+        cplx<T> m1 = cplx<T>{a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y};
+        cplx<T> m2 = cplx<T>{a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y};
+        // i * (s1 d1 + s2 d2) and i * (s2 d1 - s1 d2)
+        cplx<T> n1 = cplx<T>{-(s1 * d1.y + s2 * d2.y), s1 * d1.x + s2 * d2.x};
+        cplx<T> n2 = cplx<T>{-(s2 * d1.y - s1 * d2.y), s2 * d1.x - s1 * d2.x};
+        a[0] = cplx<T>{a[0].x + t1.x + t2.x, a[0].y + t1.y + t2.y};
+        a[1] = cadd(m1, n1);
+        a[4] = csub(m1, n1);
+        a[2] = cadd(m2, n2);
+        a[3] = csub(m2, n2);
+    }
+}
+
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void fft_stage(const cplx<T>* __restrict__ src,
+                                          cplx<T>* __restrict__ dst, int K, int npen,
+                                          int len, int s, const cplx<T>* __restrict__ tw) {
+    const int m = len / R;
+    const int per = K / R;                 // butterflies per pencil
+    const int total = npen * per;
+    const int tstep = K / len;
+    for (int item = threadIdx.x; item < total; item += blockDim.x) {
+        const int pen = item / per;
+        const int j = item - pen * per;
+        const int p = j / s;
+        const int q = j - p * s;
+        const cplx<T>* sp = src + pen * K + q + s * p;
+        cplx<T>* dp = dst + pen * K + q + s * R * p;
+        cplx<T> a[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) a[t] = sp[s * t * m];
+        small_dft<T, R, SIGN>(a);
+        dp[0] = a[0];
+#pragma unroll
+        for (int t = 1; t < R; t++) {
+            int e = (tstep * p * t) % K;
+            cplx<T> w = tw[e];
+            if (SIGN > 0) w.y = -w.y;
+            dp[s * t] = cmul(a[t], w);
+        }
+    }
+}
+
+// Runs all stages; returns the buffer holding the result (a or b).
+template <typename T, int SIGN>
+__device__ cplx<T>* fft_pencils(cplx<T>* a, cplx<T>* b, int K, int npen, const int* radix,
+                                int nst, const cplx<T>* tw) {
+    int len = K, s = 1;
+    for (int st = 0; st < nst; st++) {
+        const int r = radix[st];
+        if (r == 4) fft_stage<T, 4, SIGN>(a, b, K, npen, len, s, tw);
+        else if (r == 2) fft_stage<T, 2, SIGN>(a, b, K, npen, len, s, tw);
+        else if (r == 3) fft_stage<T, 3, SIGN>(a, b, K, npen, len, s, tw);
+        else fft_stage<T, 5, SIGN>(a, b, K, npen, len, s, tw);
+        __syncthreads();
+        len /= r;
+        s *= r;
+        cplx<T>* t = a;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+struct FFTArgs {
+    int K;
+    int nst;
+    int radix[16];
+};
+
+}  // namespace sdt

@@ -1,0 +1,509 @@
+// Host orchestration of the device mobility, Krylov displacements, steric
+// force and BD step.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include "mob_gpu.hpp"
+
+namespace sdt {
+
+// ------------------------------------------------------------ helpers ------
+__global__ void gather_sorted_kernel(int npos, const int* __restrict__ perm,
+                                     const double* __restrict__ pos, const double* __restrict__ rdi,
+                                     double* __restrict__ pos_s, double* __restrict__ rdi_s) {
+    const int is = blockIdx.x * blockDim.x + threadIdx.x;
+    if (is >= npos) return;
+    const int i = perm[is];
+    pos_s[3 * is] = pos[3 * i];
+    pos_s[3 * is + 1] = pos[3 * i + 1];
+    pos_s[3 * is + 2] = pos[3 * i + 2];
+    rdi_s[is] = rdi[i];
+}
+
+static size_t span(int rows, int ncols, int ld) {
+    return (size_t)ld * (ncols - 1) + rows;
+}
+
+// ------------------------------------------------------------ MobSpme ------
+MobSpmeGpu::MobSpmeGpu(int npos, const double* rdi, double box, double xi, double rmax, int dim,
+                       int porder, int precision)
+    : npos_(npos), K_(dim), porder_(porder), prec_(precision), box_(box), xi_(xi), rmax_(rmax) {
+    SDT_REQUIRE(npos > 0 && rdi != nullptr, SDT_EINVAL, "npos must be positive and rdi non-null");
+    rdi_.assign(rdi, rdi + npos);
+    SDT_REQUIRE(precision == kFP64 || precision == kFP32, SDT_EINVAL, "precision must be 0 or 1");
+    SDT_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+MobSpmeGpu::~MobSpmeGpu() {
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void MobSpmeGpu::init() {
+    // the checks of src/mob/mob_spme.cc:55-93
+    SDT_REQUIRE(box_ > 0.0, SDT_EINVAL, "box size must be positive");
+    SDT_REQUIRE(rmax_ > 0.0, SDT_EINVAL, "real-space cutoff must be positive");
+    SDT_REQUIRE(xi_ > 0.0, SDT_EINVAL, "Ewald parameter must be positive");
+    SDT_REQUIRE(K_ > 0, SDT_EINVAL, "mesh dimension must be positive");
+    SDT_REQUIRE(porder_ > 1, SDT_EINVAL, "interpolation order must be greater than 1");
+    SDT_REQUIRE(2.0 * rmax_ < box_, SDT_EINVAL, "rmax must be less than box_size/2");
+    SDT_REQUIRE(4 * porder_ <= K_, SDT_EINVAL, "porder must be smaller than dim/4");
+    const double mx = *std::max_element(rdi_.begin(), rdi_.end());
+    const double mn = *std::min_element(rdi_.begin(), rdi_.end());
+    SDT_REQUIRE(mn > 0.0, SDT_EINVAL, "particle radii must be positive");
+    // mono path only when every radius is exactly 1 (src/mob/mob_spme.cc:106)
+    mono_ = (mx == 1.0 && mn == 1.0);
+    if (prec_ == kFP32) {
+        spme32_.reset(new SpmeGpu<float>());
+        spme32_->create(npos_, box_, xi_, K_, porder_, !mono_);
+    } else {
+        spme64_.reset(new SpmeGpu<double>());
+        spme64_->create(npos_, box_, xi_, K_, porder_, !mono_);
+    }
+    SDT_REQUIRE(pairs_.init(npos_, rdi_.data(), box_, rmax_, 'R') > 0, SDT_EINVAL,
+                "failed to initialise the real-space pair list");
+    d_pos_.reserve(3 * (size_t)npos_);
+    d_rdi_.reserve(npos_);
+    d_pos_s_.reserve(3 * (size_t)npos_);
+    d_rdi_s_.reserve(npos_);
+    d_key_.reserve(npos_);
+    d_perm_.reserve(npos_);
+    d_iperm_.reserve(npos_);
+    const int nt = prec_ == kFP32 ? spme32_->ntile : spme64_->ntile;
+    d_tstart_.reserve((size_t)nt * nt * nt + 1);
+    inited_ = true;
+}
+
+void MobSpmeGpu::update(const double* pos, const double* rdi, bool dev) {
+    SDT_REQUIRE(inited_, SDT_ESTATE, "update called before init");
+    SDT_REQUIRE(pos != nullptr, SDT_EINVAL, "pos is null");
+    hipStream_t s = stream_;
+    const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    SDT_HIP_CHECK(hipMemcpyAsync(d_pos_.get(), pos, sizeof(double) * 3 * npos_, kind, s));
+    const double* r = rdi ? rdi : rdi_.data();
+    const hipMemcpyKind rkind = (rdi && dev) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    SDT_HIP_CHECK(hipMemcpyAsync(d_rdi_.get(), r, sizeof(double) * npos_, rkind, s));
+    int nt;
+    if (prec_ == kFP32) {
+        spme32_->prepare(d_pos_.get(), d_key_.get(), s);
+        nt = spme32_->ntile;
+    } else {
+        spme64_->prepare(d_pos_.get(), d_key_.get(), s);
+        nt = spme64_->ntile;
+    }
+    bucket_sort(d_key_.get(), npos_, nt * nt * nt, d_perm_.get(), d_tstart_.get(), d_iperm_.get(),
+                sc_, s);
+    if (prec_ == kFP32)
+        spme32_->gather_sorted(d_perm_.get(), d_pos_.get(), d_rdi_.get(), d_tstart_.get(), s);
+    else
+        spme64_->gather_sorted(d_perm_.get(), d_pos_.get(), d_rdi_.get(), d_tstart_.get(), s);
+    hipLaunchKernelGGL(gather_sorted_kernel, dim3(ceil_div(npos_, 256)), dim3(256), 0, s, npos_,
+                       d_perm_.get(), d_pos_.get(), d_rdi_.get(), d_pos_s_.get(), d_rdi_s_.get());
+    SDT_HIP_CHECK(hipGetLastError());
+    // real-space pairs in sorted order: rows by perm, columns through iperm
+    real_nnz_ = pairs_.build(d_pos_.get(), d_perm_.get(), d_iperm_.get(), false, s);
+    updated_ = true;
+}
+
+void MobSpmeGpu::apply_packed(int mode, int b, double alpha, const double* X, double beta,
+                              double* Y) {
+    SDT_REQUIRE(updated_, SDT_ESTATE, "mulvector called before update");
+    SDT_REQUIRE(b >= 1 && b <= 16, SDT_EINVAL, "at most 16 right-hand sides per pass");
+    hipStream_t s = stream_;
+    const int* rp = pairs_.d_rowptr.get();
+    const int* ci = pairs_.d_colidx.get();
+    if (mode == 0 || mode == 2) {
+        if (prec_ == kFP32) spme32_->apply(b, alpha, X, beta, Y, s);
+        else spme64_->apply(b, alpha, X, beta, Y, s);
+    }
+    if (mode == 0 || mode == 1) {
+        const double rb = (mode == 0) ? 1.0 : beta;
+        if (prec_ == kFP32)
+            real_matvec<float>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha,
+                               X, rb, Y, s);
+        else
+            real_matvec<double>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha,
+                                X, rb, Y, s);
+    }
+}
+
+void MobSpmeGpu::mul_vector(int mode, int nrhs, double alpha, int ldf, const double* f,
+                            double beta, int ldv, double* v, bool dev) {
+    SDT_REQUIRE(updated_, SDT_ESTATE, "mulvector called before update");
+    SDT_REQUIRE(mode >= 0 && mode <= 2, SDT_EINVAL, "mode must be full, real or recip");
+    SDT_REQUIRE(nrhs >= 1, SDT_EINVAL, "num_rhs must be positive");
+    SDT_REQUIRE(ldf >= 3 * npos_ && ldv >= 3 * npos_, SDT_EINVAL,
+                "leading dimensions must be at least 3*npos");
+    SDT_REQUIRE(f != nullptr && v != nullptr, SDT_EINVAL, "null vector");
+    hipStream_t s = stream_;
+    const int chunk = 16;
+    const int bmax = std::min(nrhs, chunk);
+    d_X_.reserve((size_t)3 * npos_ * bmax);
+    d_Y_.reserve((size_t)3 * npos_ * bmax);
+    const double* df = f;
+    double* dv = v;
+    if (!dev) {
+        d_io_f_.reserve(span(3 * npos_, nrhs, ldf));
+        d_io_v_.reserve(span(3 * npos_, nrhs, ldv));
+        SDT_HIP_CHECK(hipMemcpyAsync(d_io_f_.get(), f, sizeof(double) * span(3 * npos_, nrhs, ldf),
+                                     hipMemcpyHostToDevice, s));
+        if (beta != 0.0)
+            SDT_HIP_CHECK(hipMemcpyAsync(d_io_v_.get(), v,
+                                         sizeof(double) * span(3 * npos_, nrhs, ldv),
+                                         hipMemcpyHostToDevice, s));
+        df = d_io_f_.get();
+        dv = d_io_v_.get();
+    }
+    for (int r0 = 0; r0 < nrhs; r0 += chunk) {
+        const int b = std::min(chunk, nrhs - r0);
+        pack(npos_, b, d_perm_.get(), ldf, df + (size_t)r0 * ldf, d_X_.get(), s);
+        if (beta != 0.0) pack(npos_, b, d_perm_.get(), ldv, dv + (size_t)r0 * ldv, d_Y_.get(), s);
+        apply_packed(mode, b, alpha, d_X_.get(), beta, d_Y_.get());
+        unpack(npos_, b, d_perm_.get(), ldv, d_Y_.get(), dv + (size_t)r0 * ldv, s);
+    }
+    if (!dev) {
+        // copy back only the columns (the gaps between columns are untouched)
+        SDT_HIP_CHECK(hipMemcpy2DAsync(v, sizeof(double) * ldv, d_io_v_.get(), sizeof(double) * ldv,
+                                       sizeof(double) * 3 * npos_, nrhs, hipMemcpyDeviceToHost, s));
+    }
+    SDT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------- Lanczos -------
+BrwnLanczosGpu::BrwnLanczosGpu(int dim, int max_iters, int max_nrhs, double tol)
+    : dim_(dim), max_iters_(max_iters), max_nrhs_(max_nrhs), tol_(tol) {}
+
+void BrwnLanczosGpu::init() {
+    // src/brwn/brwn_lanczos.cc:224-275
+    SDT_REQUIRE(dim_ > 0, SDT_EINVAL, "dimension must be positive");
+    SDT_REQUIRE(max_iters_ > 0, SDT_EINVAL, "max_iters must be positive");
+    SDT_REQUIRE(max_nrhs_ > 0, SDT_EINVAL, "max_nrhs must be positive");
+    SDT_REQUIRE(tol_ > 0.0, SDT_EINVAL, "tolerance must be positive");
+    if (max_nrhs_ > dim_ / 3) max_nrhs_ = dim_ / 3;
+    if (max_nrhs_ > 16) max_nrhs_ = 16;   // the device block width limit
+    d_V_.reserve((size_t)(max_iters_ + 1) * max_nrhs_ * dim_);
+    d_C_.reserve((size_t)max_nrhs_ * dim_);
+    d_G_.reserve(16 * 16);
+    d_S_.reserve(8192);
+    inited_ = true;
+}
+
+// CholQR2 of the packed n x b block W in place; R (upper, row-major) with
+// W_in = W_out R.
+static bool cholqr2(long long n, int b, double* W, double* tmp, double* R, DevBuf<double>& dG,
+                    DevBuf<double>& dS, DevBuf<double>& part, hipStream_t s) {
+    std::vector<double> G(b * b), R1(b * b), Ri(b * b), R2(b * b);
+    for (int pass = 0; pass < 2; pass++) {
+        gram(n, b, W, W, dG.get(), part, s);
+        SDT_HIP_CHECK(hipMemcpyAsync(G.data(), dG.get(), sizeof(double) * b * b,
+                                     hipMemcpyDeviceToHost, s));
+        SDT_HIP_CHECK(hipStreamSynchronize(s));
+        // symmetrise
+        for (int i = 0; i < b; i++)
+            for (int j = i + 1; j < b; j++) {
+                double a = 0.5 * (G[i * b + j] + G[j * b + i]);
+                G[i * b + j] = G[j * b + i] = a;
+            }
+        double* Rp = pass == 0 ? R1.data() : R2.data();
+        if (!chol_upper(b, G.data(), Rp)) return false;
+        inv_upper(b, Rp, Ri.data());
+        SDT_HIP_CHECK(hipMemcpyAsync(dS.get(), Ri.data(), sizeof(double) * b * b,
+                                     hipMemcpyHostToDevice, s));
+        tsmm(n, b, 1, W, 0, dS.get(), 1.0, 0.0, tmp, s);
+        SDT_HIP_CHECK(hipMemcpyAsync(W, tmp, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
+    }
+    mul_upper(b, R2.data(), R1.data(), R);
+    return true;
+}
+
+int BrwnLanczosGpu::block_lanczos_packed(MobSpmeGpu* mob, int b, const double* Z, double* Y) {
+    const long long n = dim_;
+    hipStream_t s = mob->stream();
+    const size_t blk = (size_t)n * b;
+    double* V = d_V_.get();
+    const double normz = sqrt(sumsq(n * b, Z, d_part_, s));
+    SDT_HIP_CHECK(hipMemcpyAsync(V, Z, sizeof(double) * blk, hipMemcpyDeviceToDevice, s));
+    std::vector<double> R(b * b), H(b * b), B(b * b), Bt(b * b);
+    if (!cholqr2(n, b, V, d_C_.get(), R.data(), d_G_, d_S_, d_part_, s))
+        throw Error{SDT_ECONVERGE, "block Lanczos: the right-hand sides are rank deficient"};
+    const int mmax = (max_iters_ + 1) * b;
+    std::vector<double> T((size_t)mmax * mmax, 0.0), Tm, Sm, D, Dold;
+    int iter;
+    bool converged = false;
+    for (iter = 0; iter < max_iters_; iter++) {
+        double* Vi = V + (size_t)iter * blk;
+        double* W = V + (size_t)(iter + 1) * blk;
+        double beta = 0.0;
+        if (iter > 0) {
+            // W = V_{i-1} B_{i-1}^T
+            for (int i = 0; i < b; i++)
+                for (int j = 0; j < b; j++) Bt[i * b + j] = B[j * b + i];
+            SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), Bt.data(), sizeof(double) * b * b,
+                                         hipMemcpyHostToDevice, s));
+            tsmm(n, b, 1, V + (size_t)(iter - 1) * blk, 0, d_S_.get(), 1.0, 0.0, W, s);
+            beta = -1.0;
+        }
+        // W = M V_i - V_{i-1} B^T
+        mob->apply_packed(0, b, 1.0, Vi, beta, W);
+        // H = V_i^T W ; W -= V_i H
+        gram(n, b, Vi, W, d_G_.get(), d_part_, s);
+        SDT_HIP_CHECK(hipMemcpyAsync(H.data(), d_G_.get(), sizeof(double) * b * b,
+                                     hipMemcpyDeviceToHost, s));
+        SDT_HIP_CHECK(hipStreamSynchronize(s));
+        SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), H.data(), sizeof(double) * b * b,
+                                     hipMemcpyHostToDevice, s));
+        tsmm(n, b, 1, Vi, 0, d_S_.get(), -1.0, 1.0, W, s);
+        if (reorth) {
+            // full reorthogonalisation against V_0..V_i (classical Gram-Schmidt)
+            std::vector<double> C((size_t)(iter + 1) * b * b);
+            for (int k = 0; k <= iter; k++) {
+                gram(n, b, V + (size_t)k * blk, W, d_G_.get(), d_part_, s);
+                SDT_HIP_CHECK(hipMemcpyAsync(&C[(size_t)k * b * b], d_G_.get(),
+                                             sizeof(double) * b * b, hipMemcpyDeviceToHost, s));
+            }
+            SDT_HIP_CHECK(hipStreamSynchronize(s));
+            int k0 = 0;
+            while (k0 <= iter) {
+                int nk = std::min(iter + 1 - k0, 8192 / (b * b));
+                SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), &C[(size_t)k0 * b * b],
+                                             sizeof(double) * nk * b * b, hipMemcpyHostToDevice, s));
+                tsmm(n, b, nk, V + (size_t)k0 * blk, (long long)blk, d_S_.get(), -1.0, 1.0, W, s);
+                k0 += nk;
+            }
+        }
+        // [V_{i+1}, B_i] = QR(W)
+        if (!cholqr2(n, b, W, d_C_.get(), B.data(), d_G_, d_S_, d_part_, s)) {
+            // invariant subspace found: the current iterate is exact
+            converged = true;
+            if (iter == 0) {
+                // T = H only
+            }
+        }
+        const int o = iter * b;
+        for (int c = 0; c < b; c++)
+            for (int a = c; a < b; a++) {
+                // H(a, c) for a >= c, column-major reading of the row-major
+                // Gram G[a][c] = (V_i^T W)[a][c]
+                const double hv = H[a * b + c];
+                T[(size_t)(o + a) * mmax + o + c] = hv;
+                T[(size_t)(o + c) * mmax + o + a] = hv;
+            }
+        if (!converged) {
+            for (int a = 0; a < b; a++)
+                for (int c = a; c < b; c++) {
+                    const double bv = B[a * b + c];   // B(a, c), upper
+                    T[(size_t)(o + b + a) * mmax + o + c] = bv;
+                    T[(size_t)(o + c) * mmax + o + b + a] = bv;
+                }
+        }
+        const int m = (iter + 1) * b;
+        Tm.resize((size_t)m * m);
+        for (int i = 0; i < m; i++)
+            memcpy(&Tm[(size_t)i * m], &T[(size_t)i * mmax], sizeof(double) * m);
+        Sm.resize((size_t)m * b);
+        bsq_.compute(m, b, Tm.data(), b, Sm.data());
+        // D = S R
+        D.assign((size_t)m * b, 0.0);
+        for (int i = 0; i < m; i++)
+            for (int c = 0; c < b; c++) {
+                double acc = 0.0;
+                for (int k = 0; k <= c; k++) acc += Sm[(size_t)i * b + k] * R[k * b + c];
+                D[(size_t)i * b + c] = acc;
+            }
+        if (converged) break;
+        if (iter > 0) {
+            // ||Y_i - Y_{i-1}||_F = ||D_i - [D_{i-1}; 0]||_F for orthonormal V
+            double dn = 0.0;
+            for (size_t t = 0; t < D.size(); t++) {
+                double prev = t < Dold.size() ? Dold[t] : 0.0;
+                double d = D[t] - prev;
+                dn += d * d;
+            }
+            if (sqrt(dn) / normz < tol_) {
+                converged = true;
+                break;
+            }
+        }
+        Dold = D;
+    }
+    const int used = (iter < max_iters_) ? iter + 1 : max_iters_;
+    // Y = [V_0 .. V_{used-1}] D, in chunks that fit the LDS coefficient tile
+    const int per = std::max(1, 8192 / (b * b));
+    for (int k0 = 0; k0 < used; k0 += per) {
+        const int nk = std::min(per, used - k0);
+        SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), &D[(size_t)k0 * b * b], sizeof(double) * nk * b * b,
+                                     hipMemcpyHostToDevice, s));
+        tsmm(n, b, nk, V + (size_t)k0 * blk, (long long)blk, d_S_.get(), 1.0, k0 == 0 ? 0.0 : 1.0,
+             Y, s);
+    }
+    SDT_HIP_CHECK(hipStreamSynchronize(s));
+    return converged ? iter : -1;
+}
+
+void BrwnLanczosGpu::compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double* z, int ldy,
+                             double* y, bool dev) {
+    SDT_REQUIRE(inited_, SDT_ESTATE, "compute called before init");
+    SDT_REQUIRE(mob && mob->updated(), SDT_ESTATE, "mobility not updated");
+    SDT_REQUIRE(mob->dim() == dim_, SDT_EINVAL, "mobility dimension mismatch");
+    SDT_REQUIRE(num_rhs >= 1 && ldz >= dim_ && ldy >= dim_, SDT_EINVAL, "bad shapes");
+    hipStream_t s = mob->stream();
+    const int npos = dim_ / 3;
+    d_Zp_.reserve((size_t)dim_ * max_nrhs_);
+    d_Yp_.reserve((size_t)dim_ * max_nrhs_);
+    const double* dz = z;
+    double* dy = y;
+    if (!dev) {
+        d_io_z_.reserve(span(dim_, num_rhs, ldz));
+        d_io_y_.reserve(span(dim_, num_rhs, ldy));
+        SDT_HIP_CHECK(hipMemcpyAsync(d_io_z_.get(), z, sizeof(double) * span(dim_, num_rhs, ldz),
+                                     hipMemcpyHostToDevice, s));
+        dz = d_io_z_.get();
+        dy = d_io_y_.get();
+    }
+    iters_.clear();
+    for (int irhs = 0; irhs < num_rhs; irhs += max_nrhs_) {
+        const int b = std::min(max_nrhs_, num_rhs - irhs);
+        pack(npos, b, mob->perm(), ldz, dz + (size_t)irhs * ldz, d_Zp_.get(), s);
+        int it = block_lanczos_packed(mob, b, d_Zp_.get(), d_Yp_.get());
+        unpack(npos, b, mob->perm(), ldy, d_Yp_.get(), dy + (size_t)irhs * ldy, s);
+        iters_.push_back(it);
+    }
+    if (!dev)
+        SDT_HIP_CHECK(hipMemcpy2DAsync(y, sizeof(double) * ldy, d_io_y_.get(), sizeof(double) * ldy,
+                                       sizeof(double) * dim_, num_rhs, hipMemcpyDeviceToHost, s));
+    SDT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------- steric ------
+StericGpu::StericGpu(int npos, const double* rdi, double box, double r0, double k0)
+    : npos_(npos), box_(box), r0_(r0), k0_(k0) {
+    SDT_REQUIRE(npos > 0 && rdi != nullptr, SDT_EINVAL, "npos must be positive and rdi non-null");
+    rdi_.assign(rdi, rdi + npos);
+}
+
+void StericGpu::init() {
+    // src/force/force_steric.cc:77-96
+    SDT_REQUIRE(k0_ > 0.0, SDT_EINVAL, "the steric coefficient must be positive");
+    SDT_REQUIRE(pairs_.init(npos_, rdi_.data(), box_, r0_, 'S') > 0, SDT_EINVAL,
+                "failed to initialise the steric pair list");
+    d_pos_.reserve(3 * (size_t)npos_);
+    d_f_.reserve(3 * (size_t)npos_);
+    inited_ = true;
+}
+
+void StericGpu::accumulate(const double* pos, double* f, bool dev, hipStream_t s) {
+    SDT_REQUIRE(inited_, SDT_ESTATE, "steric force used before init");
+    const double* dp = pos;
+    double* df = f;
+    if (!dev) {
+        SDT_HIP_CHECK(hipMemcpyAsync(d_pos_.get(), pos, sizeof(double) * 3 * npos_,
+                                     hipMemcpyHostToDevice, s));
+        SDT_HIP_CHECK(hipMemcpyAsync(d_f_.get(), f, sizeof(double) * 3 * npos_,
+                                     hipMemcpyHostToDevice, s));
+        dp = d_pos_.get();
+        df = d_f_.get();
+    }
+    pairs_.build(dp, nullptr, nullptr, false, s);
+    steric_accumulate(npos_, dp, pairs_.d_rdi.get(), box_, r0_, k0_, pairs_.d_rowptr.get(),
+                      pairs_.d_colidx.get(), df, s);
+    if (!dev) {
+        SDT_HIP_CHECK(hipMemcpyAsync(f, d_f_.get(), sizeof(double) * 3 * npos_,
+                                     hipMemcpyDeviceToHost, s));
+        SDT_HIP_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+// ----------------------------------------------------------------- RNG -----
+__device__ __forceinline__ void philox_round(unsigned& c0, unsigned& c1, unsigned& c2, unsigned& c3,
+                                             unsigned k0, unsigned k1) {
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c0;
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c2;
+    const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+    const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+    const unsigned n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+}
+
+__global__ void gaussian_kernel(unsigned long long seed, unsigned stream, long long n,
+                                double* __restrict__ out) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (2 * q >= n) return;
+    unsigned c0 = (unsigned)q, c1 = (unsigned)((unsigned long long)q >> 32), c2 = stream, c3 = 0u;
+    unsigned k0 = (unsigned)seed, k1 = (unsigned)(seed >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        philox_round(c0, c1, c2, c3, k0, k1);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    const unsigned long long a = (((unsigned long long)c0 << 32) | c1) >> 11;
+    const unsigned long long b = (((unsigned long long)c2 << 32) | c3) >> 11;
+    const double inv53 = 1.0 / 9007199254740992.0;
+    const double u1 = ((double)a + 0.5) * inv53;
+    const double u2 = ((double)b + 0.5) * inv53;
+    const double r = sqrt(-2.0 * log(u1));
+    const double t = 2.0 * M_PI * u2;
+    out[2 * q] = r * cos(t);
+    if (2 * q + 1 < n) out[2 * q + 1] = r * sin(t);
+}
+
+void gaussian_fill_device(unsigned long long seed, unsigned stream, long long n, double* d_out,
+                          hipStream_t s) {
+    const long long npairs = (n + 1) / 2;
+    hipLaunchKernelGGL(gaussian_kernel, dim3(ceil_div(npairs, 256)), dim3(256), 0, s, seed, stream,
+                       n, d_out);
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
+// ----------------------------------------------------------------- BD ------
+BDGpu::BDGpu(int npos, const double* pos, const double* rdi, double box, double xi, double rmax,
+             int dim, int porder, int precision, int max_iters, int max_nrhs, double tol,
+             double r0, double k0, double dt, int mob_interval, unsigned long long seed)
+    : npos_(npos), box_(box), dt_(dt), mob_interval_(mob_interval), seed_(seed) {
+    SDT_REQUIRE(npos > 0 && pos && rdi, SDT_EINVAL, "bad particle arrays");
+    SDT_REQUIRE(mob_interval >= 1, SDT_EINVAL, "mob_interval must be positive");
+    SDT_REQUIRE(dt > 0.0, SDT_EINVAL, "dt must be positive");
+    h_pos0_.assign(pos, pos + 3 * (size_t)npos);
+    rdi_.assign(rdi, rdi + npos);
+    mob_.reset(new MobSpmeGpu(npos, rdi, box, xi, rmax, dim, porder, precision));
+    brwn_.reset(new BrwnLanczosGpu(3 * npos, max_iters, max_nrhs, tol));
+    steric_.reset(new StericGpu(npos, rdi, box, r0, k0));
+}
+
+void BDGpu::init() {
+    mob_->init();
+    brwn_->init();
+    steric_->init();
+    ldm_ = 3 * npos_;
+    d_pos_.reserve((size_t)ldm_);
+    d_Z_.reserve((size_t)ldm_ * mob_interval_);
+    d_D_.reserve((size_t)ldm_ * mob_interval_);
+    d_f_.reserve((size_t)ldm_);
+    SDT_HIP_CHECK(hipMemcpy(d_pos_.get(), h_pos0_.data(), sizeof(double) * ldm_,
+                            hipMemcpyHostToDevice));
+}
+
+void BDGpu::advance(int nsteps) {
+    hipStream_t s = mob_->stream();
+    for (int k = 0; k < nsteps; k++) {
+        const int irhs = (int)(istep_ % mob_interval_);
+        if (irhs == 0) {
+            const long long blk = istep_ / mob_interval_;
+            gaussian_fill_device(seed_, (unsigned)blk, (long long)mob_interval_ * ldm_, d_Z_.get(), s);
+            mob_->update(d_pos_.get(), nullptr, true);
+            brwn_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
+        }
+        double* disp = d_D_.get() + (size_t)irhs * ldm_;
+        fill(ldm_, 0.0, d_f_.get(), s);
+        steric_->accumulate(d_pos_.get(), d_f_.get(), true, s);
+        mob_->mul_vector(0, 1, dt_, ldm_, d_f_.get(), sqrt(2.0 * dt_), ldm_, disp, true);
+        axpby(ldm_, 1.0, disp, 1.0, d_pos_.get(), s);
+        istep_++;
+    }
+    SDT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void BDGpu::get_positions(double* pos) {
+    SDT_HIP_CHECK(hipMemcpy(pos, d_pos_.get(), sizeof(double) * ldm_, hipMemcpyDeviceToHost));
+}
+
+}  // namespace sdt

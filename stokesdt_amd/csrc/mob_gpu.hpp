@@ -1,0 +1,137 @@
+// GPU mobility (MobSpme), block Lanczos (BrwnLanczos), steric force and the
+// BD step driver (BDSimBox::Advance) behind the C ABI.
+#pragma once
+#include <memory>
+#include "block_gpu.hpp"
+#include "host_linalg.hpp"
+#include "pair_gpu.hpp"
+#include "real_gpu.hpp"
+#include "sort_gpu.hpp"
+#include "spme_gpu.hpp"
+
+namespace sdt {
+
+enum Precision : int { kFP64 = 0, kFP32 = 1 };
+
+// MobSpme (src/mob/mob_spme.h:21-125) with device-resident state.
+class MobSpmeGpu {
+  public:
+    MobSpmeGpu(int npos, const double* rdi, double box, double xi, double rmax, int dim,
+               int porder, int precision);
+    ~MobSpmeGpu();
+    // validation and allocation (src/mob/mob_spme.cc:53-132)
+    void init();
+    // positions and radii: host (dev = false) or device pointers
+    void update(const double* pos, const double* rdi, bool dev);
+    // v := alpha M f + beta v, reference layout; mode 0 full, 1 real, 2 recip
+    void mul_vector(int mode, int nrhs, double alpha, int ldf, const double* f, double beta,
+                    int ldv, double* v, bool dev);
+    // packed, sorted-order FP64 vectors (row-major 3N x b)
+    void apply_packed(int mode, int b, double alpha, const double* X, double beta, double* Y);
+    int dim() const { return 3 * npos_; }
+    int npos() const { return npos_; }
+    hipStream_t stream() const { return stream_; }
+    const int* perm() const { return d_perm_.get(); }
+    bool updated() const { return updated_; }
+    int precision() const { return prec_; }
+    long long real_nnz() const { return real_nnz_; }
+    int K() const { return K_; }
+    int porder() const { return porder_; }
+    double box() const { return box_; }
+    // timing hooks for the benchmark: last recip/real split is not tracked
+    SpmeGpu<float>* spme32() { return spme32_.get(); }
+    SpmeGpu<double>* spme64() { return spme64_.get(); }
+
+  private:
+    int npos_, K_, porder_, prec_;
+    double box_, xi_, rmax_;
+    std::vector<double> rdi_;
+    bool mono_ = true, inited_ = false, updated_ = false;
+    hipStream_t stream_ = nullptr;
+    std::unique_ptr<SpmeGpu<float>> spme32_;
+    std::unique_ptr<SpmeGpu<double>> spme64_;
+    PairListGpu pairs_;
+    SortScratch sc_;
+    DevBuf<double> d_pos_, d_rdi_, d_pos_s_, d_rdi_s_;
+    DevBuf<int> d_key_, d_perm_, d_iperm_, d_tstart_;
+    DevBuf<double> d_io_f_, d_io_v_, d_X_, d_Y_;
+    long long real_nnz_ = 0;
+};
+
+// BrwnLanczos (src/brwn/brwn_lanczos.h:45-46) on a device mobility.
+class BrwnLanczosGpu {
+  public:
+    BrwnLanczosGpu(int dim, int max_iters, int max_nrhs, double tol);
+    void init();
+    // reference layout z/y (host or device); returns the iteration count of
+    // every group in iters()
+    void compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double* z, int ldy, double* y,
+                 bool dev);
+    // packed layout: Z, Y are row-major 3N x b in the mobility's sorted order
+    int block_lanczos_packed(MobSpmeGpu* mob, int b, const double* Z, double* Y);
+    const std::vector<int>& iters() const { return iters_; }
+    int max_nrhs() const { return max_nrhs_; }
+    int reorth = 0;        // 0: none (reference); 1: full FP64 reorthogonalisation
+
+  private:
+    int dim_, max_iters_, max_nrhs_;
+    double tol_;
+    bool inited_ = false;
+    std::vector<int> iters_;
+    DevBuf<double> d_V_, d_G_, d_S_, d_part_, d_io_z_, d_io_y_, d_Zp_, d_Yp_, d_C_;
+    BandSqrt bsq_;
+};
+
+// Steric force (src/force/force_steric.cc) on the S pair list.
+class StericGpu {
+  public:
+    StericGpu(int npos, const double* rdi, double box, double r0, double k0);
+    void init();
+    // f := 0 then accumulate (ForceBase::Compute); accumulate adds
+    void accumulate(const double* pos, double* f, bool dev, hipStream_t s);
+    long long last_nnz() const { return pairs_.nnz; }
+    int npos() const { return npos_; }
+
+  private:
+    int npos_;
+    double box_, r0_, k0_;
+    std::vector<double> rdi_;
+    PairListGpu pairs_;
+    DevBuf<double> d_pos_, d_f_;
+    bool inited_ = false;
+};
+
+// BDSimBox::Advance (src/simbox/simbox_bd.cc:296-366) with device-resident
+// positions, Gaussian block, displacements and forces.
+class BDGpu {
+  public:
+    BDGpu(int npos, const double* pos, const double* rdi, double box, double xi, double rmax,
+          int dim, int porder, int precision, int max_iters, int max_nrhs, double tol, double r0,
+          double k0, double dt, int mob_interval, unsigned long long seed);
+    void init();
+    void advance(int nsteps);
+    void get_positions(double* pos);
+    const std::vector<int>& last_iters() const { return brwn_->iters(); }
+    MobSpmeGpu* mob() { return mob_.get(); }
+    long long istep() const { return istep_; }
+
+  private:
+    int npos_;
+    double box_, dt_;
+    int mob_interval_;
+    unsigned long long seed_;
+    std::vector<double> h_pos0_, rdi_;
+    std::unique_ptr<MobSpmeGpu> mob_;
+    std::unique_ptr<BrwnLanczosGpu> brwn_;
+    std::unique_ptr<StericGpu> steric_;
+    DevBuf<double> d_pos_, d_Z_, d_D_, d_f_;
+    long long istep_ = 0;
+    int ldm_ = 0;
+};
+
+// Philox-4x32-10 Gaussian block on the device (same stream as the oracle):
+// element e of stream `stream` -> out[e], n elements.
+void gaussian_fill_device(unsigned long long seed, unsigned stream, long long n, double* d_out,
+                          hipStream_t s);
+
+}  // namespace sdt

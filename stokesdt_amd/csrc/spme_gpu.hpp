@@ -1,0 +1,60 @@
+// Reciprocal-space SPME engine on the GPU (host-side object).
+//
+// Mesh data in HBM:
+//   real meshes   [m][z][y][x]      K^3 reals per mesh (m = rhs*3+comp,
+//                                    plus 3*b more for the polydisperse a^2 f)
+//   half spectra  [m][z][y][kx]     K*K*(K/2+1) complex per mesh
+// Particles are kept sorted by the spread tile that holds their B-spline
+// base point; vectors inside the library use the packed layout
+//   X[(3*i_sorted + c) * b + r]     (row-major 3N x b, FP64)
+#pragma once
+#include "sdt_internal.hpp"
+
+namespace sdt {
+
+struct SortedSystem;   // particle order shared by SPME and real space
+
+template <typename T>
+struct SpmeGpu {
+    int npos = 0, K = 0, porder = 0;
+    double box = 0, xi = 0;
+    bool poly = false;
+    int ts = 8;            // spread/interp tile side (mesh points)
+    int ntile = 0;         // tiles per dimension
+    int bcap = 0;          // RHS capacity of the mesh buffers
+    FFTPlan plan;
+    DevBuf<double> d_tw;   // K roots of unity (cos, sin) of -2*pi*j/K
+    DevBuf<double> d_bsp2; // |b(m)|^2 per wavenumber index, K entries
+    DevBuf<T> d_frac;      // 3N fractional offsets (sorted order)
+    DevBuf<int> d_base;    // 3N B-spline base mesh indices (sorted order)
+    DevBuf<double> d_a2;   // N radii squared (sorted order), poly only
+    DevBuf<int> d_tile_start;  // ntile^3 + 1
+    DevBuf<T> d_real;      // nmesh * K^3
+    DevBuf<cplx<T>> d_spec;    // nmesh * K*K*Kh
+
+    void create(int npos, double box, double xi, int K, int porder, bool poly);
+    void reserve_rhs(int b);
+    // positions (original order, FP64 device) -> particle tiles; returns the
+    // sort key of every particle in key_out (original order)
+    void prepare(const double* d_pos, int* d_key_out, hipStream_t s);
+    // after the shared sort: gather per-particle data into sorted order
+    void gather_sorted(const int* d_perm, const double* d_pos, const double* d_rdi,
+                       const int* d_tile_start_in, hipStream_t s);
+    // Y = alpha * M_recip X + beta * Y, packed FP64 vectors, b RHS
+    void apply(int b, double alpha, const double* X, double beta, double* Y,
+               hipStream_t s);
+
+    // stages (exposed for tests/benchmarks)
+    void spread(int b, const double* X, hipStream_t s);
+    void fft_forward(int b, hipStream_t s);
+    void influence(int b, hipStream_t s);
+    void fft_backward(int b, hipStream_t s);
+    void interp(int b, double alpha, double beta, double* Y, hipStream_t s);
+    int nmesh(int b) const { return (poly ? 6 : 3) * b; }
+
+    // scratch from prepare()
+    DevBuf<T> d_frac_u;
+    DevBuf<int> d_base_u;
+};
+
+}  // namespace sdt
