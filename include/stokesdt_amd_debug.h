@@ -33,6 +33,20 @@ int sdt_bench_stage_times(sdt_mob_t mob, int b, int reps, double* t);
  * bracketed by events on the handle's stream. */
 int sdt_bench_bd_steps(sdt_bd_t bd, int nsteps, double* ms_total);
 
+/* Per-stage device time of the BD driver, recorded with HIP events on the
+ * handle's stream around every launch group while profiling is on.
+ * Stages: 0 spread, 1 forward x/y FFT, 2 z FFT + influence + inverse z,
+ * 3 inverse y/x FFT, 4 interp, 5 real space, 6 Krylov block products,
+ * 7 steric (pair list + force), 8 mobility update. sdt_bd_set_profiling(1)
+ * clears the sums; sdt_bd_profile returns sums (ms) and counts. */
+int sdt_bd_set_profiling(sdt_bd_t bd, int on);
+int sdt_bd_profile(sdt_bd_t bd, double* sum_ms, int64_t* counts, int nstages);
+/* Random sequential addition of n non-overlapping spheres of radius a in a
+ * periodic box of side L (seeded); returns SDT_ECAP if max_attempts ran out
+ * (then *placed < n). Synthetic-input generator for benchmarks. */
+int sdt_util_rsa_positions(int n, double box, double a, uint64_t seed, int64_t max_attempts,
+                           double* pos, int64_t* placed);
+
 #ifdef __cplusplus
 }
 #endif

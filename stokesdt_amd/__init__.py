@@ -84,7 +84,13 @@ SIGNATURES = {
     "sdt_debug_spme_spread": (_I, [_P, _P, _P]),
     "sdt_bench_stage_times": (_I, [_P, _I, _I, _P]),
     "sdt_bench_bd_steps": (_I, [_P, _I, ctypes.POINTER(_D)]),
+    "sdt_bd_set_profiling": (_I, [_P, _I]),
+    "sdt_bd_profile": (_I, [_P, _P, _P, _I]),
+    "sdt_util_rsa_positions": (_I, [_I, _D, _D, ctypes.c_uint64, _I64, _P, _PI64]),
 }
+
+STAGES = ["spread", "fft_xy_fwd", "fft_z_influence", "fft_xy_inv", "interp", "real_space",
+          "krylov_block", "steric", "mob_update"]
 
 
 class SdtError(RuntimeError):
@@ -390,6 +396,16 @@ class BDSim:
         _check(lib().sdt_bd_last_iters(self.h, _ptr(out), 4096, ctypes.byref(ng)))
         return out[:ng.value].copy()
 
+    def set_profiling(self, on):
+        _check(lib().sdt_bd_set_profiling(self.h, 1 if on else 0))
+
+    def profile(self):
+        """{stage: (total ms, launches)} recorded since profiling was enabled."""
+        ms = np.zeros(len(STAGES))
+        cnt = np.zeros(len(STAGES), dtype=np.int64)
+        _check(lib().sdt_bd_profile(self.h, _ptr(ms), _ptr(cnt), len(STAGES)))
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(STAGES)}
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             lib().sdt_bd_destroy(self.h)
@@ -418,3 +434,14 @@ def band_sqrt(T, kd, kb):
     S = np.zeros((m, kb))
     _check(lib().sdt_debug_band_sqrt(m, kd, _ptr(T), kb, _ptr(S)))
     return S
+
+
+def rsa_positions(n, box, a=1.0, seed=0, max_attempts=None):
+    """Seeded random sequential addition of n non-overlapping spheres."""
+    pos = np.zeros(3 * n)
+    placed = ctypes.c_int64(0)
+    if max_attempts is None:
+        max_attempts = 2000 * n
+    _check(lib().sdt_util_rsa_positions(n, box, a, seed, max_attempts, _ptr(pos),
+                                        ctypes.byref(placed)))
+    return pos

@@ -21,7 +21,7 @@ HIP_SOURCES = [
     ("mob_gpu.hip", []),
     ("capi.hip", []),
 ]
-HOST_SOURCES = [("host_linalg.cpp", [])]
+HOST_SOURCES = [("host_linalg.cpp", []), ("host_util.cpp", [])]
 
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",

@@ -7,6 +7,8 @@
 #include "mob_gpu.hpp"
 
 namespace sdt {
+long long rsa_positions(int n, double box, double a, uint64_t seed, long long max_attempts,
+                        double* pos);
 static thread_local std::string g_err;
 void set_error(const std::string& m) { g_err = m; }
 const char* last_error() { return g_err.c_str(); }
@@ -526,6 +528,33 @@ int sdt_bench_bd_steps(sdt_bd_t bd, int nsteps, double* ms_total) {
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
     });
+}
+
+int sdt_bd_set_profiling(sdt_bd_t bd, int on) {
+    SDT_CHECK_HANDLE(bd);
+    StageTimer& t = bd->bd->timer();
+    t.on = on != 0;
+    if (t.on) t.reset();
+    return SDT_OK;
+}
+
+int sdt_bd_profile(sdt_bd_t bd, double* sum_ms, int64_t* counts, int nstages) {
+    SDT_CHECK_HANDLE(bd);
+    StageTimer& t = bd->bd->timer();
+    for (int i = 0; i < nstages && i < StageTimer::kStages; i++) {
+        if (sum_ms) sum_ms[i] = t.sum_ms[i];
+        if (counts) counts[i] = t.count[i];
+    }
+    return SDT_OK;
+}
+
+int sdt_util_rsa_positions(int n, double box, double a, uint64_t seed, int64_t max_attempts,
+                           double* pos, int64_t* placed) {
+    if (n <= 0 || box <= 0.0 || a <= 0.0 || !pos) { set_error("invalid arguments"); return SDT_EINVAL; }
+    long long k = rsa_positions(n, box, a, seed, max_attempts, pos);
+    if (placed) *placed = k;
+    if (k < n) { set_error("random sequential addition ran out of attempts"); return SDT_ECAP; }
+    return SDT_OK;
 }
 
 }  // extern "C"

@@ -73,10 +73,17 @@ void MobSpmeGpu::init() {
     inited_ = true;
 }
 
+void MobSpmeGpu::set_timer(StageTimer* t) {
+    timer_ = t;
+    if (spme32_) spme32_->timer = t;
+    if (spme64_) spme64_->timer = t;
+}
+
 void MobSpmeGpu::update(const double* pos, const double* rdi, bool dev) {
     SDT_REQUIRE(inited_, SDT_ESTATE, "update called before init");
     SDT_REQUIRE(pos != nullptr, SDT_EINVAL, "pos is null");
     hipStream_t s = stream_;
+    StageScope scope(timer_, 8, s);
     const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     SDT_HIP_CHECK(hipMemcpyAsync(d_pos_.get(), pos, sizeof(double) * 3 * npos_, kind, s));
     const double* r = rdi ? rdi : rdi_.data();
@@ -116,6 +123,7 @@ void MobSpmeGpu::apply_packed(int mode, int b, double alpha, const double* X, do
         else spme64_->apply(b, alpha, X, beta, Y, s);
     }
     if (mode == 0 || mode == 1) {
+        StageScope scope(timer_, 5, s);
         const double rb = (mode == 0) ? 1.0 : beta;
         if (prec_ == kFP32)
             real_matvec<float>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha,
@@ -190,10 +198,10 @@ void BrwnLanczosGpu::init() {
 // CholQR2 of the packed n x b block W in place; R (upper, row-major) with
 // W_in = W_out R.
 static bool cholqr2(long long n, int b, double* W, double* tmp, double* R, DevBuf<double>& dG,
-                    DevBuf<double>& dS, DevBuf<double>& part, hipStream_t s) {
+                    DevBuf<double>& dS, DevBuf<double>& part, hipStream_t s, StageTimer* timer) {
     std::vector<double> G(b * b), R1(b * b), Ri(b * b), R2(b * b);
     for (int pass = 0; pass < 2; pass++) {
-        gram(n, b, W, W, dG.get(), part, s);
+        { StageScope sc(timer, 6, s); gram(n, b, W, W, dG.get(), part, s); }
         SDT_HIP_CHECK(hipMemcpyAsync(G.data(), dG.get(), sizeof(double) * b * b,
                                      hipMemcpyDeviceToHost, s));
         SDT_HIP_CHECK(hipStreamSynchronize(s));
@@ -208,6 +216,7 @@ static bool cholqr2(long long n, int b, double* W, double* tmp, double* R, DevBu
         inv_upper(b, Rp, Ri.data());
         SDT_HIP_CHECK(hipMemcpyAsync(dS.get(), Ri.data(), sizeof(double) * b * b,
                                      hipMemcpyHostToDevice, s));
+        StageScope sc(timer, 6, s);
         tsmm(n, b, 1, W, 0, dS.get(), 1.0, 0.0, tmp, s);
         SDT_HIP_CHECK(hipMemcpyAsync(W, tmp, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
     }
@@ -223,7 +232,7 @@ int BrwnLanczosGpu::block_lanczos_packed(MobSpmeGpu* mob, int b, const double* Z
     const double normz = sqrt(sumsq(n * b, Z, d_part_, s));
     SDT_HIP_CHECK(hipMemcpyAsync(V, Z, sizeof(double) * blk, hipMemcpyDeviceToDevice, s));
     std::vector<double> R(b * b), H(b * b), B(b * b), Bt(b * b);
-    if (!cholqr2(n, b, V, d_C_.get(), R.data(), d_G_, d_S_, d_part_, s))
+    if (!cholqr2(n, b, V, d_C_.get(), R.data(), d_G_, d_S_, d_part_, s, timer))
         throw Error{SDT_ECONVERGE, "block Lanczos: the right-hand sides are rank deficient"};
     const int mmax = (max_iters_ + 1) * b;
     std::vector<double> T((size_t)mmax * mmax, 0.0), Tm, Sm, D, Dold;
@@ -239,19 +248,20 @@ int BrwnLanczosGpu::block_lanczos_packed(MobSpmeGpu* mob, int b, const double* Z
                 for (int j = 0; j < b; j++) Bt[i * b + j] = B[j * b + i];
             SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), Bt.data(), sizeof(double) * b * b,
                                          hipMemcpyHostToDevice, s));
+            StageScope sc(timer, 6, s);
             tsmm(n, b, 1, V + (size_t)(iter - 1) * blk, 0, d_S_.get(), 1.0, 0.0, W, s);
             beta = -1.0;
         }
         // W = M V_i - V_{i-1} B^T
         mob->apply_packed(0, b, 1.0, Vi, beta, W);
         // H = V_i^T W ; W -= V_i H
-        gram(n, b, Vi, W, d_G_.get(), d_part_, s);
+        { StageScope sc(timer, 6, s); gram(n, b, Vi, W, d_G_.get(), d_part_, s); }
         SDT_HIP_CHECK(hipMemcpyAsync(H.data(), d_G_.get(), sizeof(double) * b * b,
                                      hipMemcpyDeviceToHost, s));
         SDT_HIP_CHECK(hipStreamSynchronize(s));
         SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), H.data(), sizeof(double) * b * b,
                                      hipMemcpyHostToDevice, s));
-        tsmm(n, b, 1, Vi, 0, d_S_.get(), -1.0, 1.0, W, s);
+        { StageScope sc(timer, 6, s); tsmm(n, b, 1, Vi, 0, d_S_.get(), -1.0, 1.0, W, s); }
         if (reorth) {
             // full reorthogonalisation against V_0..V_i (classical Gram-Schmidt)
             std::vector<double> C((size_t)(iter + 1) * b * b);
@@ -271,7 +281,7 @@ int BrwnLanczosGpu::block_lanczos_packed(MobSpmeGpu* mob, int b, const double* Z
             }
         }
         // [V_{i+1}, B_i] = QR(W)
-        if (!cholqr2(n, b, W, d_C_.get(), B.data(), d_G_, d_S_, d_part_, s)) {
+        if (!cholqr2(n, b, W, d_C_.get(), B.data(), d_G_, d_S_, d_part_, s, timer)) {
             // invariant subspace found: the current iterate is exact
             converged = true;
             if (iter == 0) {
@@ -332,6 +342,7 @@ int BrwnLanczosGpu::block_lanczos_packed(MobSpmeGpu* mob, int b, const double* Z
         const int nk = std::min(per, used - k0);
         SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), &D[(size_t)k0 * b * b], sizeof(double) * nk * b * b,
                                      hipMemcpyHostToDevice, s));
+        StageScope sc(timer, 6, s);
         tsmm(n, b, nk, V + (size_t)k0 * blk, (long long)blk, d_S_.get(), 1.0, k0 == 0 ? 0.0 : 1.0,
              Y, s);
     }
@@ -402,6 +413,7 @@ void StericGpu::accumulate(const double* pos, double* f, bool dev, hipStream_t s
         dp = d_pos_.get();
         df = d_f_.get();
     }
+    StageScope scope(timer, 7, s);
     pairs_.build(dp, nullptr, nullptr, false, s);
     steric_accumulate(npos_, dp, pairs_.d_rdi.get(), box_, r0_, k0_, pairs_.d_rowptr.get(),
                       pairs_.d_colidx.get(), df, s);
@@ -473,6 +485,9 @@ void BDGpu::init() {
     mob_->init();
     brwn_->init();
     steric_->init();
+    mob_->set_timer(&timer_);
+    brwn_->timer = &timer_;
+    steric_->timer = &timer_;
     ldm_ = 3 * npos_;
     d_pos_.reserve((size_t)ldm_);
     d_Z_.reserve((size_t)ldm_ * mob_interval_);
@@ -500,6 +515,7 @@ void BDGpu::advance(int nsteps) {
         istep_++;
     }
     SDT_HIP_CHECK(hipStreamSynchronize(s));
+    timer_.collect();
 }
 
 void BDGpu::get_positions(double* pos) {

@@ -41,6 +41,8 @@ class MobSpmeGpu {
     // timing hooks for the benchmark: last recip/real split is not tracked
     SpmeGpu<float>* spme32() { return spme32_.get(); }
     SpmeGpu<double>* spme64() { return spme64_.get(); }
+    void set_timer(StageTimer* t);
+    StageTimer* timer() const { return timer_; }
 
   private:
     int npos_, K_, porder_, prec_;
@@ -56,6 +58,7 @@ class MobSpmeGpu {
     DevBuf<int> d_key_, d_perm_, d_iperm_, d_tstart_;
     DevBuf<double> d_io_f_, d_io_v_, d_X_, d_Y_;
     long long real_nnz_ = 0;
+    StageTimer* timer_ = nullptr;
 };
 
 // BrwnLanczos (src/brwn/brwn_lanczos.h:45-46) on a device mobility.
@@ -72,6 +75,7 @@ class BrwnLanczosGpu {
     const std::vector<int>& iters() const { return iters_; }
     int max_nrhs() const { return max_nrhs_; }
     int reorth = 0;        // 0: none (reference); 1: full FP64 reorthogonalisation
+    StageTimer* timer = nullptr;
 
   private:
     int dim_, max_iters_, max_nrhs_;
@@ -91,6 +95,7 @@ class StericGpu {
     void accumulate(const double* pos, double* f, bool dev, hipStream_t s);
     long long last_nnz() const { return pairs_.nnz; }
     int npos() const { return npos_; }
+    StageTimer* timer = nullptr;
 
   private:
     int npos_;
@@ -114,6 +119,7 @@ class BDGpu {
     const std::vector<int>& last_iters() const { return brwn_->iters(); }
     MobSpmeGpu* mob() { return mob_.get(); }
     long long istep() const { return istep_; }
+    StageTimer& timer() { return timer_; }
 
   private:
     int npos_;
@@ -127,6 +133,7 @@ class BDGpu {
     DevBuf<double> d_pos_, d_Z_, d_D_, d_f_;
     long long istep_ = 0;
     int ldm_ = 0;
+    StageTimer timer_;
 };
 
 // Philox-4x32-10 Gaussian block on the device (same stream as the oracle):

@@ -83,3 +83,74 @@ struct FFTPlan {
 bool make_fft_plan(int K, FFTPlan* plan);
 
 }  // namespace sdt
+
+namespace sdt {
+
+// Per-stage device timing with HIP events recorded on the launching stream.
+// Stages: 0 spread, 1 forward x/y FFT, 2 z FFT + influence, 3 inverse y/x
+// FFT, 4 interp, 5 real space, 6 Krylov block products, 7 steric (pair list
+// + force), 8 mobility update (sort + pair list).
+struct StageTimer {
+    static constexpr int kStages = 9;
+    bool on = false;
+    double sum_ms[kStages] = {0};
+    long long count[kStages] = {0};
+    struct Rec { int stage; hipEvent_t a, b; };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        SDT_HIP_CHECK(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t begin(hipStream_t s) {
+        hipEvent_t e = take();
+        SDT_HIP_CHECK(hipEventRecord(e, s));
+        return e;
+    }
+    void end(int stage, hipEvent_t a, hipStream_t s) {
+        hipEvent_t b = take();
+        SDT_HIP_CHECK(hipEventRecord(b, s));
+        pending.push_back(Rec{stage, a, b});
+    }
+    // call after the stream has been synchronised
+    void collect() {
+        for (const Rec& r : pending) {
+            float ms = 0.f;
+            SDT_HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+            sum_ms[r.stage] += ms;
+            count[r.stage] += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        for (int i = 0; i < kStages; i++) { sum_ms[i] = 0.0; count[i] = 0; }
+    }
+    ~StageTimer() {
+        for (const Rec& r : pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// RAII stage scope: no-op when the timer is null or off
+struct StageScope {
+    StageTimer* t;
+    int stage;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    StageScope(StageTimer* t_, int stage_, hipStream_t s_) : t(t_), stage(stage_), s(s_) {
+        if (t && t->on) a = t->begin(s);
+    }
+    ~StageScope() noexcept(false) {
+        if (t && t->on && a) t->end(stage, a, s);
+    }
+};
+
+}  // namespace sdt

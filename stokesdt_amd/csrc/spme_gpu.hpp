@@ -51,6 +51,7 @@ struct SpmeGpu {
     void fft_backward(int b, hipStream_t s);
     void interp(int b, double alpha, double beta, double* Y, hipStream_t s);
     int nmesh(int b) const { return (poly ? 6 : 3) * b; }
+    StageTimer* timer = nullptr;
 
     // scratch from prepare()
     DevBuf<T> d_frac_u;

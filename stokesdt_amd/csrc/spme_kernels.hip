@@ -679,11 +679,11 @@ template <typename T>
 void SpmeGpu<T>::apply(int b, double alpha, const double* X, double beta, double* Y,
                        hipStream_t s) {
     reserve_rhs(b);
-    spread(b, X, s);
-    fft_forward(b, s);
-    influence(b, s);
-    fft_backward(b, s);
-    interp(b, alpha, beta, Y, s);
+    { StageScope sc(timer, 0, s); spread(b, X, s); }
+    { StageScope sc(timer, 1, s); fft_forward(b, s); }
+    { StageScope sc(timer, 2, s); influence(b, s); }
+    { StageScope sc(timer, 3, s); fft_backward(b, s); }
+    { StageScope sc(timer, 4, s); interp(b, alpha, beta, Y, s); }
 }
 
 template struct SpmeGpu<float>;

@@ -1,0 +1,176 @@
+"""GPU parity of the SPME mobility (MobSpme::MulVector / RealMulVector /
+RecipMulVector, src/mob/mob_spme.cc:219-262, and rpy_spme_mex,
+mex/rpy_spme_mex.cc:8-75) against the oracle.
+
+Tolerances: FP64 GPU vs FP64 CPU <= 1e-10 relative 2-norm; FP32-mixed vs
+FP64 GPU <= 1e-5 (the north-star gates).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import uniform_system
+
+pytestmark = pytest.mark.gpu
+G = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_small.npz"))
+
+CASES = [
+    # (n, box, xi, rmax, K, p, poly, nrhs)
+    (64, 12.0, 0.9, 5.0, 24, 6, False, 3),
+    (300, 20.0, 0.9, 5.0, 32, 6, False, 1),
+    (300, 20.0, 0.9, 5.0, 32, 4, False, 2),
+    (300, 20.0, 0.9, 5.0, 36, 5, False, 3),
+    (300, 20.0, 0.9, 5.0, 32, 7, True, 2),
+    (250, 20.0, 0.8, 8.3, 48, 8, False, 17),
+    (200, 16.0, 1.0, 4.0, 40, 6, True, 4),
+]
+
+
+def _pair(sdt, orc, n, box, xi, rmax, K, p, poly, seed, prec):
+    pos, rdi = uniform_system(n, box, seed)
+    if poly:
+        rdi = np.random.default_rng(seed + 1).uniform(0.7, 1.3, n)
+    g = sdt.MobSpme(n, rdi, box, xi, rmax, K, p, prec)
+    assert g.init(), getattr(g, "init_error", "")
+    g.update(pos)
+    o = orc.MobSpme(n, rdi, box, xi, rmax, K, p)
+    assert o.init()
+    o.update(pos, rdi)
+    return g, o, pos, rdi
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fp64_matches_oracle(sdt, orc, case):
+    n, box, xi, rmax, K, p, poly, nrhs = case
+    print("case", case, flush=True)
+    g, o, _, _ = _pair(sdt, orc, n, box, xi, rmax, K, p, poly, 100 + K, sdt.FP64)
+    rng = np.random.default_rng(K)
+    f = rng.uniform(0, 1, (3 * n, nrhs))
+    v0 = rng.uniform(0, 1, (3 * n, nrhs))
+    for mode, fn in (("full", g.mul_vector), ("real", g.real_mul_vector),
+                     ("recip", g.recip_mul_vector)):
+        out = fn(f, 2.1, 1.1, v0)
+        ref = o.mul_vector(f, 2.1, 1.1, v0, mode=mode)
+        assert orc.frob_rel_err(ref, out) < 1e-10, mode
+    assert g.real_nnz() == o.real_nnz()
+
+
+def test_fp64_matches_golden(sdt, orc):
+    for tag in ("mono", "poly"):
+        box, xi, rmax, K, p = G[tag + "_params"]
+        pos, rdi, f = G[tag + "_pos"], G[tag + "_rdi"], G[tag + "_f"]
+        g = sdt.MobSpme(rdi.size, rdi, box, xi, rmax, int(K), int(p), sdt.FP64)
+        assert g.init()
+        g.update(pos)
+        for mode, fn in (("full", g.mul_vector), ("real", g.real_mul_vector),
+                         ("recip", g.recip_mul_vector)):
+            out = fn(f, 2.1, 1.1, G["mono_v0"]) if tag == "mono" else fn(f)
+            assert orc.frob_rel_err(G[tag + "_" + mode], out) < 1e-10, (tag, mode)
+
+
+@pytest.mark.parametrize("poly", [False, True])
+def test_fp32_within_1e5_of_fp64(sdt, orc, poly):
+    n, box = 2000, 30.0
+    pos, rdi = uniform_system(n, box, 7)
+    if poly:
+        rdi = np.random.default_rng(8).uniform(0.7, 1.3, n)
+    f = np.random.default_rng(9).standard_normal((3 * n, 4))
+    outs = []
+    for prec in (sdt.FP64, sdt.FP32):
+        g = sdt.MobSpme(n, rdi, box, 0.9, 5.0, 64, 6, prec)
+        assert g.init()
+        g.update(pos)
+        outs.append(g.mul_vector(f))
+    assert orc.frob_rel_err(outs[0], outs[1]) < 1e-5
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-6)])
+def test_spread_mesh_matches_oracle(sdt, orc, prec, tol):
+    n, box, K, p = 300, 20.0, 32, 6
+    pos, rdi = uniform_system(n, box, 3)
+    g = sdt.MobSpme(n, rdi, box, 0.9, 5.0, K, p, prec)
+    assert g.init()
+    g.update(pos)
+    e = orc.SpmeEngine(n, None, box, 0.9, K, p)
+    e.update(pos)
+    f = np.random.default_rng(4).standard_normal(3 * n)
+    ref = e.spread(f)
+    assert np.abs(g.debug_spread(f) - ref).max() <= tol * np.abs(ref).max()
+
+
+def test_rpy_spme_one_shot(sdt, orc):
+    n, box = 200, 18.0
+    pos, rdi = uniform_system(n, box, 5)
+    f = np.random.default_rng(6).standard_normal((3 * n, 2))
+    o = orc.MobSpme(n, rdi, box, 0.9, 5.0, 32, 6)
+    assert o.init()
+    o.update(pos, rdi)
+    for mode in ("full", "real", "recip"):
+        out = sdt.rpy_spme(pos.reshape(n, 3).T, rdi, box, f, mode, 0.9, 5.0, 32, 6)
+        assert orc.frob_rel_err(o.mul_vector(f, mode=mode), out) < 1e-10
+
+
+def test_positions_outside_box_are_wrapped(sdt):
+    # the reference discards the periodic wrap in the SPME update
+    # (src/mob/spme.cc:404-412); this path wraps like the pair lists
+    n, box = 150, 16.0
+    pos, rdi = uniform_system(n, box, 11)
+    shift = np.random.default_rng(12).integers(-2, 3, 3 * n) * box
+    f = np.random.default_rng(13).standard_normal((3 * n, 2))
+    outs = []
+    for p in (pos, pos + shift):
+        g = sdt.MobSpme(n, rdi, box, 0.9, 5.0, 32, 6, sdt.FP64)
+        assert g.init()
+        g.update(p)
+        outs.append(g.mul_vector(f))
+    assert np.abs(outs[0] - outs[1]).max() < 1e-11 * np.abs(outs[0]).max()
+
+
+def test_operator_symmetric_and_linear(sdt):
+    n, box = 1500, 28.0
+    pos, rdi = uniform_system(n, box, 14)
+    g = sdt.MobSpme(n, rdi, box, 0.9, 5.0, 48, 6, sdt.FP64)
+    assert g.init()
+    g.update(pos)
+    rng = np.random.default_rng(15)
+    x, y = rng.standard_normal(3 * n), rng.standard_normal(3 * n)
+    Mx, My = g.mul_vector(x)[:, 0], g.mul_vector(y)[:, 0]
+    assert abs(x @ My - y @ Mx) < 1e-12 * abs(x @ My)
+    Mxy = g.mul_vector(np.stack([x, y, 2 * x - 3 * y], 1))
+    assert np.abs(Mxy[:, 2] - (2 * Mx - 3 * My)).max() < 1e-12 * np.abs(Mxy).max()
+
+
+def test_init_and_state_errors(sdt):
+    n = 10
+    pos, rdi = uniform_system(n, 20.0, 1)
+    g = sdt.MobSpme(n, rdi, 20.0, 0.9, 5.0, 20, 6, sdt.FP64)   # 4p > K
+    assert not g.init() and "dim/4" in g.init_error
+    g = sdt.MobSpme(n, rdi, 20.0, 0.9, 5.0, 28, 4, sdt.FP64)   # 28 = 4 * 7
+    assert not g.init()
+    g = sdt.MobSpme(n, rdi, 20.0, 0.9, 10.0, 32, 6, sdt.FP64)  # 2 rmax >= L
+    assert not g.init()
+    g = sdt.MobSpme(n, rdi, 20.0, 0.9, 5.0, 32, 6, sdt.FP64)
+    assert g.init()
+    with pytest.raises(sdt.SdtError) as e:
+        g.mul_vector(np.zeros(3 * n))
+    assert e.value.code == sdt.SDT_ESTATE
+
+
+def test_large_system_fp32_vs_fp64(sdt, orc):
+    # size-independent properties at the N = 100 000, phi = 0.3, K = 128,
+    # p = 6 shape: FP32 within 1e-5 of FP64, FP64 operator symmetric
+    n = 100_000
+    box = (4.0 * np.pi * n / (3.0 * 0.3)) ** (1.0 / 3.0)
+    pos, rdi = uniform_system(n, box, 2024)
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal((3 * n, 2))
+    outs = []
+    for prec in (sdt.FP64, sdt.FP32):
+        g = sdt.MobSpme(n, rdi, box, 0.896, 5.0, 128, 6, prec)
+        assert g.init()
+        g.update(pos)
+        outs.append(g.mul_vector(f))
+    assert orc.frob_rel_err(outs[0], outs[1]) < 1e-5
+    x, y = f[:, 0], f[:, 1]
+    assert abs(x @ outs[0][:, 1] - y @ outs[0][:, 0]) < 1e-11 * abs(x @ outs[0][:, 1])

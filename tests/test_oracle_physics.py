@@ -1,0 +1,275 @@
+"""Pins the oracle's mobility, pair lists, steric force, Lanczos and BD step
+with the relations the reference's own tests assert and with properties the
+mathematics fixes.
+
+Reference tests restated here: tests/spme/testspme.cc (SPME vs dense Ewald
+< 1e-6 at the parameters of tests/spme/test.sh:2-9), tests/steric/
+teststeric.cc (cell list vs O(N^2) DirectSteric < 1e-6, :34-79,155),
+tests/lanczos/testlanczos.cc (single and block Lanczos vs dense eigen-sqrt
+< 1e-6 at tol 1e-7, :81-144). The reference's input files are not copied:
+equivalent systems are generated from seeds (tests/conftest.py).
+"""
+import numpy as np
+import pytest
+
+from conftest import poly_system, uniform_system
+
+L1000 = 34.729314   # box of the reference's N1000 phi=0.1 system
+
+
+def _spme_vs_ewald(orc, pos, rdi, box, xi, rmax, K, p, nrhs=3, seed=7):
+    n = rdi.size
+    M = orc.ewald_matrix(pos, rdi, box, tol=1e-12)
+    m = orc.MobSpme(n, rdi, box, xi, rmax, K, p)
+    assert m.init()
+    m.update(pos, rdi)
+    f = np.random.default_rng(seed).uniform(0, 1, (3 * n, nrhs))
+    v = m.mul_vector(f, alpha=2.1, beta=1.1, v=f)
+    ve = 2.1 * (M @ f) + 1.1 * f
+    return orc.frob_rel_err(ve, v)
+
+
+@pytest.mark.slow
+def test_spme_matches_ewald_reference_params_mono(orc):
+    # tests/spme/test.sh:2-3: N1000 phi 0.1, dim 256, porder 6, xi 0.8, rmax 8.371887
+    pos, rdi = uniform_system(1000, L1000, 11)
+    err = _spme_vs_ewald(orc, pos, rdi, L1000, 0.8, 8.371887, 256, 6, nrhs=1)
+    assert err < 1e-6, err
+
+
+def test_spme_matches_ewald_small(orc):
+    pos, rdi = uniform_system(200, 20.0, 12)
+    assert _spme_vs_ewald(orc, pos, rdi, 20.0, 0.8, 8.3, 128, 8) < 1e-7
+
+
+def test_spme_matches_ewald_poly(orc):
+    # polydisperse path (three mesh round trips, src/mob/spme.cc:771-838)
+    rng = np.random.default_rng(13)
+    pos = rng.uniform(0, 20.0, 600)
+    rdi = rng.uniform(0.5, 1.5, 200)
+    assert _spme_vs_ewald(orc, pos, rdi, 20.0, 0.8, 8.3, 128, 8) < 1e-7
+
+
+def test_spme_converges_with_mesh(orc):
+    pos, rdi = uniform_system(150, 18.0, 14)
+    errs = [_spme_vs_ewald(orc, pos, rdi, 18.0, 1.0, 6.0, K, 6, nrhs=1) for K in (32, 64, 128)]
+    assert errs[0] > errs[1] > errs[2]
+
+
+def test_ewald_xi_independent_and_symmetric(orc):
+    pos, rdi = uniform_system(60, 12.0, 15)
+    M1 = orc.ewald_matrix(pos, rdi, 12.0, tol=1e-13)
+    M2 = orc.ewald_matrix(pos, rdi, 12.0, xi=2.0 * orc.ewald_auto_xi(12.0), tol=1e-13)
+    assert np.linalg.norm(M1 - M2) / np.linalg.norm(M1) < 1e-10
+    assert np.abs(M1 - M1.T).max() == 0.0
+    assert np.linalg.eigvalsh(M1).min() > 0.0
+
+
+def test_ewald_free_space_limit(orc):
+    # a pair at r = 10 in growing boxes: the off-diagonal block tends to the
+    # free-space RPY tensor (6 pi eta a scaling, unit radii)
+    #   xa = 3/(2r) - 1/r^3 along e,  ya = 3/(4r) + 1/(2r^3) across,
+    # with the leading periodic correction decaying as 1/L
+    r = 10.0
+    xa = 1.5 / r - 1.0 / r ** 3
+    ya = 0.75 / r + 0.5 / r ** 3
+    dev = []
+    for box in (200.0, 400.0, 800.0):
+        pos = np.array([0.0, 0.0, 0.0, r, 0.0, 0.0])
+        M = orc.ewald_matrix(pos, np.ones(2), box, tol=1e-13)
+        blk = M[0:3, 3:6]
+        dev.append((blk[0, 0] - xa, blk[1, 1] - ya))
+        assert abs(blk[0, 1]) < 1e-12 and abs(blk[1, 2]) < 1e-12
+    for c in range(2):
+        assert abs(dev[0][c] / dev[1][c] - 2.0) < 0.1
+        assert abs(dev[1][c] / dev[2][c] - 2.0) < 0.05
+    assert abs(dev[2][0]) < 0.05 * xa and abs(dev[2][1]) < 0.05 * ya
+
+
+def test_spme_operator_symmetric(orc):
+    pos, rdi = uniform_system(120, 16.0, 16)
+    m = orc.MobSpme(120, rdi, 16.0, 0.9, 5.0, 32, 6)
+    assert m.init()
+    m.update(pos, rdi)
+    rng = np.random.default_rng(0)
+    x, y = rng.standard_normal(360), rng.standard_normal(360)
+    Mx, My = m.mul_vector(x)[:, 0], m.mul_vector(y)[:, 0]
+    assert abs(x @ My - y @ Mx) < 1e-12 * abs(x @ My)
+
+
+def test_spread_interp_adjoint_and_brute_force(orc):
+    rng = np.random.default_rng(17)
+    n, K, p, box = 20, 16, 4, 10.0
+    pos = rng.uniform(0, box, 3 * n)
+    e = orc.SpmeEngine(n, None, box, 0.9, K, p)
+    e.update(pos)
+    f = rng.standard_normal(3 * n)
+    G = e.spread(f)
+    # brute force: mesh point m gets sum_i M_p(wrapped distance) f_i
+    brute = np.zeros((3, K, K, K))
+    for i in range(n):
+        c = pos[3 * i:3 * i + 3] / box * K
+        g = np.floor(c)
+        base = (g - p // 2 + 1).astype(int) % K
+        w = [[orc.bspline(p, g[d] - c[d] + x) for x in range(p)] for d in range(3)]
+        for z in range(p):
+            for y in range(p):
+                for x in range(p):
+                    brute[:, (base[2] + z) % K, (base[1] + y) % K, (base[0] + x) % K] += \
+                        w[0][x] * w[1][y] * w[2][z] * f[3 * i:3 * i + 3]
+    np.testing.assert_allclose(G, brute, atol=1e-13)
+    H = rng.standard_normal((3, K, K, K))
+    assert abs(e.interp(H) @ f - (H * G).sum()) < 1e-10 * abs((H * G).sum())
+
+
+def test_lm2_closed_form(orc):
+    # lm2 = ScalarRecip(|k|) |b_x b_y b_z|^2 (src/mob/spme.cc:71-160,
+    # Essmann et al. 1995 moduli); checked at a few wavenumbers
+    K, p, box, xi = 24, 6, 12.0, 0.7
+    e = orc.SpmeEngine(4, None, box, xi, K, p)
+    lm2 = e.lm2()
+    sv = [orc.bspline(p, i) for i in range(p - 1)]
+
+    def bmod2(m):
+        s = sum(sv[i] * np.exp(2j * np.pi * m * i / K) for i in range(p - 1))
+        return 1.0 / abs(s) ** 2
+
+    for (mz, my, mx) in [(1, 0, 0), (3, -2, 5), (-7, 12, 12), (0, 1, 1), (-11, -5, 9)]:
+        k = 2 * np.pi / box * np.array([mx, my, mz])
+        kk = k @ k
+        kx2 = kk / xi ** 2
+        sr = 6 * np.pi * np.exp(-kx2 / 4) / kk * (1 + kx2 * (0.25 + kx2 / 8))
+        ref = sr * bmod2(mx) * bmod2(my) * bmod2(mz)
+        assert abs(lm2[mz % K, my % K, mx] - ref) < 1e-12 * ref
+
+
+def _brute_pairs(pos, rdi, box, cutoff, mode):
+    n = rdi.size
+    P = pos.reshape(n, 3)
+    x0 = np.mod(P, box)
+    rows = []
+    for i in range(n):
+        d = np.abs(x0 - x0[i])
+        d = np.where(d > box / 2.0, d - box, d)
+        rr = (d ** 2).sum(1)
+        if mode == "R":
+            sel = np.sqrt(rr) <= cutoff
+        else:
+            rr = np.where(rr == 0.0, 1e-10, rr)
+            sel = 2.0 * np.sqrt(rr) / (rdi[i] + rdi) < cutoff
+        rows.append(set(np.nonzero(sel)[0].tolist()))
+    return rows
+
+
+@pytest.mark.parametrize("mode,cutoff", [("R", 5.0), ("R", 2.7), ("S", 2.0), ("S", 2.3)])
+def test_pairlist_matches_brute_force(orc, mode, cutoff):
+    pos, rdi = uniform_system(400, 24.0, 18)
+    if mode == "S":
+        rdi = np.random.default_rng(1).uniform(0.6, 1.4, 400)
+    ia, ja = orc.pairlist(pos, rdi, 24.0, cutoff, mode)
+    rows = _brute_pairs(pos, rdi, 24.0, cutoff, mode)
+    for i in range(400):
+        got = ja[ia[i]:ia[i + 1]]
+        assert list(got) == sorted(got)
+        assert set(got.tolist()) == rows[i], i
+        assert i in set(got.tolist())   # self pair included
+
+
+def _direct_steric(pos, rdi, box, cutoff, k0):
+    # O(N^2) force of tests/steric/teststeric.cc:34-79
+    n = rdi.size
+    P = pos.reshape(n, 3)
+    f = np.zeros((n, 3))
+    for i in range(n):
+        d = np.remainder(P[i] - P + box / 2.0, box) - box / 2.0
+        r = np.sqrt((d ** 2).sum(1))
+        r = np.where(r == 0.0, 1e-10, r)
+        s = 2.0 * r / (rdi[i] + rdi)
+        sel = (s - cutoff < 0.0)
+        sel[i] = False
+        force = k0 * ((rdi[i] + rdi) * cutoff / 2.0 - r)
+        f[i] = ((force / r)[sel, None] * d[sel]).sum(0)
+    return f.reshape(-1)
+
+
+def test_steric_matches_direct(orc):
+    # tests/steric/test.sh:2: polydisperse, cutoff 2.3, k0 125
+    pos, rdi = poly_system(412, 1000.0, 19)
+    f = orc.steric(pos, rdi, 1000.0, 2.3, 125.0)
+    fd = _direct_steric(pos, rdi, 1000.0, 2.3, 125.0)
+    assert np.linalg.norm(f) > 0
+    assert np.linalg.norm(f - fd) / np.linalg.norm(fd) < 1e-6
+    # Newton's third law
+    assert np.abs(f.reshape(-1, 3).sum(0)).max() < 1e-9 * np.abs(f).max()
+
+
+@pytest.fixture(scope="module")
+def ewald1000(orc):
+    pos, rdi = uniform_system(1000, L1000, 20)
+    return orc.ewald_matrix(pos, rdi, L1000, tol=1e-12)
+
+
+def _eig_sqrt(M):
+    w, V = np.linalg.eigh(M)
+    return (V * np.sqrt(w)) @ V.T
+
+
+@pytest.mark.slow
+def test_lanczos_matches_eigen_sqrt(orc, ewald1000):
+    # tests/lanczos/testlanczos.cc:81-144: maxiters 100, maxnrhs 16, tol 1e-7,
+    # z ~ Uniform(0, 1); single (1 rhs) and block (3 rhs), both < 1e-6
+    M = ewald1000
+    M2 = _eig_sqrt(M)
+    z = np.random.default_rng(21).uniform(0, 1, (3000, 3))
+    y1, it1 = orc.lanczos(M, z[:, :1], 100, 16, 1e-7)
+    assert orc.frob_rel_err(M2 @ z[:, :1], y1) < 1e-6
+    y3, it3 = orc.lanczos(M, z, 100, 16, 1e-7)
+    assert orc.frob_rel_err(M2 @ z, y3) < 1e-6
+    assert it1[0] > 0 and it3[0] > 0
+
+
+def test_lanczos_non_convergence_reported(orc):
+    rng = np.random.default_rng(22)
+    A = rng.standard_normal((90, 90))
+    M = A @ A.T + 0.01 * np.eye(90)   # ill-conditioned
+    y, it = orc.lanczos(M, rng.standard_normal((90, 2)), 3, 2, 1e-12)
+    assert it[0] == -1
+
+
+def test_cholesky_displacements(orc):
+    # BrwnChol (src/brwn/brwn_chol.cc:52-83): y = L z with M = L L^T; not
+    # pointwise equal to M^{1/2} z, but L L^T reproduces M
+    pos, rdi = uniform_system(40, 10.0, 23)
+    M = orc.ewald_matrix(pos, rdi, 10.0, tol=1e-12)
+    eye = np.eye(120)
+    Lz, info = orc.chol_displacements(M, eye)
+    assert info == 0
+    np.testing.assert_allclose(Lz @ Lz.T, M, atol=1e-12 * np.abs(M).max())
+
+
+def test_bd_step_composition(orc):
+    # one BD step without overlaps (f = 0): pos' = pos + sqrt(2 dt) M^{1/2} z
+    # with z the first column of the Philox block of stream 0
+    rng = np.random.default_rng(24)
+    n, box = 30, 14.0
+    pos = np.zeros(3 * n)
+    k = 0
+    while k < n:   # random sequential addition, no overlap
+        c = rng.uniform(0, box, 3)
+        ok = all(np.linalg.norm(np.remainder(c - pos[3 * j:3 * j + 3] + box / 2, box) - box / 2)
+                 >= 2.05 for j in range(k))
+        if ok:
+            pos[3 * k:3 * k + 3] = c
+            k += 1
+    rdi = np.ones(n)
+    K, p, xi, rmax, dt, mi = 32, 6, 0.9, 5.0, 1e-3, 4
+    bd = orc.BDSim(pos, rdi, box, xi, rmax, K, p, 200, 4, 1e-10, 2.0, 125.0, dt, mi, 99)
+    bd.advance(1)
+    m = orc.MobSpme(n, rdi, box, xi, rmax, K, p)
+    assert m.init()
+    m.update(pos, rdi)
+    Mdense = m.mul_vector(np.eye(3 * n))
+    Mdense = 0.5 * (Mdense + Mdense.T)
+    z = orc.gaussian(99, 0, 3 * n * mi)[:3 * n]
+    expect = pos + np.sqrt(2 * dt) * (_eig_sqrt(Mdense) @ z)
+    np.testing.assert_allclose(bd.positions(), expect, atol=1e-8)
