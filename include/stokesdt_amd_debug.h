@@ -18,6 +18,11 @@ extern "C" {
 /* G = A^T B (b x b, row-major) for row-major n x b host arrays, on the
  * device Gram kernel (FP64 MFMA). */
 int sdt_debug_gram(int64_t n, int b, const double* A, const double* B, double* G);
+/* Column-group Gram: for row-major n x btot host arrays split into groups of
+ * bs columns (the last may be narrower), G[g] = A_g^T B_g stored as
+ * [ngroup][bs][bs] (entries beyond a narrower group are zero). */
+int sdt_debug_gram_groups(int64_t n, int btot, int bs, const double* A, const double* B,
+                          double* G);
 /* T^{1/2} E_1 (m x kb, row-major) of a symmetric band matrix T (m x m,
  * row-major, bandwidth kd): the host routine used by block Lanczos. */
 int sdt_debug_band_sqrt(int m, int kd, const double* T, int kb, double* S);

@@ -80,6 +80,7 @@ SIGNATURES = {
     "sdt_gaussian": (_I, [ctypes.c_uint64, ctypes.c_uint32, _I64, _P]),
     "sdt_device_synchronize": (_I, []),
     "sdt_debug_gram": (_I, [_I64, _I, _P, _P, _P]),
+    "sdt_debug_gram_groups": (_I, [_I64, _I, _I, _P, _P, _P]),
     "sdt_debug_band_sqrt": (_I, [_I, _I, _P, _I, _P]),
     "sdt_debug_spme_spread": (_I, [_P, _P, _P]),
     "sdt_bench_stage_times": (_I, [_P, _I, _I, _P]),
@@ -424,6 +425,17 @@ def debug_gram(A, B):
     n, b = A.shape
     G = np.zeros((b, b))
     _check(lib().sdt_debug_gram(n, b, _ptr(A), _ptr(B), _ptr(G)))
+    return G
+
+
+def debug_gram_groups(A, B, bs):
+    """[A_g^T B_g] for the column groups of width bs of n x btot A, B."""
+    A = _f64(A)
+    B = _f64(B)
+    n, btot = A.shape
+    ng = (btot + bs - 1) // bs
+    G = np.zeros((ng, bs, bs))
+    _check(lib().sdt_debug_gram_groups(n, btot, bs, _ptr(A), _ptr(B), _ptr(G)))
     return G
 
 

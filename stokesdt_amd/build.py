@@ -50,7 +50,7 @@ def build(verbose=False, jobs=8):
     for src, extra in HOST_SOURCES:
         o = os.path.join(OBJ, src + ".o")
         objs.append(o)
-        cmds.append(["g++", "-O2", "-fPIC", "-std=c++17"] + extra +
+        cmds.append(["g++", "-O3", "-fPIC", "-std=c++17"] + extra +
                     ["-c", os.path.join(CSRC, src), "-o", o])
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         for out in ex.map(_run, cmds):

@@ -4,13 +4,22 @@
 
 namespace sdt {
 
-// G (b x b, row-major, device) = A^T B for packed n x b blocks A, B
+// G (16 x 16 tile holding b x b, row-major, device) = A^T B for packed
+// n x b blocks A, B (b <= 16)
 void gram(long long n, int b, const double* A, const double* B, double* d_G, DevBuf<double>& part,
           hipStream_t s);
+// Column groups of a packed n x btot block (leading dimension btot): group g
+// holds columns [g bs, min((g + 1) bs, btot)). d_G[g][16][16] = A_g^T B_g.
+void gram_groups(long long n, int btot, int bs, const double* A, const double* B, double* d_G,
+                 DevBuf<double>& part, hipStream_t s);
 // C = beta C + alpha [A_0 .. A_{nblk-1}] S, blocks n x b at stride blk_stride,
 // S row-major (nblk*b) x b on the device
 void tsmm(long long n, int b, int nblk, const double* A, long long blk_stride, const double* d_S,
           double alpha, double beta, double* C, hipStream_t s);
+// per group: C_g = beta C_g + alpha sum_k A_{k,g} S[g][k] with S laid out
+// [ngroup][nblk][bs][bs] (rows/columns beyond a narrower last group unused)
+void tsmm_groups(long long n, int btot, int bs, int nblk, const double* A, long long blk_stride,
+                 const double* d_S, double alpha, double beta, double* C, hipStream_t s);
 // sum of squares (synchronises)
 double sumsq(long long n, const double* x, DevBuf<double>& part, hipStream_t s);
 // reference layout (column-major 3N x b, leading dimension ld, original

@@ -5,22 +5,32 @@
 namespace sdt {
 
 // ---------------------------------------------------------------- Gram -----
-// Partial G = A^T B over a chunk of rows per workgroup using the FP64 MFMA
-// (v_mfma_f64_16x16x4_f64): each wave takes 4 rows per instruction; lanes
-// hold A^T (16 x 4) and B (4 x 16) fragments, columns >= b are zero.
-// Partials are reduced by a second kernel in fixed order (deterministic).
+// Column groups: a packed row-major n x btot block (leading dimension btot)
+// is split into groups of bs consecutive columns (the last may be narrower).
+// Each group is an independent Krylov block of width <= 16.
+//
+// Partial G_g = A_g^T B_g over a chunk of rows per workgroup using the FP64
+// MFMA (v_mfma_f64_16x16x4_f64): each wave takes 4 rows per instruction;
+// lanes hold A^T (16 x 4) and B (4 x 16) fragments, columns >= b_g are
+// zero. Consecutive workgroups take the groups of one row chunk, so the rows
+// are read from HBM once and re-read from L2. Partials are reduced by a
+// second kernel in fixed order (deterministic).
 constexpr int kGramRows = 2048;   // rows per workgroup
 constexpr int kGramThreads = 256;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(kGramThreads)
-gram_partial_kernel(long long n, int b, const double* __restrict__ A, const double* __restrict__ B,
-                    double* __restrict__ part) {
+gram_partial_kernel(long long n, int btot, int bs, int ngroup, const double* __restrict__ A,
+                    const double* __restrict__ B, double* __restrict__ part) {
     __shared__ double s_red[kGramThreads / 64][256];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const long long row0 = (long long)blockIdx.x * kGramRows;
+    const int g = blockIdx.x % ngroup;
+    const int chunk = blockIdx.x / ngroup;
+    const int c0 = g * bs;
+    const int bg = min(bs, btot - c0);
+    const long long row0 = (long long)chunk * kGramRows;
     const long long rend = min(row0 + kGramRows, n);
     // fragment layout of v_mfma_f64_16x16x4f64 (measured on gfx950): A
     // operand lane l holds A[m = l % 16][k = l / 16]; B operand lane l holds
@@ -32,9 +42,9 @@ gram_partial_kernel(long long n, int b, const double* __restrict__ A, const doub
     for (long long r = row0 + wave * 4; r < rend; r += 4 * (kGramThreads / 64)) {
         const long long rr = r + krow;
         double a = 0.0, bb = 0.0;
-        if (rr < rend && mcol < b) {
-            a = A[rr * b + mcol];
-            bb = B[rr * b + mcol];
+        if (rr < rend && mcol < bg) {
+            a = A[rr * btot + c0 + mcol];
+            bb = B[rr * btot + c0 + mcol];
         }
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
     }
@@ -46,62 +56,80 @@ gram_partial_kernel(long long n, int b, const double* __restrict__ A, const doub
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < kGramThreads / 64; w++) s += s_red[w][t];
-    const int i = t / 16, j = t % 16;
-    if (i < b && j < b) part[(size_t)blockIdx.x * b * b + i * b + j] = s;
+    part[(size_t)blockIdx.x * 256 + t] = s;
 }
 
-__global__ void gram_reduce_kernel(int nparts, int b, const double* __restrict__ part,
+// G[g][16][16] = sum over chunks of the partial tiles of group g
+__global__ void gram_reduce_kernel(int nchunk, int ngroup, const double* __restrict__ part,
                                    double* __restrict__ G) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b * b) return;
+    const int g = blockIdx.x, t = threadIdx.x;
     double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[(size_t)p * b * b + t];
-    G[t] = s;
+    for (int p = 0; p < nchunk; p++) s += part[((size_t)p * ngroup + g) * 256 + t];
+    G[(size_t)g * 256 + t] = s;
+}
+
+void gram_groups(long long n, int btot, int bs, const double* A, const double* B, double* d_G,
+                 DevBuf<double>& part, hipStream_t s) {
+    SDT_REQUIRE(bs >= 1 && bs <= 16 && btot >= 1, SDT_EINVAL, "group width must be in [1, 16]");
+    const int ngroup = (btot + bs - 1) / bs;
+    const unsigned nchunk = ceil_div(n, kGramRows);
+    part.reserve((size_t)nchunk * ngroup * 256);
+    hipLaunchKernelGGL(gram_partial_kernel, dim3(nchunk * ngroup), dim3(kGramThreads), 0, s, n,
+                       btot, bs, ngroup, A, B, part.get());
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3(ngroup), dim3(256), 0, s, (int)nchunk, ngroup,
+                       part.get(), d_G);
+    SDT_HIP_CHECK(hipGetLastError());
 }
 
 void gram(long long n, int b, const double* A, const double* B, double* d_G, DevBuf<double>& part,
           hipStream_t s) {
-    SDT_REQUIRE(b >= 1 && b <= 16, SDT_EINVAL, "block width must be in [1, 16]");
-    const unsigned np = ceil_div(n, kGramRows);
-    part.reserve((size_t)np * b * b);
-    hipLaunchKernelGGL(gram_partial_kernel, dim3(np), dim3(kGramThreads), 0, s, n, b, A, B,
-                       part.get());
-    hipLaunchKernelGGL(gram_reduce_kernel, dim3(1), dim3(256), 0, s, (int)np, b, part.get(), d_G);
-    SDT_HIP_CHECK(hipGetLastError());
+    gram_groups(n, b, b, A, B, d_G, part, s);
 }
 
 // ------------------------------------------------------- small products ----
-// C[i][c] = beta * C[i][c] + alpha * sum_a A[i][a] S[a][c]   (n x ka) (ka x b)
-// S is row-major ka x b in device memory; A is given as nblk blocks of
-// n x b (block k at A + k * blk_stride), so ka = nblk * b.
+// C[i][c] = beta C[i][c] + alpha sum_k sum_j A_k[i][c0 + j] S[g][k][j][c - c0]
+// for column c of group g (columns c0 .. c0 + b_g - 1). A is nblk blocks of
+// n x btot (block k at A + k * blk_stride); S is [ngroup][nblk][bs][bs] on
+// the device, staged in LDS.
 __global__ void __launch_bounds__(256)
-tsmm_kernel(long long n, int b, int nblk, const double* __restrict__ A, long long blk_stride,
-            const double* __restrict__ S, double alpha, double beta, double* __restrict__ C) {
+tsmm_kernel(long long n, int btot, int bs, int nblk, const double* __restrict__ A,
+            long long blk_stride, const double* __restrict__ S, double alpha, double beta,
+            double* __restrict__ C) {
     extern __shared__ double s_S[];
-    const int ka = nblk * b;
-    for (int t = threadIdx.x; t < ka * b; t += blockDim.x) s_S[t] = S[t];
+    const int ngroup = (btot + bs - 1) / bs;
+    const int ns = ngroup * nblk * bs * bs;
+    for (int t = threadIdx.x; t < ns; t += blockDim.x) s_S[t] = S[t];
     __syncthreads();
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n * b) return;
-    const long long i = gid / b;
-    const int c = (int)(gid - i * b);
+    if (gid >= n * btot) return;
+    const long long i = gid / btot;
+    const int c = (int)(gid - i * btot);
+    const int g = c / bs, cl = c - g * bs, c0 = g * bs;
+    const int bg = min(bs, btot - c0);
     double acc = 0.0;
     for (int k = 0; k < nblk; k++) {
-        const double* a = A + k * blk_stride + i * b;
-        const double* sk = s_S + (size_t)k * b * b + c;
-        for (int j = 0; j < b; j++) acc += a[j] * sk[(size_t)j * b];
+        const double* a = A + k * blk_stride + i * btot + c0;
+        const double* sk = s_S + ((size_t)(g * nblk + k) * bs) * bs + cl;
+        for (int j = 0; j < bg; j++) acc += a[j] * sk[(size_t)j * bs];
     }
-    double* cp = C + i * b + c;
+    double* cp = C + i * btot + c;
     *cp = (beta == 0.0 ? 0.0 : beta * *cp) + alpha * acc;
+}
+
+void tsmm_groups(long long n, int btot, int bs, int nblk, const double* A, long long blk_stride,
+                 const double* d_S, double alpha, double beta, double* C, hipStream_t s) {
+    const int ngroup = (btot + bs - 1) / bs;
+    const size_t lds = sizeof(double) * (size_t)ngroup * nblk * bs * bs;
+    SDT_REQUIRE(lds <= 64 * 1024, SDT_ECAP, "tsmm: coefficient block too large for LDS");
+    SDT_REQUIRE(n * btot < (1ll << 40), SDT_EINVAL, "tsmm: block too large");
+    hipLaunchKernelGGL(tsmm_kernel, dim3(ceil_div(n * btot, 256)), dim3(256), lds, s, n, btot, bs,
+                       nblk, A, blk_stride, d_S, alpha, beta, C);
+    SDT_HIP_CHECK(hipGetLastError());
 }
 
 void tsmm(long long n, int b, int nblk, const double* A, long long blk_stride, const double* d_S,
           double alpha, double beta, double* C, hipStream_t s) {
-    const size_t lds = sizeof(double) * (size_t)nblk * b * b;
-    SDT_REQUIRE(lds <= 64 * 1024, SDT_ECAP, "tsmm: coefficient block too large for LDS");
-    hipLaunchKernelGGL(tsmm_kernel, dim3(ceil_div(n * b, 256)), dim3(256), lds, s, n, b, nblk, A,
-                       blk_stride, d_S, alpha, beta, C);
-    SDT_HIP_CHECK(hipGetLastError());
+    tsmm_groups(n, b, b, nblk, A, blk_stride, d_S, alpha, beta, C, s);
 }
 
 // ------------------------------------------------------- norms / copies ----

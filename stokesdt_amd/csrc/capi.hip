@@ -401,18 +401,30 @@ int sdt_device_synchronize(void) {
 }
 
 // ----------------------------------------------------------- debug hooks ---
-int sdt_debug_gram(int64_t n, int b, const double* A, const double* B, double* G) {
+int sdt_debug_gram_groups(int64_t n, int btot, int bs, const double* A, const double* B,
+                          double* G) {
     return guard([&] {
-        SDT_REQUIRE(n > 0 && b >= 1 && b <= 16 && A && B && G, SDT_EINVAL, "bad arguments");
+        SDT_REQUIRE(n > 0 && btot >= 1 && bs >= 1 && bs <= 16 && A && B && G, SDT_EINVAL,
+                    "bad arguments");
+        const int ng = (btot + bs - 1) / bs;
         DevBuf<double> dA, dB, dG, part;
-        dA.reserve((size_t)n * b);
-        dB.reserve((size_t)n * b);
-        dG.reserve((size_t)b * b);
-        SDT_HIP_CHECK(hipMemcpy(dA.get(), A, sizeof(double) * n * b, hipMemcpyHostToDevice));
-        SDT_HIP_CHECK(hipMemcpy(dB.get(), B, sizeof(double) * n * b, hipMemcpyHostToDevice));
-        gram(n, b, dA.get(), dB.get(), dG.get(), part, 0);
-        SDT_HIP_CHECK(hipMemcpy(G, dG.get(), sizeof(double) * b * b, hipMemcpyDeviceToHost));
+        dA.reserve((size_t)n * btot);
+        dB.reserve((size_t)n * btot);
+        dG.reserve((size_t)ng * 256);
+        SDT_HIP_CHECK(hipMemcpy(dA.get(), A, sizeof(double) * n * btot, hipMemcpyHostToDevice));
+        SDT_HIP_CHECK(hipMemcpy(dB.get(), B, sizeof(double) * n * btot, hipMemcpyHostToDevice));
+        gram_groups(n, btot, bs, dA.get(), dB.get(), dG.get(), part, 0);
+        std::vector<double> h((size_t)ng * 256);
+        SDT_HIP_CHECK(hipMemcpy(h.data(), dG.get(), sizeof(double) * h.size(),
+                                hipMemcpyDeviceToHost));
+        for (int g = 0; g < ng; g++)
+            for (int i = 0; i < bs; i++)
+                for (int j = 0; j < bs; j++)
+                    G[((size_t)g * bs + i) * bs + j] = h[(size_t)g * 256 + i * 16 + j];
     });
+}
+int sdt_debug_gram(int64_t n, int b, const double* A, const double* B, double* G) {
+    return sdt_debug_gram_groups(n, b, b, A, B, G);
 }
 int sdt_debug_band_sqrt(int m, int kd, const double* T, int kb, double* S) {
     return guard([&] {
@@ -499,7 +511,7 @@ int sdt_bench_stage_times(sdt_mob_t mob, int b, int reps, double* t) {
     return guard([&] {
         MobSpmeGpu* m = mob->m;
         SDT_REQUIRE(m->updated(), SDT_ESTATE, "mobility not updated");
-        SDT_REQUIRE(b >= 1 && b <= 16 && reps >= 1 && t, SDT_EINVAL, "bad arguments");
+        SDT_REQUIRE(b >= 1 && b <= 256 && reps >= 1 && t, SDT_EINVAL, "bad arguments");
         const size_t n = (size_t)m->dim() * b;
         DevBuf<double> dX, dY;
         dX.reserve(n);

@@ -5,11 +5,41 @@
 // src[q + s*(p + j*m)], j < r, applies an r-point DFT and the twiddle
 // w_len^(p*t), and writes dst[q + s*(r*p + t)]. After all stages the result
 // is in natural order. Twiddles come from one table of K roots of unity
-// w_K^j (w_len^(p t) = w_K^((K/len) p t mod K)).
+// w_K^j (w_len^(p t) = w_K^((K/len) p t), and (K/len) p t < K).
+//
+// LDS layout: element e of pencil k sits at k * PS + e + (e >> 4), i.e. one
+// pad slot every 16 elements and an odd pencil stride PS, so that strided
+// butterfly accesses and the transposed pencil loads of the y/z passes hit
+// distinct banks. Index splits use multiply-high reciprocals precomputed on
+// the host instead of integer division.
 #pragma once
 #include "sdt_internal.hpp"
 
 namespace sdt {
+
+struct FFTArgs {
+    int K;
+    int nst;
+    unsigned K_m, Kh_m;            // reciprocals of K and K / 2 + 1
+    int radix[16];
+    unsigned per[16], per_m[16];   // butterflies per pencil (K / r) and reciprocal
+    unsigned str[16], str_m[16];   // sub-sequence stride s = K / len and reciprocal
+};
+
+// floor(n / d) = umulhi(n, floor(2^32 / d) + 1) for n < 2^17 and 2 <= d <
+// 2^15 (the error term n (2^32 mod d) / 2^32 / d stays below 1 / d); the
+// reciprocal of 1 is encoded as 0.
+inline unsigned fft_magic(unsigned d) {
+    return d == 1 ? 0u : (unsigned)((1ull << 32) / d + 1ull);
+}
+
+__device__ __forceinline__ unsigned fdiv(unsigned n, unsigned m) {
+    return m ? __umulhi(n, m) : n;
+}
+
+__host__ __device__ inline int pencil_stride(int K) { return (K + (K >> 4) + 1) | 1; }
+
+__device__ __forceinline__ int lpad(int e) { return e + (e >> 4); }
 
 template <typename T>
 __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
@@ -23,7 +53,7 @@ template <typename T>
 __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
     return cplx<T>{a.x - b.x, a.y - b.y};
 }
-// multiply by -i*sign... : rot(a) = a * (sign < 0 ? -i : +i)
+// rot(a) = a * (sign < 0 ? -i : +i)
 template <typename T, int SIGN>
 __device__ __forceinline__ cplx<T> rot90(cplx<T> a) {
     return SIGN < 0 ? cplx<T>{a.y, -a.x} : cplx<T>{-a.y, a.x};
@@ -49,7 +79,6 @@ __device__ __forceinline__ void small_dft(cplx<T>* a) {
         cplx<T> t1 = cadd(a[1], a[2]);
         cplx<T> t2 = csub(a[1], a[2]);
         cplx<T> m = cplx<T>{a[0].x + c1 * t1.x, a[0].y + c1 * t1.y};
-        // s1 * i * t2
         cplx<T> j = cplx<T>{-s1 * t2.y, s1 * t2.x};
         a[0] = cadd(a[0], t1);
         a[1] = cadd(m, j);
@@ -63,7 +92,6 @@ __device__ __forceinline__ void small_dft(cplx<T>* a) {
         cplx<T> d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
         cplx<T> m1 = cplx<T>{a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y};
         cplx<T> m2 = cplx<T>{a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y};
-        // i * (s1 d1 + s2 d2) and i * (s2 d1 - s1 d2)
         cplx<T> n1 = cplx<T>{-(s1 * d1.y + s2 * d2.y), s1 * d1.x + s2 * d2.x};
         cplx<T> n2 = cplx<T>{-(s2 * d1.y - s1 * d2.y), s2 * d1.x - s1 * d2.x};
         a[0] = cplx<T>{a[0].x + t1.x + t2.x, a[0].y + t1.y + t2.y};
@@ -76,59 +104,51 @@ __device__ __forceinline__ void small_dft(cplx<T>* a) {
 
 template <typename T, int R, int SIGN>
 __device__ __forceinline__ void fft_stage(const cplx<T>* __restrict__ src,
-                                          cplx<T>* __restrict__ dst, int K, int npen,
-                                          int len, int s, const cplx<T>* __restrict__ tw) {
-    const int m = len / R;
-    const int per = K / R;                 // butterflies per pencil
-    const int total = npen * per;
-    const int tstep = K / len;
-    for (int item = threadIdx.x; item < total; item += blockDim.x) {
-        const int pen = item / per;
-        const int j = item - pen * per;
-        const int p = j / s;
-        const int q = j - p * s;
-        const cplx<T>* sp = src + pen * K + q + s * p;
-        cplx<T>* dp = dst + pen * K + q + s * R * p;
+                                          cplx<T>* __restrict__ dst, int npen, int PS,
+                                          const FFTArgs& fa, int st,
+                                          const cplx<T>* __restrict__ tw) {
+    const unsigned per = fa.per[st], per_m = fa.per_m[st];
+    const unsigned s = fa.str[st], s_m = fa.str_m[st];
+    const unsigned m = per / s;            // = len / R
+    const unsigned total = (unsigned)npen * per;
+    for (unsigned item = threadIdx.x; item < total; item += blockDim.x) {
+        const unsigned pen = fdiv(item, per_m);
+        const unsigned j = item - pen * per;
+        const unsigned p = fdiv(j, s_m);
+        const unsigned q = j - p * s;
+        const cplx<T>* sp = src + pen * PS;
+        cplx<T>* dp = dst + pen * PS;
         cplx<T> a[R];
 #pragma unroll
-        for (int t = 0; t < R; t++) a[t] = sp[s * t * m];
+        for (int t = 0; t < R; t++) a[t] = sp[lpad(q + s * (p + t * m))];
         small_dft<T, R, SIGN>(a);
-        dp[0] = a[0];
+        const unsigned ob = q + s * R * p;
+        dp[lpad(ob)] = a[0];
 #pragma unroll
         for (int t = 1; t < R; t++) {
-            int e = (tstep * p * t) % K;
-            cplx<T> w = tw[e];
+            cplx<T> w = tw[s * p * t];    // w_len^(p t) = w_K^(s p t), s p t < K
             if (SIGN > 0) w.y = -w.y;
-            dp[s * t] = cmul(a[t], w);
+            dp[lpad(ob + s * t)] = cmul(a[t], w);
         }
     }
 }
 
 // Runs all stages; returns the buffer holding the result (a or b).
 template <typename T, int SIGN>
-__device__ cplx<T>* fft_pencils(cplx<T>* a, cplx<T>* b, int K, int npen, const int* radix,
-                                int nst, const cplx<T>* tw) {
-    int len = K, s = 1;
-    for (int st = 0; st < nst; st++) {
-        const int r = radix[st];
-        if (r == 4) fft_stage<T, 4, SIGN>(a, b, K, npen, len, s, tw);
-        else if (r == 2) fft_stage<T, 2, SIGN>(a, b, K, npen, len, s, tw);
-        else if (r == 3) fft_stage<T, 3, SIGN>(a, b, K, npen, len, s, tw);
-        else fft_stage<T, 5, SIGN>(a, b, K, npen, len, s, tw);
+__device__ cplx<T>* fft_pencils(cplx<T>* a, cplx<T>* b, int npen, int PS, const FFTArgs& fa,
+                                const cplx<T>* tw) {
+    for (int st = 0; st < fa.nst; st++) {
+        const int r = fa.radix[st];
+        if (r == 4) fft_stage<T, 4, SIGN>(a, b, npen, PS, fa, st, tw);
+        else if (r == 2) fft_stage<T, 2, SIGN>(a, b, npen, PS, fa, st, tw);
+        else if (r == 3) fft_stage<T, 3, SIGN>(a, b, npen, PS, fa, st, tw);
+        else fft_stage<T, 5, SIGN>(a, b, npen, PS, fa, st, tw);
         __syncthreads();
-        len /= r;
-        s *= r;
         cplx<T>* t = a;
         a = b;
         b = t;
     }
     return a;
 }
-
-struct FFTArgs {
-    int K;
-    int nst;
-    int radix[16];
-};
 
 }  // namespace sdt

@@ -51,31 +51,59 @@ void mul_upper(int b, const double* A, const double* B, double* C) {
 }
 
 void BandSqrt::compute(int m, int kd, const double* T, int kb, double* S) {
-    A_.assign(T, T + (size_t)m * m);
+    const int ldt = 2 * kd + 1;
+    std::vector<double> Tb((size_t)m * ldt, 0.0);
+    for (int i = 0; i < m; i++)
+        for (int j = std::max(0, i - kd); j <= std::min(m - 1, i + kd); j++)
+            Tb[(size_t)i * ldt + (j - i + kd)] = T[(size_t)i * m + j];
+    compute_band(m, kd, Tb.data(), ldt, kb, S);
+}
+
+void BandSqrt::compute_band(int m, int kd, const double* Tb, int ldt, int kb, double* S) {
+    // band storage of half-width w (the bandwidth kd plus room for the
+    // bulge chased during the reduction): element (i, j), |i - j| <= w, at
+    // A[i * ld + j - i + w]
+    const int w = kd + 2;
+    const int ld = 2 * w + 1;
+    A_.assign((size_t)m * ld, 0.0);
+    double* A = A_.data();
+    auto at = [&](int i, int j) -> double& { return A[(size_t)i * ld + (j - i + w)]; };
+    for (int i = 0; i < m; i++)
+        for (int j = std::max(0, i - kd); j <= std::min(m - 1, i + kd); j++)
+            at(i, j) = Tb[(size_t)i * ldt + (j - i + kd)];
     band_rot_.clear();
     ql_rot_.clear();
-    double* A = A_.data();
     auto rotate = [&](int p, double c, double s) {
         // A <- G A G^T with G = [[c, s], [-s, c]] on rows/cols (p, p+1)
         const int q = p + 1;
-        const int lo = std::max(0, p - kd - 2), hi = std::min(m - 1, q + kd + 2);
-        for (int l = lo; l <= hi; l++) {
-            double ap = A[(size_t)p * m + l], aq = A[(size_t)q * m + l];
-            A[(size_t)p * m + l] = c * ap + s * aq;
-            A[(size_t)q * m + l] = -s * ap + c * aq;
+        const int lo = std::max(0, q - w), hi = std::min(m - 1, p + w);
+        // rows p and q (columns lo..hi lie within both bands)
+        double* rp = &at(p, lo);
+        double* rq = &at(q, lo);
+        for (int l = 0; l <= hi - lo; l++) {
+            double ap = rp[l], aq = rq[l];
+            rp[l] = c * ap + s * aq;
+            rq[l] = -s * ap + c * aq;
         }
+        // columns p and q, by symmetry, outside the 2x2 block
         for (int l = lo; l <= hi; l++) {
-            double ap = A[(size_t)l * m + p], aq = A[(size_t)l * m + q];
-            A[(size_t)l * m + p] = c * ap + s * aq;
-            A[(size_t)l * m + q] = -s * ap + c * aq;
+            if (l == p || l == q) continue;
+            at(l, p) = at(p, l);
+            at(l, q) = at(q, l);
         }
+        // the 2x2 block: rows were rotated, now the columns
+        double app = at(p, p), apq = at(p, q), aqp = at(q, p), aqq = at(q, q);
+        at(p, p) = c * app + s * apq;
+        at(p, q) = -s * app + c * apq;
+        at(q, p) = c * aqp + s * aqq;
+        at(q, q) = -s * aqp + c * aqq;
         band_rot_.push_back(Rot{p, c, s});
     };
     auto zero_with = [&](int p, int col) {
         // zero A[p+1][col] against A[p][col]
-        const double a = A[(size_t)p * m + col], b = A[(size_t)(p + 1) * m + col];
+        const double a = at(p, col), b = at(p + 1, col);
         if (b == 0.0) return;
-        const double r = hypot(a, b);
+        const double r = sqrt(a * a + b * b);
         rotate(p, a / r, b / r);
     };
     // band -> tridiagonal (bulge chasing)
@@ -95,8 +123,8 @@ void BandSqrt::compute(int m, int kd, const double* T, int kb, double* S) {
     }
     d_.resize(m);
     e_.assign(m, 0.0);
-    for (int i = 0; i < m; i++) d_[i] = A[(size_t)i * m + i];
-    for (int i = 0; i + 1 < m; i++) e_[i] = A[(size_t)(i + 1) * m + i];
+    for (int i = 0; i < m; i++) d_[i] = at(i, i);
+    for (int i = 0; i + 1 < m; i++) e_[i] = at(i + 1, i);
     // implicit QL with Wilkinson-type shifts, rotations recorded
     {
         double* d = d_.data();
@@ -112,13 +140,13 @@ void BandSqrt::compute(int m, int kd, const double* T, int kb, double* S) {
                 if (mm != l) {
                     if (++iter > 60) break;
                     double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                    double r = hypot(g, 1.0);
+                    double r = sqrt(g * g + 1.0);
                     g = d[mm] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
                     double s = 1.0, c = 1.0, p = 0.0;
                     bool early = false;
                     for (int i = mm - 1; i >= l; i--) {
                         double f = s * e[i], bb = c * e[i];
-                        r = hypot(f, g);
+                        r = sqrt(f * f + g * g);
                         e[i + 1] = r;
                         if (r == 0.0) {
                             d[i + 1] -= p;

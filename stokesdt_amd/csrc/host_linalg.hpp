@@ -21,6 +21,9 @@ void mul_upper(int b, const double* A, const double* B, double* C);
 class BandSqrt {
   public:
     void compute(int m, int kd, const double* T, int kb, double* S);
+    // the same with T in band storage: T(i, j), |i - j| <= kd, at
+    // Tb[i * ldt + j - i + kd] (rows 0 .. m-1; entries outside T unused)
+    void compute_band(int m, int kd, const double* Tb, int ldt, int kb, double* S);
 
   private:
     struct Rot { int p; double c, s; };

@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <future>
+#include <thread>
 #include "mob_gpu.hpp"
 
 namespace sdt {
@@ -114,7 +116,7 @@ void MobSpmeGpu::update(const double* pos, const double* rdi, bool dev) {
 void MobSpmeGpu::apply_packed(int mode, int b, double alpha, const double* X, double beta,
                               double* Y) {
     SDT_REQUIRE(updated_, SDT_ESTATE, "mulvector called before update");
-    SDT_REQUIRE(b >= 1 && b <= 16, SDT_EINVAL, "at most 16 right-hand sides per pass");
+    SDT_REQUIRE(b >= 1, SDT_EINVAL, "at least one right-hand side");
     hipStream_t s = stream_;
     const int* rp = pairs_.d_rowptr.get();
     const int* ci = pairs_.d_colidx.get();
@@ -177,6 +179,14 @@ void MobSpmeGpu::mul_vector(int mode, int nrhs, double alpha, int ldf, const dou
 }
 
 // ----------------------------------------------------------- Lanczos -------
+// BrwnLanczos::Compute (src/brwn/brwn_lanczos.cc:279-304) splits the
+// right-hand sides into groups of max_nrhs and runs BlockLanczos
+// (brwn_lanczos.cc:36-221) on each group in turn. Here the groups of a batch
+// run in lockstep: one mobility application per iteration covers every
+// column of the batch, the small per-group products (Gram, projection,
+// CholQR) are batched kernels over column groups, and the per-group
+// convergence test T^{1/2} E_1 runs on host threads while the device computes
+// the next iteration.
 BrwnLanczosGpu::BrwnLanczosGpu(int dim, int max_iters, int max_nrhs, double tol)
     : dim_(dim), max_iters_(max_iters), max_nrhs_(max_nrhs), tol_(tol) {}
 
@@ -188,166 +198,337 @@ void BrwnLanczosGpu::init() {
     SDT_REQUIRE(tol_ > 0.0, SDT_EINVAL, "tolerance must be positive");
     if (max_nrhs_ > dim_ / 3) max_nrhs_ = dim_ / 3;
     if (max_nrhs_ > 16) max_nrhs_ = 16;   // the device block width limit
-    d_V_.reserve((size_t)(max_iters_ + 1) * max_nrhs_ * dim_);
-    d_C_.reserve((size_t)max_nrhs_ * dim_);
-    d_G_.reserve(16 * 16);
     d_S_.reserve(8192);
     inited_ = true;
 }
 
-// CholQR2 of the packed n x b block W in place; R (upper, row-major) with
-// W_in = W_out R.
-static bool cholqr2(long long n, int b, double* W, double* tmp, double* R, DevBuf<double>& dG,
-                    DevBuf<double>& dS, DevBuf<double>& part, hipStream_t s, StageTimer* timer) {
-    std::vector<double> G(b * b), R1(b * b), Ri(b * b), R2(b * b);
-    for (int pass = 0; pass < 2; pass++) {
-        { StageScope sc(timer, 6, s); gram(n, b, W, W, dG.get(), part, s); }
-        SDT_HIP_CHECK(hipMemcpyAsync(G.data(), dG.get(), sizeof(double) * b * b,
-                                     hipMemcpyDeviceToHost, s));
-        SDT_HIP_CHECK(hipStreamSynchronize(s));
-        // symmetrise
-        for (int i = 0; i < b; i++)
-            for (int j = i + 1; j < b; j++) {
-                double a = 0.5 * (G[i * b + j] + G[j * b + i]);
-                G[i * b + j] = G[j * b + i] = a;
-            }
-        double* Rp = pass == 0 ? R1.data() : R2.data();
-        if (!chol_upper(b, G.data(), Rp)) return false;
-        inv_upper(b, Rp, Ri.data());
-        SDT_HIP_CHECK(hipMemcpyAsync(dS.get(), Ri.data(), sizeof(double) * b * b,
-                                     hipMemcpyHostToDevice, s));
-        StageScope sc(timer, 6, s);
-        tsmm(n, b, 1, W, 0, dS.get(), 1.0, 0.0, tmp, s);
-        SDT_HIP_CHECK(hipMemcpyAsync(W, tmp, sizeof(double) * n * b, hipMemcpyDeviceToDevice, s));
+namespace {
+
+// host state of one column group
+struct LzGroup {
+    int c0 = 0, b = 0;
+    bool done = false;
+    int it = -1;                  // iteration of convergence, -1 if none
+    int start = 1;                // first iteration whose criterion is evaluated
+    double normz = 0.0;           // ||Z_g||_F
+    std::vector<double> R, H, B;  // b x b row-major: Z = V_0 R, H_i, B_i
+    std::vector<double> Tb;       // block tridiagonal T in band storage
+    std::vector<double> S, D, Dprev, Dfin;
+    int d_iter = -1;              // iteration that D belongs to
+    bool met = false;             // the last evaluation met the tolerance
+    BandSqrt bsq;
+};
+
+// D_i = T_i^{1/2} E_1 R and the reference's criterion
+// ||Y_i - Y_{i-1}||_F / ||Z||_F < tol (src/brwn/brwn_lanczos.cc:179-207),
+// where Y_i - Y_{i-1} = V (D_i - [D_{i-1}; 0]) and V has orthonormal columns.
+void lz_eval(LzGroup& g, int iter, double tol) {
+    const int b = g.b, m = (iter + 1) * b;
+    g.S.resize((size_t)m * b);
+    g.bsq.compute_band(m, b, g.Tb.data(), 2 * b + 1, b, g.S.data());
+    std::swap(g.Dprev, g.D);
+    const int prev_iter = g.d_iter;
+    g.D.assign((size_t)m * b, 0.0);
+    for (int i = 0; i < m; i++)
+        for (int c = 0; c < b; c++) {
+            double acc = 0.0;
+            for (int k = 0; k <= c; k++) acc += g.S[(size_t)i * b + k] * g.R[k * b + c];
+            g.D[(size_t)i * b + c] = acc;
+        }
+    g.d_iter = iter;
+    g.met = false;
+    if (prev_iter == iter - 1 && iter >= g.start) {
+        double dn = 0.0;
+        for (size_t t = 0; t < g.D.size(); t++) {
+            const double prev = t < g.Dprev.size() ? g.Dprev[t] : 0.0;
+            const double d = g.D[t] - prev;
+            dn += d * d;
+        }
+        g.met = sqrt(dn) / g.normz < tol;
     }
-    mul_upper(b, R2.data(), R1.data(), R);
-    return true;
 }
 
-int BrwnLanczosGpu::block_lanczos_packed(MobSpmeGpu* mob, int b, const double* Z, double* Y) {
-    const long long n = dim_;
-    hipStream_t s = mob->stream();
-    const size_t blk = (size_t)n * b;
-    double* V = d_V_.get();
-    const double normz = sqrt(sumsq(n * b, Z, d_part_, s));
-    SDT_HIP_CHECK(hipMemcpyAsync(V, Z, sizeof(double) * blk, hipMemcpyDeviceToDevice, s));
-    std::vector<double> R(b * b), H(b * b), B(b * b), Bt(b * b);
-    if (!cholqr2(n, b, V, d_C_.get(), R.data(), d_G_, d_S_, d_part_, s, timer))
-        throw Error{SDT_ECONVERGE, "block Lanczos: the right-hand sides are rank deficient"};
-    const int mmax = (max_iters_ + 1) * b;
-    std::vector<double> T((size_t)mmax * mmax, 0.0), Tm, Sm, D, Dold;
-    int iter;
-    bool converged = false;
-    for (iter = 0; iter < max_iters_; iter++) {
-        double* Vi = V + (size_t)iter * blk;
-        double* W = V + (size_t)(iter + 1) * blk;
-        double beta = 0.0;
-        if (iter > 0) {
-            // W = V_{i-1} B_{i-1}^T
-            for (int i = 0; i < b; i++)
-                for (int j = 0; j < b; j++) Bt[i * b + j] = B[j * b + i];
-            SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), Bt.data(), sizeof(double) * b * b,
-                                         hipMemcpyHostToDevice, s));
-            StageScope sc(timer, 6, s);
-            tsmm(n, b, 1, V + (size_t)(iter - 1) * blk, 0, d_S_.get(), 1.0, 0.0, W, s);
-            beta = -1.0;
-        }
-        // W = M V_i - V_{i-1} B^T
-        mob->apply_packed(0, b, 1.0, Vi, beta, W);
-        // H = V_i^T W ; W -= V_i H
-        { StageScope sc(timer, 6, s); gram(n, b, Vi, W, d_G_.get(), d_part_, s); }
-        SDT_HIP_CHECK(hipMemcpyAsync(H.data(), d_G_.get(), sizeof(double) * b * b,
+// runs f(0 .. n-1) on up to 16 host threads
+template <typename F>
+void parallel_for(int n, F f) {
+    const int nt = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
+    if (nt == 1) {
+        for (int i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            for (int i = t; i < n; i += nt) f(i);
+        });
+    for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+// CholQR2 of every column group of the packed n x btot block W in place:
+// W_g = Q_g R_g, R_g = R2 R1 stored into (g.*dst). Done groups are zeroed.
+// Returns false in ok[g] when W_g is numerically rank deficient (the group's
+// columns are then zeroed). trace (optional) receives tr(W_g^T W_g).
+static void cholqr2_groups(long long n, int btot, int bs, double* W, double* tmp,
+                           std::vector<LzGroup>& grp, std::vector<double> LzGroup::*dst,
+                           std::vector<char>& ok, std::vector<double>* trace, DevBuf<double>& dG,
+                           DevBuf<double>& dS, DevBuf<double>& part, hipStream_t s,
+                           StageTimer* timer) {
+    const int ng = (int)grp.size();
+    std::vector<double> G((size_t)ng * 256), hS((size_t)ng * bs * bs);
+    std::vector<std::vector<double>> R1(ng);
+    ok.assign(ng, 1);
+    for (int pass = 0; pass < 2; pass++) {
+        { StageScope sc(timer, 6, s); gram_groups(n, btot, bs, W, W, dG.get(), part, s); }
+        SDT_HIP_CHECK(hipMemcpyAsync(G.data(), dG.get(), sizeof(double) * G.size(),
                                      hipMemcpyDeviceToHost, s));
         SDT_HIP_CHECK(hipStreamSynchronize(s));
-        SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), H.data(), sizeof(double) * b * b,
-                                     hipMemcpyHostToDevice, s));
-        { StageScope sc(timer, 6, s); tsmm(n, b, 1, Vi, 0, d_S_.get(), -1.0, 1.0, W, s); }
-        if (reorth) {
-            // full reorthogonalisation against V_0..V_i (classical Gram-Schmidt)
-            std::vector<double> C((size_t)(iter + 1) * b * b);
-            for (int k = 0; k <= iter; k++) {
-                gram(n, b, V + (size_t)k * blk, W, d_G_.get(), d_part_, s);
-                SDT_HIP_CHECK(hipMemcpyAsync(&C[(size_t)k * b * b], d_G_.get(),
-                                             sizeof(double) * b * b, hipMemcpyDeviceToHost, s));
-            }
-            SDT_HIP_CHECK(hipStreamSynchronize(s));
-            int k0 = 0;
-            while (k0 <= iter) {
-                int nk = std::min(iter + 1 - k0, 8192 / (b * b));
-                SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), &C[(size_t)k0 * b * b],
-                                             sizeof(double) * nk * b * b, hipMemcpyHostToDevice, s));
-                tsmm(n, b, nk, V + (size_t)k0 * blk, (long long)blk, d_S_.get(), -1.0, 1.0, W, s);
-                k0 += nk;
-            }
-        }
-        // [V_{i+1}, B_i] = QR(W)
-        if (!cholqr2(n, b, W, d_C_.get(), B.data(), d_G_, d_S_, d_part_, s, timer)) {
-            // invariant subspace found: the current iterate is exact
-            converged = true;
-            if (iter == 0) {
-                // T = H only
-            }
-        }
-        const int o = iter * b;
-        for (int c = 0; c < b; c++)
-            for (int a = c; a < b; a++) {
-                // H(a, c) for a >= c, column-major reading of the row-major
-                // Gram G[a][c] = (V_i^T W)[a][c]
-                const double hv = H[a * b + c];
-                T[(size_t)(o + a) * mmax + o + c] = hv;
-                T[(size_t)(o + c) * mmax + o + a] = hv;
-            }
-        if (!converged) {
-            for (int a = 0; a < b; a++)
-                for (int c = a; c < b; c++) {
-                    const double bv = B[a * b + c];   // B(a, c), upper
-                    T[(size_t)(o + b + a) * mmax + o + c] = bv;
-                    T[(size_t)(o + c) * mmax + o + b + a] = bv;
+        std::fill(hS.begin(), hS.end(), 0.0);
+        for (int gi = 0; gi < ng; gi++) {
+            LzGroup& g = grp[gi];
+            const int b = g.b;
+            if (g.done || !ok[gi]) continue;
+            std::vector<double> Gg((size_t)b * b), Rp((size_t)b * b), Ri((size_t)b * b);
+            for (int i = 0; i < b; i++)
+                for (int j = 0; j < b; j++) {
+                    // symmetrise
+                    Gg[i * b + j] = 0.5 * (G[(size_t)gi * 256 + i * 16 + j] +
+                                           G[(size_t)gi * 256 + j * 16 + i]);
                 }
+            if (pass == 0 && trace) {
+                double tr = 0.0;
+                for (int i = 0; i < b; i++) tr += Gg[i * b + i];
+                (*trace)[gi] = tr;
+            }
+            if (!chol_upper(b, Gg.data(), Rp.data())) {
+                ok[gi] = 0;
+                continue;
+            }
+            inv_upper(b, Rp.data(), Ri.data());
+            for (int i = 0; i < b; i++)
+                for (int j = 0; j < b; j++) hS[((size_t)gi * bs + i) * bs + j] = Ri[i * b + j];
+            if (pass == 0) R1[gi] = Rp;
+            else mul_upper(b, Rp.data(), R1[gi].data(), (g.*dst).data());
         }
-        const int m = (iter + 1) * b;
-        Tm.resize((size_t)m * m);
-        for (int i = 0; i < m; i++)
-            memcpy(&Tm[(size_t)i * m], &T[(size_t)i * mmax], sizeof(double) * m);
-        Sm.resize((size_t)m * b);
-        bsq_.compute(m, b, Tm.data(), b, Sm.data());
-        // D = S R
-        D.assign((size_t)m * b, 0.0);
-        for (int i = 0; i < m; i++)
-            for (int c = 0; c < b; c++) {
-                double acc = 0.0;
-                for (int k = 0; k <= c; k++) acc += Sm[(size_t)i * b + k] * R[k * b + c];
-                D[(size_t)i * b + c] = acc;
-            }
-        if (converged) break;
-        if (iter > 0) {
-            // ||Y_i - Y_{i-1}||_F = ||D_i - [D_{i-1}; 0]||_F for orthonormal V
-            double dn = 0.0;
-            for (size_t t = 0; t < D.size(); t++) {
-                double prev = t < Dold.size() ? Dold[t] : 0.0;
-                double d = D[t] - prev;
-                dn += d * d;
-            }
-            if (sqrt(dn) / normz < tol_) {
-                converged = true;
-                break;
-            }
-        }
-        Dold = D;
-    }
-    const int used = (iter < max_iters_) ? iter + 1 : max_iters_;
-    // Y = [V_0 .. V_{used-1}] D, in chunks that fit the LDS coefficient tile
-    const int per = std::max(1, 8192 / (b * b));
-    for (int k0 = 0; k0 < used; k0 += per) {
-        const int nk = std::min(per, used - k0);
-        SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), &D[(size_t)k0 * b * b], sizeof(double) * nk * b * b,
+        SDT_HIP_CHECK(hipMemcpyAsync(dS.get(), hS.data(), sizeof(double) * hS.size(),
                                      hipMemcpyHostToDevice, s));
         StageScope sc(timer, 6, s);
-        tsmm(n, b, nk, V + (size_t)k0 * blk, (long long)blk, d_S_.get(), 1.0, k0 == 0 ? 0.0 : 1.0,
-             Y, s);
+        tsmm_groups(n, btot, bs, 1, W, 0, dS.get(), 1.0, 0.0, tmp, s);
+        SDT_HIP_CHECK(hipMemcpyAsync(W, tmp, sizeof(double) * n * btot, hipMemcpyDeviceToDevice, s));
+    }
+}
+
+void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const double* Z,
+                                   double* Y, std::vector<int>& iters) {
+    const long long n = dim_;
+    hipStream_t s = mob->stream();
+    const int bs = max_nrhs_;
+    const int ng = (btot + bs - 1) / bs;
+    const size_t blk = (size_t)n * btot;
+    d_V_.reserve((size_t)(max_iters_ + 1) * blk);
+    d_C_.reserve(blk);
+    d_G_.reserve((size_t)ng * 256);
+    double* V = d_V_.get();
+    const size_t sset = (size_t)ng * bs * bs;   // one coefficient block per group
+    std::vector<double> hS(sset), hG((size_t)ng * 256);
+    std::vector<LzGroup> grp(ng);
+    for (int gi = 0; gi < ng; gi++) {
+        LzGroup& g = grp[gi];
+        g.c0 = gi * bs;
+        g.b = std::min(bs, btot - g.c0);
+        g.R.assign((size_t)g.b * g.b, 0.0);
+        g.H.assign((size_t)g.b * g.b, 0.0);
+        g.B.assign((size_t)g.b * g.b, 0.0);
+        g.Tb.assign((size_t)(max_iters_ + 1) * g.b * (2 * g.b + 1), 0.0);
+        const int hint = (gbase + gi < (int)hint_.size()) ? hint_[gbase + gi] : 0;
+        g.start = std::max(1, hint - kHintMargin);
+    }
+    SDT_HIP_CHECK(hipMemcpyAsync(V, Z, sizeof(double) * blk, hipMemcpyDeviceToDevice, s));
+    std::vector<char> ok;
+    std::vector<double> tr(ng, 0.0);
+    cholqr2_groups(n, btot, bs, V, d_C_.get(), grp, &LzGroup::R, ok, &tr, d_G_, d_S_, d_part_, s,
+                   timer);
+    for (int gi = 0; gi < ng; gi++) {
+        if (!ok[gi])
+            throw Error{SDT_ECONVERGE, "block Lanczos: the right-hand sides are rank deficient"};
+        grp[gi].normz = sqrt(tr[gi]);
+    }
+    std::future<void> task;
+    int task_iter = -1;
+    auto collect = [&]() {
+        if (!task.valid()) return;
+        task.get();
+        for (LzGroup& g : grp)
+            if (!g.done && g.d_iter == task_iter && g.met) {
+                g.done = true;
+                g.it = task_iter;
+                g.Dfin = g.D;
+            }
+    };
+    auto upload_S = [&](const std::vector<double>& h) {
+        SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), h.data(), sizeof(double) * h.size(),
+                                     hipMemcpyHostToDevice, s));
+    };
+    int iter;
+    try {
+        for (iter = 0; iter < max_iters_; iter++) {
+            double* Vi = V + (size_t)iter * blk;
+            double* W = V + (size_t)(iter + 1) * blk;
+            double beta = 0.0;
+            if (iter > 0) {
+                // W = V_{i-1} B_{i-1}^T
+                std::fill(hS.begin(), hS.end(), 0.0);
+                for (int gi = 0; gi < ng; gi++) {
+                    const LzGroup& g = grp[gi];
+                    if (g.done) continue;
+                    for (int i = 0; i < g.b; i++)
+                        for (int j = 0; j < g.b; j++)
+                            hS[((size_t)gi * bs + i) * bs + j] = g.B[j * g.b + i];
+                }
+                upload_S(hS);
+                StageScope sc(timer, 6, s);
+                tsmm_groups(n, btot, bs, 1, V + (size_t)(iter - 1) * blk, 0, d_S_.get(), 1.0, 0.0,
+                            W, s);
+                beta = -1.0;
+            }
+            // W = M V_i - V_{i-1} B^T, one application for every column
+            mob->apply_packed(0, btot, 1.0, Vi, beta, W);
+            // H = V_i^T W ; W -= V_i H
+            { StageScope sc(timer, 6, s); gram_groups(n, btot, bs, Vi, W, d_G_.get(), d_part_, s); }
+            SDT_HIP_CHECK(hipMemcpyAsync(hG.data(), d_G_.get(), sizeof(double) * hG.size(),
+                                         hipMemcpyDeviceToHost, s));
+            SDT_HIP_CHECK(hipStreamSynchronize(s));
+            std::fill(hS.begin(), hS.end(), 0.0);
+            for (int gi = 0; gi < ng; gi++) {
+                LzGroup& g = grp[gi];
+                if (g.done) continue;
+                for (int i = 0; i < g.b; i++)
+                    for (int j = 0; j < g.b; j++) {
+                        g.H[i * g.b + j] = hG[(size_t)gi * 256 + i * 16 + j];
+                        hS[((size_t)gi * bs + i) * bs + j] = g.H[i * g.b + j];
+                    }
+            }
+            upload_S(hS);
+            { StageScope sc(timer, 6, s); tsmm_groups(n, btot, bs, 1, Vi, 0, d_S_.get(), -1.0, 1.0, W, s); }
+            if (reorth) {
+                // full reorthogonalisation against V_0 .. V_i (classical
+                // Gram-Schmidt, one pass)
+                std::vector<double> C((size_t)(iter + 1) * ng * 256);
+                for (int k = 0; k <= iter; k++) {
+                    gram_groups(n, btot, bs, V + (size_t)k * blk, W, d_G_.get(), d_part_, s);
+                    SDT_HIP_CHECK(hipMemcpyAsync(&C[(size_t)k * ng * 256], d_G_.get(),
+                                                 sizeof(double) * ng * 256, hipMemcpyDeviceToHost,
+                                                 s));
+                }
+                SDT_HIP_CHECK(hipStreamSynchronize(s));
+                const int per = std::max(1, 8192 / (int)sset);
+                for (int k0 = 0; k0 <= iter; k0 += per) {
+                    const int nk = std::min(per, iter + 1 - k0);
+                    std::vector<double> hC((size_t)nk * sset, 0.0);
+                    for (int gi = 0; gi < ng; gi++) {
+                        if (grp[gi].done) continue;
+                        for (int k = 0; k < nk; k++)
+                            for (int i = 0; i < bs; i++)
+                                for (int j = 0; j < bs; j++)
+                                    hC[(((size_t)gi * nk + k) * bs + i) * bs + j] =
+                                        C[((size_t)(k0 + k) * ng + gi) * 256 + i * 16 + j];
+                    }
+                    upload_S(hC);
+                    tsmm_groups(n, btot, bs, nk, V + (size_t)k0 * blk, (long long)blk, d_S_.get(),
+                                -1.0, 1.0, W, s);
+                }
+            }
+            // [V_{i+1}, B_i] = QR(W)
+            cholqr2_groups(n, btot, bs, W, d_C_.get(), grp, &LzGroup::B, ok, nullptr, d_G_, d_S_,
+                           d_part_, s, timer);
+            // T: H_i on the diagonal, B_i below it (and B_i^T above)
+            for (int gi = 0; gi < ng; gi++) {
+                LzGroup& g = grp[gi];
+                if (g.done) continue;
+                const int b = g.b, o = iter * b, lt = 2 * b + 1;
+                auto T = [&](int r, int c) -> double& { return g.Tb[(size_t)r * lt + (c - r + b)]; };
+                for (int c = 0; c < b; c++)
+                    for (int a = c; a < b; a++) {
+                        // H(a, c) for a >= c from the row-major Gram (V_i^T W)[a][c]
+                        const double hv = g.H[a * b + c];
+                        T(o + a, o + c) = hv;
+                        T(o + c, o + a) = hv;
+                    }
+                if (ok[gi])
+                    for (int a = 0; a < b; a++)
+                        for (int c = a; c < b; c++) {
+                            const double bv = g.B[a * b + c];   // B(a, c), upper
+                            T(o + b + a, o + c) = bv;
+                            T(o + c, o + b + a) = bv;
+                        }
+            }
+            collect();   // the evaluation of iteration iter - 1
+            // an invariant subspace was found: the current iterate is exact
+            for (int gi = 0; gi < ng; gi++) {
+                LzGroup& g = grp[gi];
+                if (g.done || ok[gi]) continue;
+                lz_eval(g, iter, tol_);
+                g.done = true;
+                g.it = iter;
+                g.Dfin = g.D;
+            }
+            bool all = true;
+            for (const LzGroup& g : grp) all = all && g.done;
+            if (all) break;
+            std::vector<int> todo;
+            for (int gi = 0; gi < ng; gi++)
+                if (!grp[gi].done && iter >= grp[gi].start - 1) todo.push_back(gi);
+            if (!todo.empty()) {
+                const double tol = tol_;
+                task = std::async(std::launch::async, [&grp, todo, iter, tol] {
+                    parallel_for((int)todo.size(), [&](int t) { lz_eval(grp[todo[t]], iter, tol); });
+                });
+                task_iter = iter;
+            }
+        }
+        collect();
+    } catch (...) {
+        if (task.valid()) task.wait();
+        throw;
+    }
+    // groups that did not converge keep the last iterate (brwn_lanczos.cc:208-221)
+    for (LzGroup& g : grp) {
+        if (g.done) continue;
+        if (g.d_iter != max_iters_ - 1) lz_eval(g, max_iters_ - 1, tol_);
+        g.Dfin = g.D;
+        g.it = -1;
+    }
+    int used = 0;
+    for (const LzGroup& g : grp)
+        used = std::max(used, g.it >= 0 ? g.it + 1 : max_iters_);
+    // Y = [V_0 .. V_{used-1}] D, in chunks that fit the LDS coefficient tile
+    const int per = std::max(1, 8192 / (int)sset);
+    for (int k0 = 0; k0 < used; k0 += per) {
+        const int nk = std::min(per, used - k0);
+        std::vector<double> hD((size_t)nk * sset, 0.0);
+        for (int gi = 0; gi < ng; gi++) {
+            const LzGroup& g = grp[gi];
+            const int rows = (int)(g.Dfin.size() / g.b);
+            for (int k = 0; k < nk; k++)
+                for (int i = 0; i < g.b; i++) {
+                    const int r = (k0 + k) * g.b + i;
+                    if (r >= rows) continue;
+                    for (int j = 0; j < g.b; j++)
+                        hD[(((size_t)gi * nk + k) * bs + i) * bs + j] = g.Dfin[(size_t)r * g.b + j];
+                }
+        }
+        upload_S(hD);
+        StageScope sc(timer, 6, s);
+        tsmm_groups(n, btot, bs, nk, V + (size_t)k0 * blk, (long long)blk, d_S_.get(), 1.0,
+                    k0 == 0 ? 0.0 : 1.0, Y, s);
     }
     SDT_HIP_CHECK(hipStreamSynchronize(s));
-    return converged ? iter : -1;
+    for (int gi = 0; gi < ng; gi++) {
+        iters.push_back(grp[gi].it);
+        if ((int)hint_.size() <= gbase + gi) hint_.resize(gbase + gi + 1, 0);
+        hint_[gbase + gi] = grp[gi].it >= 0 ? grp[gi].it : 0;
+    }
 }
 
 void BrwnLanczosGpu::compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double* z, int ldy,
@@ -358,8 +539,16 @@ void BrwnLanczosGpu::compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double
     SDT_REQUIRE(num_rhs >= 1 && ldz >= dim_ && ldy >= dim_, SDT_EINVAL, "bad shapes");
     hipStream_t s = mob->stream();
     const int npos = dim_ / 3;
-    d_Zp_.reserve((size_t)dim_ * max_nrhs_);
-    d_Yp_.reserve((size_t)dim_ * max_nrhs_);
+    // columns per lockstep batch: whole groups, at most kMaxBatchCols, and a
+    // Krylov basis within half of the free device memory
+    size_t free_b = 0, total_b = 0;
+    SDT_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const size_t col_bytes = sizeof(double) * (size_t)dim_ * (max_iters_ + 4);
+    int cols = (int)std::min<size_t>(kMaxBatchCols, (free_b / 2 + d_V_.bytes()) / col_bytes);
+    cols = std::max(max_nrhs_, cols / max_nrhs_ * max_nrhs_);
+    const int bcols = std::min(cols, (num_rhs + max_nrhs_ - 1) / max_nrhs_ * max_nrhs_);
+    d_Zp_.reserve((size_t)dim_ * bcols);
+    d_Yp_.reserve((size_t)dim_ * bcols);
     const double* dz = z;
     double* dy = y;
     if (!dev) {
@@ -371,12 +560,11 @@ void BrwnLanczosGpu::compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double
         dy = d_io_y_.get();
     }
     iters_.clear();
-    for (int irhs = 0; irhs < num_rhs; irhs += max_nrhs_) {
-        const int b = std::min(max_nrhs_, num_rhs - irhs);
-        pack(npos, b, mob->perm(), ldz, dz + (size_t)irhs * ldz, d_Zp_.get(), s);
-        int it = block_lanczos_packed(mob, b, d_Zp_.get(), d_Yp_.get());
-        unpack(npos, b, mob->perm(), ldy, d_Yp_.get(), dy + (size_t)irhs * ldy, s);
-        iters_.push_back(it);
+    for (int irhs = 0; irhs < num_rhs; irhs += bcols) {
+        const int btot = std::min(bcols, num_rhs - irhs);
+        pack(npos, btot, mob->perm(), ldz, dz + (size_t)irhs * ldz, d_Zp_.get(), s);
+        lanczos_batch(mob, btot, irhs / max_nrhs_, d_Zp_.get(), d_Yp_.get(), iters_);
+        unpack(npos, btot, mob->perm(), ldy, d_Yp_.get(), dy + (size_t)irhs * ldy, s);
     }
     if (!dev)
         SDT_HIP_CHECK(hipMemcpy2DAsync(y, sizeof(double) * ldy, d_io_y_.get(), sizeof(double) * ldy,

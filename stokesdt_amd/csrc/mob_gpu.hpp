@@ -67,23 +67,29 @@ class BrwnLanczosGpu {
     BrwnLanczosGpu(int dim, int max_iters, int max_nrhs, double tol);
     void init();
     // reference layout z/y (host or device); returns the iteration count of
-    // every group in iters()
+    // every group of max_nrhs columns in iters()
     void compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double* z, int ldy, double* y,
                  bool dev);
-    // packed layout: Z, Y are row-major 3N x b in the mobility's sorted order
-    int block_lanczos_packed(MobSpmeGpu* mob, int b, const double* Z, double* Y);
     const std::vector<int>& iters() const { return iters_; }
     int max_nrhs() const { return max_nrhs_; }
     int reorth = 0;        // 0: none (reference); 1: full FP64 reorthogonalisation
     StageTimer* timer = nullptr;
+    // the convergence test of a group starts this many iterations before the
+    // count the same group needed in the previous call
+    static constexpr int kHintMargin = 3;
+    // columns of one lockstep batch
+    static constexpr int kMaxBatchCols = 128;
 
   private:
+    // lockstep block Lanczos of the groups of a packed 3N x btot block (row
+    // major, sorted particle order); gbase is the index of its first group
+    void lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const double* Z, double* Y,
+                       std::vector<int>& iters);
     int dim_, max_iters_, max_nrhs_;
     double tol_;
     bool inited_ = false;
-    std::vector<int> iters_;
+    std::vector<int> iters_, hint_;
     DevBuf<double> d_V_, d_G_, d_S_, d_part_, d_io_z_, d_io_y_, d_Zp_, d_Yp_, d_C_;
-    BandSqrt bsq_;
 };
 
 // Steric force (src/force/force_steric.cc) on the S pair list.

@@ -65,6 +65,7 @@ struct DevBuf {
         n = count;
     }
     T* get() const { return ptr; }
+    size_t bytes() const { return n * sizeof(T); }
 };
 
 template <typename T>

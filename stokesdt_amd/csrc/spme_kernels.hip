@@ -9,6 +9,7 @@
 // global atomics, no zero-fill pass), and fuses the influence multiply
 // between the forward and inverse z-transforms.
 #include <cmath>
+#include <cstring>
 #include "fft_lds.hpp"
 #include "spme_gpu.hpp"
 
@@ -77,144 +78,262 @@ __global__ void spme_gather_kernel(int npos, const int* __restrict__ perm,
 }
 
 // ------------------------------------------------------------- spread ------
-// One workgroup owns one mesh tile of side TS for RB right-hand sides and
-// NC components (3 mono; 6 poly: f and a^2 f). It gathers the particles
-// whose base point lies in the tile or in the 7 tiles below it (TS >= P-1
-// guarantees no other tile reaches it), accumulates their clipped stencils
-// in LDS with ds_add atomics, and stores the tile.
-constexpr int kSpreadChunk = 64;
+// Gather form: each workgroup owns a super-tile of 4 ts x ts x ts mesh tiles
+// along x (one per wave) for RB right-hand sides. Lane l of a wave owns the
+// x-row (y = l % ts, z = l / ts) of its tile and accumulates its ts points
+// in registers. The particles whose B-spline stencils reach the super-tile
+// (base points in the 4+1 x 2 x 2 surrounding tiles, filtered by overlap) are
+// staged in LDS with their weights and forces; every lane then walks the
+// staged list. No atomics: every mesh point is written once, by its owner.
+constexpr int kStageChunk = 128;
 
-template <typename T, int P>
+// exclusive prefix over the workgroup of a 0/1 predicate; returns this
+// thread's slot and the total (all threads must call)
+__device__ __forceinline__ int wg_compact(bool pred, int* s_wave, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(pred);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wave[wave] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; w++) {
+        const int c = s_wave[w];
+        off += (w < wave) ? c : 0;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + before;
+}
+
+__device__ __forceinline__ int wrapk(int d, int K) {
+    d += (d < 0) ? K : 0;
+    return d - ((d >= K) ? K : 0);
+}
+
+template <typename T, int P, int RBNC>
 __global__ void __launch_bounds__(256)
-spread_tile_kernel(int K, int ts, int ntile, const int* __restrict__ tile_start,
-                   const T* __restrict__ frac, const int* __restrict__ base,
-                   const double* __restrict__ a2, const double* __restrict__ X, int b, int RB,
-                   int NC, T* __restrict__ mesh) {
+spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile_start,
+                     const T* __restrict__ frac, const int* __restrict__ base,
+                     const double* __restrict__ a2, const double* __restrict__ X, int b, int RB,
+                     int NC, int cap, T* __restrict__ mesh) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int tile = blockIdx.x;
-    const int tx = tile % ntile, ty = (tile / ntile) % ntile, tz = tile / (ntile * ntile);
+    constexpr int REC = 3 * P + 3 + RBNC;      // wx[P] wy[P] wz[P] bx by bz f[RBNC]
+    T* stage = reinterpret_cast<T*>(smem_raw);
+    __shared__ int s_wave[4];
+    __shared__ int s_off[21];                   // candidate offsets of the base tiles
+    __shared__ int s_beg[20];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // block -> (super-tile x, tile y, tile z), rhs chunk
+    const int sxi = blockIdx.x % nsx;
+    const int tyz = blockIdx.x / nsx;
+    const int ty = tyz % ntile, tz = tyz / ntile;
     const int r0 = blockIdx.y * RB;
     const int rb = min(RB, b - r0);
-    const int t0x = tx * ts, t0y = ty * ts, t0z = tz * ts;
-    const int sy = ts + 1;                 // padded LDS strides
-    const int sz = ts * sy + 1;
-    const int tile_words = ts * sz;
-    const int nacc = RB * NC * tile_words;
-    T* acc = reinterpret_cast<T*>(smem_raw);
-    T* s_w = acc + nacc;                               // [CH][3][P]
-    T* s_f = s_w + kSpreadChunk * 3 * P;               // [CH][RB*NC]
-    int* s_b = reinterpret_cast<int*>(s_f + kSpreadChunk * RB * NC);   // [CH][3]
+    const int rbnc = rb * NC;
+    const int stx = sxi * 4;                    // first x tile of the super-tile
+    const int nxt = min(4, ntile - stx);        // x tiles present
+    const int sx0 = stx * ts, sxlen = nxt * ts;
+    const int sy0 = ty * ts, sz0 = tz * ts;
+    // this lane's row and x range
+    const int wx0 = sx0 + wave * ts;
+    const bool wave_on = wave < nxt;
+    const bool lane_on = wave_on && lane < ts * ts;
+    const int yl = sy0 + (lane % ts), zl = sz0 + (lane / ts);
+    T acc[8][RBNC];
+#pragma unroll
+    for (int x = 0; x < 8; x++)
+#pragma unroll
+        for (int q = 0; q < RBNC; q++) acc[x][q] = T(0);
 
-    for (int i = threadIdx.x; i < nacc; i += blockDim.x) acc[i] = T(0);
-    __syncthreads();
-
-    const long long K3 = (long long)K * K * K;
-    for (int nbr = 0; nbr < 8; nbr++) {
-        const int dx = nbr & 1, dy = (nbr >> 1) & 1, dz = (nbr >> 2) & 1;
-        const int sx = (tx - dx + ntile) % ntile;
-        const int syy = (ty - dy + ntile) % ntile;
-        const int szz = (tz - dz + ntile) % ntile;
-        const int src = (szz * ntile + syy) * ntile + sx;
-        const int pbeg = tile_start[src], pend = tile_start[src + 1];
-        for (int c0 = pbeg; c0 < pend; c0 += kSpreadChunk) {
-            const int cnt = min(kSpreadChunk, pend - c0);
-            // stage weights, base points and forces of the chunk
-            for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-                const int is = c0 + t;
-                T w[P];
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    bspline_weights<T, P>(frac[3 * is + d], w);
-#pragma unroll
-                    for (int k = 0; k < P; k++) s_w[(t * 3 + d) * P + k] = w[k];
-                    s_b[t * 3 + d] = base[3 * is + d];
-                }
-                const T sc = (NC == 6) ? (T)a2[is] : T(1);
-                for (int r = 0; r < rb; r++) {
-                    for (int c = 0; c < 3; c++) {
-                        T f = (T)X[(size_t)(3 * is + c) * b + r0 + r];
-                        s_f[t * RB * NC + r * NC + c] = f;
-                        if (NC == 6) s_f[t * RB * NC + r * NC + 3 + c] = f * sc;
-                    }
-                }
-            }
-            __syncthreads();
-            // (particle, iy, iz) stencil lines, particle index fastest
-            const int nitems = cnt * P * P;
-            for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
-                const int t = item % cnt;
-                const int line = item / cnt;
-                const int iy = line % P, iz = line / P;
-                int ly = s_b[t * 3 + 1] + iy - t0y;
-                ly += (ly < 0) ? K : 0;
-                ly -= (ly >= K) ? K : 0;
-                int lz = s_b[t * 3 + 2] + iz - t0z;
-                lz += (lz < 0) ? K : 0;
-                lz -= (lz >= K) ? K : 0;
-                if (ly >= ts || lz >= ts) continue;
-                const T wyz = s_w[(t * 3 + 1) * P + iy] * s_w[(t * 3 + 2) * P + iz];
-                const int bx = s_b[t * 3 + 0];
-                const T* fp = s_f + t * RB * NC;
-#pragma unroll
-                for (int ix = 0; ix < P; ix++) {
-                    int lx = bx + ix - t0x;
-                    lx += (lx < 0) ? K : 0;
-                    lx -= (lx >= K) ? K : 0;
-                    if (lx >= ts) continue;
-                    const T w = wyz * s_w[(t * 3 + 0) * P + ix];
-                    const int off = lz * sz + ly * sy + lx;
-                    for (int rc = 0; rc < rb * NC; rc++)
-                        atomicAdd(&acc[rc * tile_words + off], w * fp[rc]);
-                }
-            }
-            __syncthreads();
+    // contributor base tiles: x tiles stx-1 .. stx+nxt-1, y ty-1..ty, z tz-1..tz
+    const int nbx = min(nxt + 1, ntile);
+    const int nbt = 4 * nbx;
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int k = 0; k < nbt; k++) {
+            const int jx = k % nbx, jy = (k / nbx) & 1, jz = k / (2 * nbx);
+            const int btz = (tz - 1 + jz + ntile) % ntile;
+            const int bty = (ty - 1 + jy + ntile) % ntile;
+            const int btx = (stx - 1 + jx + ntile) % ntile;
+            const int bt = (btz * ntile + bty) * ntile + btx;
+            const int pb = tile_start[bt], pe = tile_start[bt + 1];
+            s_beg[k] = pb;
+            s_off[k] = off;
+            off += pe - pb;
         }
+        s_off[nbt] = off;
     }
-    // store the tile (rows of ts contiguous points)
-    const int npts = ts * ts * ts;
-    for (int rc = 0; rc < rb * NC; rc++) {
-        T* m = mesh + (size_t)(r0 * NC + rc) * K3;
-        const T* a = acc + rc * tile_words;
-        for (int i = threadIdx.x; i < npts; i += blockDim.x) {
-            const int lx = i % ts, ly = (i / ts) % ts, lz = i / (ts * ts);
-            m[((size_t)(t0z + lz) * K + (t0y + ly)) * K + t0x + lx] = a[lz * sz + ly * sy + lx];
+    __syncthreads();
+    const int ncand = s_off[nbt];
+
+    auto accumulate = [&](int n) {
+        if (!lane_on) return;
+        for (int sidx = 0; sidx < n; sidx++) {
+            const T* rec = stage + (size_t)sidx * REC;
+            const int bx = (int)rec[3 * P];
+            const int dxw = wrapk(bx - wx0, K);
+            if (!(dxw < ts || K - dxw < P)) continue;          // wave-uniform
+            const int by = (int)rec[3 * P + 1], bz = (int)rec[3 * P + 2];
+            T wxr[P], f[RBNC];
+#pragma unroll
+            for (int k = 0; k < P; k++) wxr[k] = rec[k];
+#pragma unroll
+            for (int q = 0; q < RBNC; q++) f[q] = rec[3 * P + 3 + q];
+            const int ly = wrapk(yl - by, K), lz = wrapk(zl - bz, K);
+            if (ly >= P || lz >= P) continue;
+            const T wyz = rec[P + ly] * rec[2 * P + lz];
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                if (x >= ts) break;
+                const int lx = wrapk(wx0 + x - bx, K);
+                T w = T(0);
+#pragma unroll
+                for (int k = 0; k < P; k++) w = (lx == k) ? wxr[k] : w;
+                if (lx < P) {
+                    w *= wyz;
+#pragma unroll
+                    for (int q = 0; q < RBNC; q++) acc[x][q] += w * f[q];
+                }
+            }
         }
+    };
+
+    // candidates in rounds of 2 per thread: all loads of a round are in flight
+    // together; takers are compacted in candidate order (deterministic sums)
+    constexpr int U = 2;
+    int nst = 0;
+    for (int c0 = 0; c0 < ncand; c0 += U * 256) {
+        int is_u[U];
+        bool take_u[U];
+        int b3u[U][3];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int ci = c0 + u * 256 + (int)threadIdx.x;
+            is_u[u] = -1;
+            take_u[u] = false;
+            if (ci < ncand) {
+                int k = 0;
+                while (k + 1 < nbt && s_off[k + 1] <= ci) k++;
+                is_u[u] = s_beg[k] + (ci - s_off[k]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (is_u[u] >= 0) {
+                const int is = is_u[u];
+                b3u[u][0] = base[3 * is]; b3u[u][1] = base[3 * is + 1]; b3u[u][2] = base[3 * is + 2];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (is_u[u] >= 0) {
+                const int dx = wrapk(b3u[u][0] - sx0, K);
+                const int dy = wrapk(b3u[u][1] - sy0, K);
+                const int dz = wrapk(b3u[u][2] - sz0, K);
+                take_u[u] = (dx < sxlen || K - dx < P) && (dy < ts || K - dy < P) &&
+                            (dz < ts || K - dz < P);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            int tot;
+            const int slot = wg_compact(take_u[u], s_wave, &tot);
+            // the takers fill the stage in candidate order; a full stage is
+            // accumulated and emptied (tot may exceed the capacity)
+            for (int done = 0; done < tot;) {
+                if (nst == cap) {
+                    __syncthreads();
+                    accumulate(nst);
+                    __syncthreads();
+                    nst = 0;
+                }
+                const int take = min(cap - nst, tot - done);
+                if (take_u[u] && slot >= done && slot < done + take) {
+                    const int is = is_u[u];
+                    T* rec = stage + (size_t)(nst + slot - done) * REC;
+                    T fr[3];
+                    fr[0] = frac[3 * is]; fr[1] = frac[3 * is + 1]; fr[2] = frac[3 * is + 2];
+                    double xv[RBNC > 0 ? RBNC : 1];
+                    const T sc = (NC == 6) ? (T)a2[is] : T(1);
+                    for (int r = 0; r < rb; r++)
+                        for (int c = 0; c < 3; c++)
+                            xv[r * 3 + c] = X[(size_t)(3 * is + c) * b + r0 + r];
+                    T w[P];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        bspline_weights<T, P>(fr[d], w);
+#pragma unroll
+                        for (int k = 0; k < P; k++) rec[d * P + k] = w[k];
+                        rec[3 * P + d] = (T)b3u[u][d];
+                    }
+                    for (int r = 0; r < rb; r++) {
+                        for (int c = 0; c < 3; c++) {
+                            const T fv = (T)xv[r * 3 + c];
+                            rec[3 * P + 3 + r * NC + c] = fv;
+                            if (NC == 6) rec[3 * P + 3 + r * NC + 3 + c] = fv * sc;
+                        }
+                    }
+                    for (int q = rbnc; q < RBNC; q++) rec[3 * P + 3 + q] = T(0);
+                }
+                nst += take;
+                done += take;
+            }
+        }
+        __syncthreads();
+    }
+    accumulate(nst);
+    if (!lane_on) return;
+    const size_t K3 = (size_t)K * K * K;
+    const size_t row = ((size_t)zl * K + yl) * K + wx0;
+    for (int q = 0; q < rbnc; q++) {
+        T* m = mesh + (size_t)(r0 * NC + q) * K3 + row;
+#pragma unroll
+        for (int x = 0; x < 8; x++)
+            if (x < ts) m[x] = acc[x][q];
     }
 }
 
 // ------------------------------------------------------ x transforms -------
 // Real rows (2 per complex FFT: z = a + i b). Each workgroup takes NR rows.
+// LDS: K twiddles, then two buffers of NR / 2 pencils of stride PS.
+template <typename T>
+__device__ __forceinline__ void load_twiddles(cplx<T>* tw, const double* __restrict__ tw_g, int K) {
+    for (int i = threadIdx.x; i < K; i += blockDim.x)
+        tw[i] = cplx<T>{(T)tw_g[2 * i], (T)tw_g[2 * i + 1]};
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 fft_x_r2c_kernel(FFTArgs fa, long long nrows, const T* __restrict__ real,
                  cplx<T>* __restrict__ spec, const double* __restrict__ tw_g, int NR) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int K = fa.K, Kh = K / 2 + 1;
+    const int K = fa.K, Kh = K / 2 + 1, PS = pencil_stride(K);
     const int npen = NR / 2;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* bufa = tw + K;
-    cplx<T>* bufb = bufa + npen * K;
-    for (int i = threadIdx.x; i < K; i += blockDim.x)
-        tw[i] = cplx<T>{(T)tw_g[2 * i], (T)tw_g[2 * i + 1]};
+    cplx<T>* bufb = bufa + npen * PS;
+    load_twiddles(tw, tw_g, K);
     const long long row0 = (long long)blockIdx.x * NR;
     for (int i = threadIdx.x; i < npen * K; i += blockDim.x) {
-        const int pen = i / K, x = i - pen * K;
+        const int pen = (int)fdiv(i, fa.K_m), x = i - pen * K;
         const long long ra = row0 + 2 * pen;
         T va = T(0), vb = T(0);
         if (ra < nrows) {
             va = real[ra * K + x];
             vb = real[(ra + 1) * K + x];
         }
-        bufa[i] = cplx<T>{va, vb};
+        bufa[pen * PS + lpad(x)] = cplx<T>{va, vb};
     }
     __syncthreads();
-    cplx<T>* res = fft_pencils<T, -1>(bufa, bufb, K, npen, fa.radix, fa.nst, tw);
+    const cplx<T>* res = fft_pencils<T, -1>(bufa, bufb, npen, PS, fa, tw);
     for (int i = threadIdx.x; i < npen * Kh; i += blockDim.x) {
-        const int pen = i / Kh, k = i - pen * Kh;
+        const int pen = (int)fdiv(i, fa.Kh_m), k = i - pen * Kh;
         const long long ra = row0 + 2 * pen;
         if (ra >= nrows) continue;
-        const cplx<T> zk = res[pen * K + k];
-        const cplx<T> zm = res[pen * K + (K - k) % K];
+        const cplx<T> zk = res[pen * PS + lpad(k)];
+        const cplx<T> zm = res[pen * PS + lpad(k == 0 ? 0 : K - k)];
         const T h = T(0.5);
         spec[ra * Kh + k] = cplx<T>{h * (zk.x + zm.x), h * (zk.y - zm.y)};
         spec[(ra + 1) * Kh + k] = cplx<T>{h * (zk.y + zm.y), h * (zm.x - zk.x)};
@@ -226,16 +345,15 @@ __global__ void __launch_bounds__(256)
 fft_x_c2r_kernel(FFTArgs fa, long long nrows, const cplx<T>* __restrict__ spec,
                  T* __restrict__ real, const double* __restrict__ tw_g, int NR) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int K = fa.K, Kh = K / 2 + 1;
+    const int K = fa.K, Kh = K / 2 + 1, PS = pencil_stride(K);
     const int npen = NR / 2;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* bufa = tw + K;
-    cplx<T>* bufb = bufa + npen * K;
-    for (int i = threadIdx.x; i < K; i += blockDim.x)
-        tw[i] = cplx<T>{(T)tw_g[2 * i], (T)tw_g[2 * i + 1]};
+    cplx<T>* bufb = bufa + npen * PS;
+    load_twiddles(tw, tw_g, K);
     const long long row0 = (long long)blockIdx.x * NR;
     for (int i = threadIdx.x; i < npen * K; i += blockDim.x) {
-        const int pen = i / K, k = i - pen * K;
+        const int pen = (int)fdiv(i, fa.K_m), k = i - pen * K;
         const long long ra = row0 + 2 * pen;
         cplx<T> z{T(0), T(0)};
         if (ra < nrows) {
@@ -246,92 +364,97 @@ fft_x_c2r_kernel(FFTArgs fa, long long nrows, const cplx<T>* __restrict__ spec,
             if (k <= K / 2) z = cplx<T>{A.x - B.y, A.y + B.x};
             else z = cplx<T>{A.x + B.y, B.x - A.y};
         }
-        bufa[i] = z;
+        bufa[pen * PS + lpad(k)] = z;
     }
     __syncthreads();
-    cplx<T>* res = fft_pencils<T, +1>(bufa, bufb, K, npen, fa.radix, fa.nst, tw);
+    const cplx<T>* res = fft_pencils<T, +1>(bufa, bufb, npen, PS, fa, tw);
     for (int i = threadIdx.x; i < npen * K; i += blockDim.x) {
-        const int pen = i / K, x = i - pen * K;
+        const int pen = (int)fdiv(i, fa.K_m), x = i - pen * K;
         const long long ra = row0 + 2 * pen;
         if (ra >= nrows) continue;
-        real[ra * K + x] = res[i].x;
-        real[(ra + 1) * K + x] = res[i].y;
+        const cplx<T> v = res[pen * PS + lpad(x)];
+        real[ra * K + x] = v.x;
+        real[(ra + 1) * K + x] = v.y;
     }
 }
 
 // ------------------------------------------------------- y transform -------
-// In place along y (element stride Kh) for NP consecutive kx of one
+// In place along y (element stride Kh) for NP = 2^lnp consecutive kx of one
 // (mesh, z) plane.
 template <typename T, int SIGN>
 __global__ void __launch_bounds__(256)
-fft_y_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ tw_g, int NP) {
+fft_y_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ tw_g, int lnp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int K = fa.K, Kh = K / 2 + 1;
+    const int K = fa.K, Kh = K / 2 + 1, PS = pencil_stride(K);
+    const int NP = 1 << lnp;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* bufa = tw + K;
-    cplx<T>* bufb = bufa + NP * K;
-    for (int i = threadIdx.x; i < K; i += blockDim.x)
-        tw[i] = cplx<T>{(T)tw_g[2 * i], (T)tw_g[2 * i + 1]};
-    const int kx0 = blockIdx.x * NP;
-    const long long plane = blockIdx.y;          // mesh * K + z
+    cplx<T>* bufb = bufa + NP * PS;
+    load_twiddles(tw, tw_g, K);
+    const int nkb = (Kh + NP - 1) >> lnp;
+    const int kx0 = (int)(blockIdx.x % nkb) * NP;
+    const long long plane = blockIdx.x / nkb;    // mesh * K + z
     cplx<T>* base = spec + plane * K * Kh;
     for (int i = threadIdx.x; i < NP * K; i += blockDim.x) {
-        const int y = i / NP, j = i - y * NP;
+        const int y = i >> lnp, j = i & (NP - 1);
         const int kx = kx0 + j;
-        bufa[j * K + y] = (kx < Kh) ? base[(size_t)y * Kh + kx] : cplx<T>{T(0), T(0)};
+        bufa[j * PS + lpad(y)] = (kx < Kh) ? base[(size_t)y * Kh + kx] : cplx<T>{T(0), T(0)};
     }
     __syncthreads();
-    cplx<T>* res = fft_pencils<T, SIGN>(bufa, bufb, K, NP, fa.radix, fa.nst, tw);
+    const cplx<T>* res = fft_pencils<T, SIGN>(bufa, bufb, NP, PS, fa, tw);
     for (int i = threadIdx.x; i < NP * K; i += blockDim.x) {
-        const int y = i / NP, j = i - y * NP;
+        const int y = i >> lnp, j = i & (NP - 1);
         const int kx = kx0 + j;
-        if (kx < Kh) base[(size_t)y * Kh + kx] = res[j * K + y];
+        if (kx < Kh) base[(size_t)y * Kh + kx] = res[j * PS + lpad(y)];
     }
 }
 
 // ------------------------------------------ z transform + influence -------
-// For NP consecutive kx of one (rhs, y): forward z-FFT of all components,
-// the RPY influence tensor (src/mob/spme.cc:201-310, spme.h:114-123), inverse
-// z-FFT, store. Mono (NC = 3): B = lm2 (1 - k^2/3)(I - k k^T / k^2) applied to
-// the 3-vector. Poly (NC = 6, spreads of f and a^2 f): the reference's three
-// mesh round trips (flags 1, 2, 2) reduce to two outputs
+// For NP = 2^lnp consecutive kx of one (rhs, y): forward z-FFT of all
+// components, the RPY influence tensor (src/mob/spme.cc:201-310,
+// spme.h:114-123), inverse z-FFT, store. Mono (NC = 3): B = lm2 (1 - k^2/3)
+// (I - k k^T / k^2) applied to the 3-vector. Poly (NC = 6, spreads of f and
+// a^2 f): the reference's three mesh round trips (flags 1, 2, 2) reduce to
+// two outputs
 //   U = G (F f - k^2/6 F[a^2 f]),  W = -k^2/6 G F f,  G = lm2 (I - k k^T / k^2)
 // interpolated as interp(U) + a_i^2 interp(W).
 template <typename T>
 __global__ void __launch_bounds__(256)
 fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ tw_g,
-                       const double* __restrict__ bsp2, int NP, int NC, double box, double xi) {
+                       const double* __restrict__ bsp2, int lnp, int NC, double box, double xi) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int K = fa.K, Kh = K / 2 + 1;
+    const int K = fa.K, Kh = K / 2 + 1, PS = pencil_stride(K);
+    const int NP = 1 << lnp;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* bufa = tw + K;
-    cplx<T>* bufb = bufa + NC * NP * K;
-    for (int i = threadIdx.x; i < K; i += blockDim.x)
-        tw[i] = cplx<T>{(T)tw_g[2 * i], (T)tw_g[2 * i + 1]};
-    const int kx0 = blockIdx.x * NP;
-    const int y = blockIdx.y % K;
-    const int rhs = blockIdx.y / K;
+    cplx<T>* bufb = bufa + NC * NP * PS;
+    load_twiddles(tw, tw_g, K);
+    const int nkb = (Kh + NP - 1) >> lnp;
+    const int kx0 = (int)(blockIdx.x % nkb) * NP;
+    const int y = (int)((blockIdx.x / nkb) % K);
+    const int rhs = (int)(blockIdx.x / ((unsigned)nkb * K));
     const size_t mesh_sz = (size_t)K * K * Kh;
     cplx<T>* base = spec + (size_t)rhs * NC * mesh_sz + (size_t)y * Kh;
     const size_t zs = (size_t)K * Kh;
     for (int i = threadIdx.x; i < NC * NP * K; i += blockDim.x) {
-        const int j = i % NP;
-        const int z = (i / NP) % K;
-        const int c = i / (NP * K);
+        const int j = i & (NP - 1);
+        const int cz = i >> lnp;
+        const int c = (int)fdiv(cz, fa.K_m), z = cz - c * K;
         const int kx = kx0 + j;
-        bufa[(c * NP + j) * K + z] =
+        bufa[(c * NP + j) * PS + lpad(z)] =
             (kx < Kh) ? base[c * mesh_sz + z * zs + kx] : cplx<T>{T(0), T(0)};
     }
     __syncthreads();
-    cplx<T>* res = fft_pencils<T, -1>(bufa, bufb, K, NC * NP, fa.radix, fa.nst, tw);
+    cplx<T>* res = fft_pencils<T, -1>(bufa, bufb, NC * NP, PS, fa, tw);
     cplx<T>* oth = (res == bufa) ? bufb : bufa;
     // influence
     const T twopi_l = (T)(2.0 * M_PI / box);
     const T inv_xi2 = (T)(1.0 / (xi * xi));
     const int my = (y <= K / 2) ? y : y - K;
     const T ky = twopi_l * (T)my;
+    const int cs = NP * PS;    // component stride in LDS
     for (int i = threadIdx.x; i < NP * K; i += blockDim.x) {
-        const int j = i / K, z = i - j * K;
+        const int j = (int)fdiv(i, fa.K_m), z = i - j * K;
         const int kxi = kx0 + j;
         if (kxi >= Kh) continue;
         const int mz = (z <= K / 2) ? z : z - K;
@@ -347,23 +470,23 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
             const T rk = T(1) / sqrt(kk);
             ex = kx * rk; ey = ky * rk; ez = kz * rk;
         }
-        const int o = j * K + z;
+        const int o = j * PS + lpad(z);
         if (NC == 3) {
             const T s = m2 * (T(1) - kk / T(3));
-            cplx<T> g0 = res[o], g1 = res[NP * K + o], g2 = res[2 * NP * K + o];
+            cplx<T> g0 = res[o], g1 = res[cs + o], g2 = res[2 * cs + o];
             // (I - e e^T) g, scaled
             const T dr = ex * g0.x + ey * g1.x + ez * g2.x;
             const T di = ex * g0.y + ey * g1.y + ez * g2.y;
             res[o] = cplx<T>{s * (g0.x - ex * dr), s * (g0.y - ex * di)};
-            res[NP * K + o] = cplx<T>{s * (g1.x - ey * dr), s * (g1.y - ey * di)};
-            res[2 * NP * K + o] = cplx<T>{s * (g2.x - ez * dr), s * (g2.y - ez * di)};
+            res[cs + o] = cplx<T>{s * (g1.x - ey * dr), s * (g1.y - ey * di)};
+            res[2 * cs + o] = cplx<T>{s * (g2.x - ez * dr), s * (g2.y - ez * di)};
         } else {
             const T q = -kk / T(6);
             cplx<T> f[3], a[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                f[c] = res[c * NP * K + o];
-                a[c] = res[(3 + c) * NP * K + o];
+                f[c] = res[c * cs + o];
+                a[c] = res[(3 + c) * cs + o];
             }
             cplx<T> u[3], w[3];
 #pragma unroll
@@ -378,81 +501,128 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
             const T e3[3] = {ex, ey, ez};
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                res[c * NP * K + o] = cplx<T>{m2 * (u[c].x - e3[c] * ur), m2 * (u[c].y - e3[c] * ui)};
-                res[(3 + c) * NP * K + o] =
+                res[c * cs + o] = cplx<T>{m2 * (u[c].x - e3[c] * ur), m2 * (u[c].y - e3[c] * ui)};
+                res[(3 + c) * cs + o] =
                     cplx<T>{m2 * (w[c].x - e3[c] * wr), m2 * (w[c].y - e3[c] * wi)};
             }
         }
     }
     __syncthreads();
-    cplx<T>* out = fft_pencils<T, +1>(res, oth, K, NC * NP, fa.radix, fa.nst, tw);
+    const cplx<T>* out = fft_pencils<T, +1>(res, oth, NC * NP, PS, fa, tw);
     for (int i = threadIdx.x; i < NC * NP * K; i += blockDim.x) {
-        const int j = i % NP;
-        const int z = (i / NP) % K;
-        const int c = i / (NP * K);
+        const int j = i & (NP - 1);
+        const int cz = i >> lnp;
+        const int c = (int)fdiv(cz, fa.K_m), z = cz - c * K;
         const int kx = kx0 + j;
-        if (kx < Kh) base[c * mesh_sz + z * zs + kx] = out[(c * NP + j) * K + z];
+        if (kx < Kh) base[c * mesh_sz + z * zs + kx] = out[(c * NP + j) * PS + lpad(z)];
     }
 }
 
 // ------------------------------------------------------------- interp ------
-// One thread per (sorted particle, rhs): gathers the p^3 stencil of every
-// component (src/mob/spme.h:219-265). Y = beta Y + alpha (sum_U + a^2 sum_W).
+// A workgroup takes the particles whose base points lie in one super-tile
+// (4 ts x ts x ts tiles along x) for RB right-hand sides. For every
+// (rhs, component) it stages the mesh region their stencils cover,
+// (4 ts + P - 1) x (ts + P - 1) x (ts + P - 1) points, in LDS; items
+// (particle, z line) sum their P x P stencil slice from LDS, and each
+// particle adds its P slices in fixed order (src/mob/spme.h:219-265 without
+// the stored P/ind tables). Y = beta Y + alpha (sum_U + a^2 sum_W).
+
 template <typename T, int P>
 __global__ void __launch_bounds__(256)
-interp_kernel(int npos, int K, const T* __restrict__ frac, const int* __restrict__ base,
-              const double* __restrict__ a2, const T* __restrict__ mesh, int b, int NC,
-              double alpha, double beta, double* __restrict__ Y) {
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long long)npos * b) return;
-    const int r = (int)(gid / npos);
-    const int is = (int)(gid - (long long)r * npos);
-    T wx[P], wy[P], wz[P];
-    bspline_weights<T, P>(frac[3 * is + 0], wx);
-    bspline_weights<T, P>(frac[3 * is + 1], wy);
-    bspline_weights<T, P>(frac[3 * is + 2], wz);
-    const int bx = base[3 * is + 0], by = base[3 * is + 1], bz = base[3 * is + 2];
-    int xs[P];
-#pragma unroll
-    for (int i = 0; i < P; i++) {
-        int x = bx + i;
-        xs[i] = x >= K ? x - K : x;
-    }
+interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
+                   const int* __restrict__ tile_start, const T* __restrict__ frac,
+                   const int* __restrict__ base, const double* __restrict__ a2,
+                   const T* __restrict__ mesh, int b, int RB, int NC, double alpha, double beta,
+                   double* __restrict__ Y) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int sxi = blockIdx.x % nsx;
+    const int tyz = blockIdx.x / nsx;
+    const int ty = tyz % ntile, tz = tyz / ntile;
+    const int r0 = blockIdx.y * RB;
+    const int rb = min(RB, b - r0);
+    const int stx = sxi * sxt;
+    const int nxt = min(sxt, ntile - stx);
+    const int sx0 = stx * ts, sy0 = ty * ts, sz0 = tz * ts;
+    const int RX = nxt * ts + P - 1, RY = ts + P - 1, RZ = ts + P - 1;
+    const int nreg = RX * RY * RZ;
+    // multiply-high reciprocals of RX and RY (the region index is < 2^17)
+    const unsigned rx_m = 0xffffffffu / (unsigned)RX + 1u;
+    const unsigned ry_m = 0xffffffffu / (unsigned)RY + 1u;
+    // LDS: region [RZ][RY][RX] | weights [CH][3][P] | bases [CH][3] |
+    //      partial sums [CH][P] | results [CH][RB][3] (double)
+    T* region = reinterpret_cast<T*>(smem_raw);
+    T* s_w = region + (size_t)(sxt * ts + P - 1) * RY * RZ;
+    int* s_b = reinterpret_cast<int*>(s_w + ch * 3 * P);
+    T* s_part = reinterpret_cast<T*>(s_b + ch * 3);
+    double* s_res = reinterpret_cast<double*>(
+        (reinterpret_cast<uintptr_t>(s_part + ch * P) + 7) & ~(uintptr_t)7);
     const size_t K3 = (size_t)K * K * K;
-    T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-    const T* m0 = mesh + (size_t)r * NC * K3;
+    // own particles: base tiles stx .. stx+nxt-1 of row (ty, tz) are contiguous
+    const int t_first = (tz * ntile + ty) * ntile + stx;
+    const int pb = tile_start[t_first], pe = tile_start[t_first + nxt];
+    for (int c0 = pb; c0 < pe; c0 += ch) {
+        const int cnt = min(ch, pe - c0);
+        for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+            const int is = c0 + t;
+            T w[P];
 #pragma unroll
-    for (int iz = 0; iz < P; iz++) {
-        int z = bz + iz;
-        z = z >= K ? z - K : z;
+            for (int d = 0; d < 3; d++) {
+                bspline_weights<T, P>(frac[3 * is + d], w);
 #pragma unroll
-        for (int iy = 0; iy < P; iy++) {
-            int yy = by + iy;
-            yy = yy >= K ? yy - K : yy;
-            const T wyz = wz[iz] * wy[iy];
-            const size_t rowoff = ((size_t)z * K + yy) * K;
+                for (int k = 0; k < P; k++) s_w[(t * 3 + d) * P + k] = w[k];
+            }
+            // stencil origin relative to the region
+            s_b[t * 3 + 0] = wrapk(base[3 * is + 0] - sx0, K);
+            s_b[t * 3 + 1] = wrapk(base[3 * is + 1] - sy0, K);
+            s_b[t * 3 + 2] = wrapk(base[3 * is + 2] - sz0, K);
+            for (int q = 0; q < rb * 3; q++) s_res[t * RB * 3 + q] = 0.0;
+        }
+        for (int r = 0; r < rb; r++) {
+            for (int q = 0; q < NC; q++) {
+                __syncthreads();
+                const T* m = mesh + (size_t)((r0 + r) * NC + q) * K3;
+                for (int i = threadIdx.x; i < nreg; i += blockDim.x) {
+                    const int row = (int)fdiv(i, rx_m), lx = i - row * RX;
+                    const int lz = (int)fdiv(row, ry_m), ly = row - lz * RY;
+                    region[i] = m[((size_t)wrapk(sz0 + lz, K) * K + wrapk(sy0 + ly, K)) * K +
+                                  wrapk(sx0 + lx, K)];
+                }
+                __syncthreads();
+                for (int item = threadIdx.x; item < cnt * P; item += blockDim.x) {
+                    const int t = item / P, iz = item - t * P;
+                    const T* wx = s_w + (t * 3 + 0) * P;
+                    const T* wy = s_w + (t * 3 + 1) * P;
+                    const T* rp = region + ((size_t)(s_b[t * 3 + 2] + iz) * RY + s_b[t * 3 + 1]) * RX +
+                                  s_b[t * 3 + 0];
+                    T acc = T(0);
 #pragma unroll
-            for (int ix = 0; ix < P; ix++) {
-                const T w = wyz * wx[ix];
-                const size_t o = rowoff + xs[ix];
-                acc[0] += w * m0[o];
-                acc[1] += w * m0[K3 + o];
-                acc[2] += w * m0[2 * K3 + o];
-                if (NC == 6) {
-                    acc[3] += w * m0[3 * K3 + o];
-                    acc[4] += w * m0[4 * K3 + o];
-                    acc[5] += w * m0[5 * K3 + o];
+                    for (int iy = 0; iy < P; iy++) {
+                        T row = T(0);
+#pragma unroll
+                        for (int ix = 0; ix < P; ix++) row += wx[ix] * rp[iy * RX + ix];
+                        acc += wy[iy] * row;
+                    }
+                    s_part[t * P + iz] = acc * s_w[(t * 3 + 2) * P + iz];
+                }
+                __syncthreads();
+                for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+                    T sum = T(0);
+#pragma unroll
+                    for (int iz = 0; iz < P; iz++) sum += s_part[t * P + iz];
+                    double v = (double)sum;
+                    if (q >= 3) v *= a2[c0 + t];
+                    s_res[t * RB * 3 + r * 3 + (q % 3)] += v;
                 }
             }
         }
-    }
-    const double s2 = (NC == 6) ? a2[is] : 0.0;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        double v = (double)acc[c];
-        if (NC == 6) v += s2 * (double)acc[3 + c];
-        double* y = &Y[(size_t)(3 * is + c) * b + r];
-        *y = (beta == 0.0 ? 0.0 : beta * *y) + alpha * v;
+        __syncthreads();
+        for (int i = threadIdx.x; i < cnt * rb * 3; i += blockDim.x) {
+            const int t = i / (rb * 3), rc = i - t * rb * 3;
+            const int r = rc / 3, c = rc - r * 3;
+            double* y = &Y[(size_t)(3 * (c0 + t) + c) * b + r0 + r];
+            *y = (beta == 0.0 ? 0.0 : beta * *y) + alpha * s_res[t * RB * 3 + r * 3 + c];
+        }
+        __syncthreads();
     }
 }
 
@@ -499,9 +669,21 @@ bool make_fft_plan(int K, FFTPlan* plan) {
 
 static FFTArgs fft_args(const FFTPlan& p) {
     FFTArgs a;
+    memset(&a, 0, sizeof(a));
     a.K = p.K;
     a.nst = p.nst;
-    for (int i = 0; i < 16; i++) a.radix[i] = p.radix[i];
+    a.K_m = fft_magic(p.K);
+    a.Kh_m = fft_magic(p.K / 2 + 1);
+    int len = p.K;
+    for (int i = 0; i < p.nst; i++) {
+        const int r = p.radix[i];
+        a.radix[i] = r;
+        a.per[i] = p.K / r;
+        a.per_m[i] = fft_magic(a.per[i]);
+        a.str[i] = p.K / len;
+        a.str_m[i] = fft_magic(a.str[i]);
+        len /= r;
+    }
     return a;
 }
 
@@ -513,9 +695,10 @@ void SpmeGpu<T>::create(int npos_, double box_, double xi_, int K_, int porder_,
     SDT_REQUIRE(make_fft_plan(K, &plan), SDT_EINVAL,
                 "mesh dimension must factor into 2, 3 and 5");
     SDT_REQUIRE(K <= 1024, SDT_EINVAL, "mesh dimension must be <= 1024");
-    // tile side: a divisor of K in [P-1, 16], preferring 8
+    // tile side: a divisor of K in [P-1, 8], preferring 8 (one wave owns
+    // ts x ts mesh rows)
     ts = 0;
-    const int pref[] = {8, 6, 10, 12, 9, 16, 15, 7, 5};
+    const int pref[] = {8, 6, 7, 5, 4};
     for (int c : pref)
         if (c >= porder - 1 && K % c == 0 && K / c >= 3) { ts = c; break; }
     SDT_REQUIRE(ts > 0, SDT_EINVAL, "no spread tile size divides the mesh dimension");
@@ -579,24 +762,21 @@ template <typename T>
 void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s) {
     SDT_REQUIRE(b <= bcap, SDT_ECAP, "spread: rhs count exceeds mesh capacity");
     const int NC = poly ? 6 : 3;
-    // right-hand sides per workgroup: keep the LDS tile under ~64 KB
-    const int sy = ts + 1, sz = ts * sy + 1, tile_words = ts * sz;
-    // right-hand sides per workgroup: accumulator tile <= 48 KB, split b
-    // evenly over the chunks
-    int rbmax = (int)(49152 / (NC * tile_words * sizeof(T)));
-    rbmax = std::max(1, std::min(rbmax, b));
-    const int nchunk = (b + rbmax - 1) / rbmax;
-    const int RB = (b + nchunk - 1) / nchunk;
-    const size_t lds = sizeof(T) * ((size_t)RB * NC * tile_words + kSpreadChunk * 3 * porder +
-                                    (size_t)kSpreadChunk * RB * NC) +
-                       sizeof(int) * kSpreadChunk * 3 + 16;
-    SDT_REQUIRE(lds <= 64 * 1024, SDT_EINVAL, "spread: LDS tile too large");
-    dim3 grid(ntile * ntile * ntile, ceil_div(b, RB));
-#define SDT_SPREAD(PP)                                                                      \
-    case PP:                                                                                \
-        hipLaunchKernelGGL((spread_tile_kernel<T, PP>), grid, dim3(256), lds, s, K, ts,     \
-                           ntile, d_tile_start.get(), d_frac.get(), d_base.get(),           \
-                           poly ? d_a2.get() : (double*)nullptr, X, b, RB, NC, d_real.get()); \
+    // right-hand sides per workgroup: RB * NC register accumulators per point
+    const int RBNC = poly ? 12 : 12;
+    const int RB = RBNC / NC;
+    const int nsx = (ntile + 3) / 4;
+    const int rec = 3 * porder + 3 + RBNC;
+    int cap = (int)(49152 / (rec * sizeof(T)));
+    SDT_REQUIRE(cap >= kStageChunk, SDT_EINVAL, "spread: staging buffer too small");
+    const size_t lds = (size_t)cap * rec * sizeof(T);
+    dim3 grid(nsx * ntile * ntile, ceil_div(b, RB));
+#define SDT_SPREAD(PP)                                                                        \
+    case PP:                                                                                  \
+        hipLaunchKernelGGL((spread_gather_kernel<T, PP, 12>), grid, dim3(256), lds, s, K, ts, \
+                           ntile, nsx, d_tile_start.get(), d_frac.get(), d_base.get(),        \
+                           poly ? d_a2.get() : (double*)nullptr, X, b, RB, NC, cap,          \
+                           d_real.get());                                                     \
         break;
     switch (porder) { SDT_SPREAD(4) SDT_SPREAD(5) SDT_SPREAD(6) SDT_SPREAD(7) SDT_SPREAD(8) }
 #undef SDT_SPREAD
@@ -604,11 +784,20 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s) {
 }
 
 template <typename T>
-static int pick_np(int K, int ncomp_pen, size_t budget) {
-    // pencils per workgroup so that 2 buffers + twiddles fit the budget
-    int np = 16;
-    while (np > 1 && (2 * (size_t)ncomp_pen * np * K + K) * sizeof(cplx<T>) > budget) np /= 2;
-    return np;
+static int pick_lnp(int K, int ncomp_pen, size_t budget) {
+    // log2 of the pencils per workgroup so that 2 padded buffers + twiddles
+    // fit the budget
+    int lnp = 4;
+    while (lnp > 0 &&
+           (2 * (size_t)ncomp_pen * ((size_t)1 << lnp) * pencil_stride(K) + K) * sizeof(cplx<T>) >
+               budget)
+        lnp--;
+    return lnp;
+}
+
+template <typename T>
+static size_t fft_lds_bytes(int K, int npen) {
+    return (K + 2 * (size_t)npen * pencil_stride(K)) * sizeof(cplx<T>);
 }
 
 template <typename T>
@@ -616,15 +805,15 @@ void SpmeGpu<T>::fft_forward(int b, hipStream_t s) {
     const FFTArgs fa = fft_args(plan);
     const int Kh = K / 2 + 1;
     const long long nrows = (long long)nmesh(b) * K * K;
-    int NR = 2 * pick_np<T>(K, 1, 48 * 1024);
-    size_t lds = (K + 2 * (size_t)(NR / 2) * K) * sizeof(cplx<T>);
-    hipLaunchKernelGGL((fft_x_r2c_kernel<T>), dim3(ceil_div(nrows, NR)), dim3(256), lds, s, fa,
-                       nrows, d_real.get(), d_spec.get(), d_tw.get(), NR);
+    const int NR = 2 << pick_lnp<T>(K, 1, 48 * 1024);
+    hipLaunchKernelGGL((fft_x_r2c_kernel<T>), dim3(ceil_div(nrows, NR)), dim3(256),
+                       fft_lds_bytes<T>(K, NR / 2), s, fa, nrows, d_real.get(), d_spec.get(),
+                       d_tw.get(), NR);
     SDT_HIP_CHECK(hipGetLastError());
-    int NP = pick_np<T>(K, 1, 48 * 1024);
-    lds = (K + 2 * (size_t)NP * K) * sizeof(cplx<T>);
-    hipLaunchKernelGGL((fft_y_kernel<T, -1>), dim3(ceil_div(Kh, NP), nmesh(b) * K), dim3(256),
-                       lds, s, fa, d_spec.get(), d_tw.get(), NP);
+    const int lnp = pick_lnp<T>(K, 1, 48 * 1024);
+    hipLaunchKernelGGL((fft_y_kernel<T, -1>), dim3(ceil_div(Kh, 1 << lnp) * nmesh(b) * K),
+                       dim3(256), fft_lds_bytes<T>(K, 1 << lnp), s, fa, d_spec.get(), d_tw.get(),
+                       lnp);
     SDT_HIP_CHECK(hipGetLastError());
 }
 
@@ -633,10 +822,10 @@ void SpmeGpu<T>::influence(int b, hipStream_t s) {
     const FFTArgs fa = fft_args(plan);
     const int Kh = K / 2 + 1;
     const int NC = poly ? 6 : 3;
-    int NP = pick_np<T>(K, NC, 64 * 1024);
-    size_t lds = (K + 2 * (size_t)NC * NP * K) * sizeof(cplx<T>);
-    hipLaunchKernelGGL((fft_z_influence_kernel<T>), dim3(ceil_div(Kh, NP), b * K), dim3(256), lds,
-                       s, fa, d_spec.get(), d_tw.get(), d_bsp2.get(), NP, NC, box, xi);
+    const int lnp = pick_lnp<T>(K, NC, 64 * 1024);
+    hipLaunchKernelGGL((fft_z_influence_kernel<T>), dim3(ceil_div(Kh, 1 << lnp) * b * K),
+                       dim3(256), fft_lds_bytes<T>(K, NC << lnp), s, fa, d_spec.get(), d_tw.get(),
+                       d_bsp2.get(), lnp, NC, box, xi);
     SDT_HIP_CHECK(hipGetLastError());
 }
 
@@ -645,15 +834,15 @@ void SpmeGpu<T>::fft_backward(int b, hipStream_t s) {
     const FFTArgs fa = fft_args(plan);
     const int Kh = K / 2 + 1;
     const long long nrows = (long long)nmesh(b) * K * K;
-    int NP = pick_np<T>(K, 1, 48 * 1024);
-    size_t lds = (K + 2 * (size_t)NP * K) * sizeof(cplx<T>);
-    hipLaunchKernelGGL((fft_y_kernel<T, +1>), dim3(ceil_div(Kh, NP), nmesh(b) * K), dim3(256),
-                       lds, s, fa, d_spec.get(), d_tw.get(), NP);
+    const int lnp = pick_lnp<T>(K, 1, 48 * 1024);
+    hipLaunchKernelGGL((fft_y_kernel<T, +1>), dim3(ceil_div(Kh, 1 << lnp) * nmesh(b) * K),
+                       dim3(256), fft_lds_bytes<T>(K, 1 << lnp), s, fa, d_spec.get(), d_tw.get(),
+                       lnp);
     SDT_HIP_CHECK(hipGetLastError());
-    int NR = 2 * pick_np<T>(K, 1, 48 * 1024);
-    lds = (K + 2 * (size_t)(NR / 2) * K) * sizeof(cplx<T>);
-    hipLaunchKernelGGL((fft_x_c2r_kernel<T>), dim3(ceil_div(nrows, NR)), dim3(256), lds, s, fa,
-                       nrows, d_spec.get(), d_real.get(), d_tw.get(), NR);
+    const int NR = 2 << pick_lnp<T>(K, 1, 48 * 1024);
+    hipLaunchKernelGGL((fft_x_c2r_kernel<T>), dim3(ceil_div(nrows, NR)), dim3(256),
+                       fft_lds_bytes<T>(K, NR / 2), s, fa, nrows, d_spec.get(), d_real.get(),
+                       d_tw.get(), NR);
     SDT_HIP_CHECK(hipGetLastError());
 }
 
@@ -662,13 +851,30 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
     const int NC = poly ? 6 : 3;
     // alpha * K^3 / L^3 with the unnormalised inverse transform: alpha / L^3
     const double a_eff = alpha / (box * box * box);
-    const long long n = (long long)npos * b;
-#define SDT_INTERP(PP)                                                                       \
-    case PP:                                                                                 \
-        hipLaunchKernelGGL((interp_kernel<T, PP>), dim3(ceil_div(n, 256)), dim3(256), 0, s,  \
-                           npos, K, d_frac.get(), d_base.get(),                              \
-                           poly ? d_a2.get() : (double*)nullptr, d_real.get(), b, NC, a_eff, \
-                           beta, Y);                                                         \
+    const int RB = 4;
+    const int R = porder - 1;
+    // super-tile width (tiles along x) and particle chunk: fit 64 KB LDS
+    int sxt = 4, ch = 128;
+    auto lds_of = [&](int sx, int c) {
+        const size_t nreg = (size_t)(sx * ts + R) * (ts + R) * (ts + R);
+        return nreg * sizeof(T) + (size_t)c * 3 * porder * sizeof(T) + (size_t)c * 3 * sizeof(int) +
+               (size_t)c * porder * sizeof(T) + 8 + (size_t)c * RB * 3 * sizeof(double);
+    };
+    while (lds_of(sxt, ch) > 64 * 1024 && (sxt > 1 || ch > 32)) {
+        if (ch > 64) ch /= 2;
+        else if (sxt > 1) sxt /= 2;
+        else ch /= 2;
+    }
+    const size_t lds = lds_of(sxt, ch);
+    SDT_REQUIRE(lds <= 64 * 1024, SDT_EINVAL, "interp: LDS region too large");
+    const int nsx = (ntile + sxt - 1) / sxt;
+    dim3 grid(nsx * ntile * ntile, ceil_div(b, RB));
+#define SDT_INTERP(PP)                                                                        \
+    case PP:                                                                                  \
+        hipLaunchKernelGGL((interp_tile_kernel<T, PP>), grid, dim3(256), lds, s, K, ts, ntile, \
+                           nsx, sxt, ch, d_tile_start.get(), d_frac.get(), d_base.get(),      \
+                           poly ? d_a2.get() : (double*)nullptr, d_real.get(), b, RB, NC,    \
+                           a_eff, beta, Y);                                                   \
         break;
     switch (porder) { SDT_INTERP(4) SDT_INTERP(5) SDT_INTERP(6) SDT_INTERP(7) SDT_INTERP(8) }
 #undef SDT_INTERP

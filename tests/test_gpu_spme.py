@@ -174,3 +174,26 @@ def test_large_system_fp32_vs_fp64(sdt, orc):
     assert orc.frob_rel_err(outs[0], outs[1]) < 1e-5
     x, y = f[:, 0], f[:, 1]
     assert abs(x @ outs[0][:, 1] - y @ outs[0][:, 0]) < 1e-11 * abs(x @ outs[0][:, 1])
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-5)])
+def test_workload_size_matches_oracle(sdt, orc, prec, tol):
+    # the N = 100 000, phi = 0.3 (L = 111.7693), K = 128, p = 6, rc = 5,
+    # xi = 0.896 system from random sequential addition, 12 right-hand sides:
+    # dense tiles put more particles into a spread workgroup's staging list
+    # than one pass of it holds
+    n = 100_000
+    box = (4.0 * np.pi * n / (3.0 * 0.3)) ** (1.0 / 3.0)
+    pos = sdt.rsa_positions(n, box, 1.0, 2024)
+    rdi = np.ones(n)
+    g = sdt.MobSpme(n, rdi, box, 0.896, 5.0, 128, 6, prec)
+    assert g.init()
+    g.update(pos)
+    o = orc.MobSpme(n, rdi, box, 0.896, 5.0, 128, 6)
+    assert o.init()
+    o.update(pos, rdi)
+    f = np.random.default_rng(11).standard_normal((3 * n, 12))
+    for mode in ("recip", "real"):
+        got = getattr(g, mode + "_mul_vector")(f)
+        ref = o.mul_vector(f, mode=mode)
+        assert orc.frob_rel_err(ref, got) < tol, mode
