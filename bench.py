@@ -58,6 +58,55 @@ def alg_bytes(stage, b, n, K, e, poly=False):
     return 0
 
 
+def lanczos_batch_cols():
+    """Columns of the fused Lanczos matvec: the library runs up to 128
+    columns (whole groups of b) in lockstep."""
+    c = CFG
+    return min(c["mob_interval"], (128 // c["b"]) * c["b"])
+
+
+def read_traffic(stage):
+    """HBM bytes per launch of a stage by launch width, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines 'traffic stage=<s>
+    columns=<b> bytes=<B>'); empty when absent."""
+    out = {}
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_bd_n100k.txt")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if not line.startswith("traffic "):
+                    continue
+                kv = dict(t.split("=", 1) for t in line.split()[1:])
+                if kv.get("stage") == stage:
+                    out[int(kv["columns"])] = float(kv["bytes"])
+    except OSError:
+        pass
+    return out
+
+
+def kernel_sample():
+    """Short fixed kernel sample for counter runs: a calibration copy of
+    2^26 doubles, then the FP32 matvec pipeline at the workload shape with
+    1 and with the Lanczos batch width of columns (2 repetitions each)."""
+    import stokesdt_amd as S
+    c = CFG
+    n = c["n"]
+    L = box_size(n, c["phi"])
+    ms = S.bench_copy(1 << 26, 3)
+    print("copy 2^26 doubles: %.4f ms (%.1f GB/s)" % (ms, 2 * 8 * (1 << 26) / ms / 1e6),
+          flush=True)
+    pos = S.rsa_positions(n, L, 1.0, c["seed"])
+    m = S.MobSpme(n, np.ones(n), L, c["xi"], c["rmax"], c["K"], c["p"], S.FP32)
+    assert m.init()
+    m.update(pos)
+    for b in (1, lanczos_batch_cols()):
+        t = m.stage_times(b, 2)
+        print("columns %d: " % b + " ".join("%s=%.4f" % kv for kv in
+                                           zip(["spread", "fft_xy_fwd", "fft_z_influence",
+                                                "fft_xy_inv", "interp", "real_space", "whole"],
+                                               t)), flush=True)
+
+
 def run_gpu(S, pos, rdi, L, prec, steps, warmup, per_step):
     c = CFG
     bd = S.BDSim(pos, rdi, L, c["xi"], c["rmax"], c["K"], c["p"], prec, c["max_iters"], c["b"],
@@ -154,7 +203,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU path")
     ap.add_argument("--no-err", action="store_true", help="skip the FP32-vs-FP64 error check")
+    ap.add_argument("--kernel-sample", action="store_true",
+                    help="run only the short kernel sample used for counter collection")
     args = ap.parse_args()
+    if args.kernel_sample:
+        kernel_sample()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the Brownian-dynamics step of one system runs on "
               "a single MI355X here (multi-GPU sharding is design-only); run with --gpus 1",
@@ -189,32 +243,34 @@ def main():
     stage = {k: dict(total_ms=round(v[0], 4), launches=v[1]) for k, v in r32["prof"].items()}
     res["stage_device_ms"] = stage
 
-    # roofline of the dominant kernel (largest device time in the timed region)
-    rhs_per_launch = []
-    groups = [c["b"]] * (c["mob_interval"] // c["b"])
-    if c["mob_interval"] % c["b"]:
-        groups.append(c["mob_interval"] % c["b"])
-    lanczos_in_window = r32["prof"]["mob_update"][1] > 0
-    if lanczos_in_window:
-        for k, g in zip(r32["iters"], groups):
-            rhs_per_launch += [g] * (k + 1 if k >= 0 else c["max_iters"])
-    rhs_per_launch += [1] * args.steps
+    # roofline of the dominant kernel (largest device time in the timed
+    # region). Its launches there: one single-column matvec per step and,
+    # when the window holds a mobility update, one fused matvec over the
+    # whole Lanczos batch per block iteration.
     cand = ["spread", "fft_xy_fwd", "fft_z_influence", "fft_xy_inv", "interp"]
     dom = max(cand, key=lambda k: r32["prof"][k][0])
     tot_ms, launches = r32["prof"][dom]
-    if launches == len(rhs_per_launch):
-        tot_bytes = sum(alg_bytes(dom, b, n, c["K"], 4) for b in rhs_per_launch)
+    btot = lanczos_batch_cols()
+    n_lz = launches - args.steps
+    if n_lz >= 0 and tot_ms > 0:
+        tot_bytes = (args.steps * alg_bytes(dom, 1, n, c["K"], 4) +
+                     n_lz * alg_bytes(dom, btot, n, c["K"], 4))
         achieved = tot_bytes / (tot_ms * 1e-3) / 1e9
+        traffic = None
+        tr = read_traffic(dom)
+        if 1 in tr and btot in tr:
+            traffic = round((args.steps * tr[1] + n_lz * tr[btot]) / launches)
         res["roofline"] = dict(kernel=dom, bound="hbm", achieved=round(achieved, 2),
                                peak=HBM_PEAK_GBS, unit="GB/s",
-                               frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                               launches=launches, avg_launch_ms=round(tot_ms / launches, 5),
+                               frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,
+                               launches=launches, launch_columns={"1": args.steps,
+                                                                  str(btot): n_lz},
+                               avg_launch_ms=round(tot_ms / launches, 5),
                                alg_bytes_per_launch=round(tot_bytes / launches))
     else:
         res["roofline"] = dict(kernel=dom, bound="hbm", achieved=None, peak=HBM_PEAK_GBS,
                                unit="GB/s", frac=None, traffic=None,
-                               note="launch count %d != expected %d" % (launches,
-                                                                       len(rhs_per_launch)))
+                               note="unexpected launch count %d" % launches)
     O = None
     if not args.no_cpu or not args.no_err:
         try:

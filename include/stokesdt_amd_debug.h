@@ -34,6 +34,9 @@ int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh);
  * t[0] spread, t[1] forward x/y FFT, t[2] z FFT + influence + inverse z,
  * t[3] inverse y/x FFT, t[4] interp, t[5] real space, t[6] whole matvec. */
 int sdt_bench_stage_times(sdt_mob_t mob, int b, int reps, double* t);
+/* reps launches of a plain device copy of n doubles (reads and writes 8 n
+ * bytes each), for scaling HBM byte counters; ms_avg = mean launch time. */
+int sdt_bench_copy(int64_t n, int reps, double* ms_avg);
 /* Mean device time (ms) of one full BD step sequence: runs `nsteps` steps
  * bracketed by events on the handle's stream. */
 int sdt_bench_bd_steps(sdt_bd_t bd, int nsteps, double* ms_total);

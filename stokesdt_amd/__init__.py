@@ -84,6 +84,7 @@ SIGNATURES = {
     "sdt_debug_band_sqrt": (_I, [_I, _I, _P, _I, _P]),
     "sdt_debug_spme_spread": (_I, [_P, _P, _P]),
     "sdt_bench_stage_times": (_I, [_P, _I, _I, _P]),
+    "sdt_bench_copy": (_I, [_I64, _I, ctypes.POINTER(_D)]),
     "sdt_bench_bd_steps": (_I, [_P, _I, ctypes.POINTER(_D)]),
     "sdt_bd_set_profiling": (_I, [_P, _I]),
     "sdt_bd_profile": (_I, [_P, _P, _P, _I]),
@@ -426,6 +427,13 @@ def debug_gram(A, B):
     G = np.zeros((b, b))
     _check(lib().sdt_debug_gram(n, b, _ptr(A), _ptr(B), _ptr(G)))
     return G
+
+
+def bench_copy(n, reps=3):
+    """Mean time (ms) of a plain device copy of n doubles."""
+    ms = _D()
+    _check(lib().sdt_bench_copy(int(n), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 def debug_gram_groups(A, B, bs):

@@ -523,6 +523,44 @@ int sdt_bench_stage_times(sdt_mob_t mob, int b, int reps, double* t) {
     });
 }
 
+}  // extern "C"
+
+// Plain streaming copy of doubles: the reference kernel for scaling HBM
+// byte counters (it reads and writes exactly 8 n bytes each).
+__global__ void sdt_copy_kernel(long long n, const double* __restrict__ src,
+                                double* __restrict__ dst) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
+extern "C" {
+
+int sdt_bench_copy(int64_t n, int reps, double* ms_avg) {
+    return guard([&] {
+        SDT_REQUIRE(n > 0 && reps >= 1, SDT_EINVAL, "bad arguments");
+        DevBuf<double> a, b;
+        a.reserve((size_t)n);
+        b.reserve((size_t)n);
+        fill(n, 1.0, a.get(), 0);
+        hipEvent_t e0, e1;
+        SDT_HIP_CHECK(hipEventCreate(&e0));
+        SDT_HIP_CHECK(hipEventCreate(&e1));
+        SDT_HIP_CHECK(hipEventRecord(e0, 0));
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(sdt_copy_kernel, dim3(4096), dim3(256), 0, 0, (long long)n, a.get(),
+                               b.get());
+        SDT_HIP_CHECK(hipGetLastError());
+        SDT_HIP_CHECK(hipEventRecord(e1, 0));
+        SDT_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SDT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms_avg) *ms_avg = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
 int sdt_bench_bd_steps(sdt_bd_t bd, int nsteps, double* ms_total) {
     SDT_CHECK_HANDLE(bd);
     return guard([&] {

@@ -168,7 +168,7 @@ spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ 
     __syncthreads();
     const int ncand = s_off[nbt];
 
-    auto accumulate = [&](int n) {
+    auto accumulate = [&](int n) __attribute__((always_inline)) {
         if (!lane_on) return;
         for (int sidx = 0; sidx < n; sidx++) {
             const T* rec = stage + (size_t)sidx * REC;
@@ -253,29 +253,30 @@ spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ 
                 if (take_u[u] && slot >= done && slot < done + take) {
                     const int is = is_u[u];
                     T* rec = stage + (size_t)(nst + slot - done) * REC;
-                    T fr[3];
-                    fr[0] = frac[3 * is]; fr[1] = frac[3 * is + 1]; fr[2] = frac[3 * is + 2];
-                    double xv[RBNC > 0 ? RBNC : 1];
-                    const T sc = (NC == 6) ? (T)a2[is] : T(1);
-                    for (int r = 0; r < rb; r++)
-                        for (int c = 0; c < 3; c++)
-                            xv[r * 3 + c] = X[(size_t)(3 * is + c) * b + r0 + r];
+                    // forces: slot q holds rhs r, component c (mono: q = 3 r + c;
+                    // poly: q = 6 r + c for f, 6 r + 3 + c for a^2 f); every
+                    // index is a compile-time constant (no register arrays
+                    // indexed at run time)
+                    const double sc = (NC == 6) ? a2[is] : 1.0;
+#pragma unroll
+                    for (int q = 0; q < RBNC; q++) {
+                        const int r = (NC == 6) ? q / 6 : q / 3;
+                        const bool scaled = (NC == 6) && (q % 6) >= 3;
+                        T fv = T(0);
+                        if (r < rb) {
+                            const double x = X[(size_t)(3 * is + q % 3) * b + r0 + r];
+                            fv = (T)(scaled ? x * sc : x);
+                        }
+                        rec[3 * P + 3 + q] = fv;
+                    }
                     T w[P];
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
-                        bspline_weights<T, P>(fr[d], w);
+                        bspline_weights<T, P>(frac[3 * is + d], w);
 #pragma unroll
                         for (int k = 0; k < P; k++) rec[d * P + k] = w[k];
                         rec[3 * P + d] = (T)b3u[u][d];
                     }
-                    for (int r = 0; r < rb; r++) {
-                        for (int c = 0; c < 3; c++) {
-                            const T fv = (T)xv[r * 3 + c];
-                            rec[3 * P + 3 + r * NC + c] = fv;
-                            if (NC == 6) rec[3 * P + 3 + r * NC + 3 + c] = fv * sc;
-                        }
-                    }
-                    for (int q = rbnc; q < RBNC; q++) rec[3 * P + 3 + q] = T(0);
                 }
                 nst += take;
                 done += take;
@@ -287,11 +288,14 @@ spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ 
     if (!lane_on) return;
     const size_t K3 = (size_t)K * K * K;
     const size_t row = ((size_t)zl * K + yl) * K + wx0;
-    for (int q = 0; q < rbnc; q++) {
-        T* m = mesh + (size_t)(r0 * NC + q) * K3 + row;
 #pragma unroll
-        for (int x = 0; x < 8; x++)
-            if (x < ts) m[x] = acc[x][q];
+    for (int q = 0; q < RBNC; q++) {
+        if (q < rbnc) {
+            T* m = mesh + (size_t)(r0 * NC + q) * K3 + row;
+#pragma unroll
+            for (int x = 0; x < 8; x++)
+                if (x < ts) m[x] = acc[x][q];
+        }
     }
 }
 
