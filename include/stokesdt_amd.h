@@ -97,8 +97,9 @@ int sdt_pairlist_build(sdt_pairlist_t pl, const double* pos, int64_t* nnz);
 int sdt_pairlist_get_pairs(sdt_pairlist_t pl, int32_t* rowptr, int32_t* colidx);
 int sdt_pairlist_destroy(sdt_pairlist_t pl);
 /* One-shot [ia, ja] = pairlist_mex(pos, rdi, L, cutoff, mode)
- * (mex/pairlist_mex.cc:9-80): 0-based CSR. If ja_cap < nnz only *nnz is
- * set and SDT_ECAP is returned (call again with a larger ja). */
+ * (mex/pairlist_mex.cc:9-80): 0-based CSR. If ja_cap < nnz, or ia / ja
+ * are null (a size query), only *nnz is set and SDT_ECAP is returned (call
+ * again with buffers of npos + 1 and nnz). */
 int sdt_pairlist(const double* pos, const double* rdi, int npos, double L, double cutoff,
                  char mode, int32_t* ia, int32_t* ja, int64_t ja_cap, int64_t* nnz);
 

@@ -223,7 +223,7 @@ int sdt_pairlist_destroy(sdt_pairlist_t pl) {
 int sdt_pairlist(const double* pos, const double* rdi, int npos, double L, double cutoff,
                  char mode, int32_t* ia, int32_t* ja, int64_t ja_cap, int64_t* nnz) {
     // argument checks of mex/pairlist_mex.cc:13-53
-    if (!pos || !rdi || !ia || npos <= 0) { set_error("invalid array arguments"); return SDT_EINVAL; }
+    if (!pos || !rdi || npos <= 0) { set_error("invalid array arguments"); return SDT_EINVAL; }
     if (L <= 0. || cutoff <= 0.) { set_error("Scalar arguments must be positive."); return SDT_EINVAL; }
     if (2. * cutoff >= L) { set_error("cutoff must be less than L/2."); return SDT_EINVAL; }
     if (mode != 'R' && mode != 'S') { set_error("mode must be R or S."); return SDT_EINVAL; }
@@ -235,7 +235,8 @@ int sdt_pairlist(const double* pos, const double* rdi, int npos, double L, doubl
     if (!rc) rc = sdt_pairlist_build(h, pos, &n);
     if (!rc) {
         if (nnz) *nnz = n;
-        if (n > ja_cap || (n > 0 && !ja)) {
+        if (n > ja_cap || !ia || (n > 0 && !ja)) {
+            // a size query (null outputs) or too small a ja: report nnz only
             set_error("ja capacity is smaller than nnz");
             rc = SDT_ECAP;
         } else {
