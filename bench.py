@@ -76,8 +76,8 @@ def read_traffic(stage):
             for line in fh:
                 if not line.startswith("traffic "):
                     continue
-                kv = dict(t.split("=", 1) for t in line.split()[1:])
-                if kv.get("stage") == stage:
+                kv = dict(t.split("=", 1) for t in line.split()[1:] if "=" in t)
+                if kv.get("stage") == stage and "columns" in kv and "bytes" in kv:
                     out[int(kv["columns"])] = float(kv["bytes"])
     except OSError:
         pass

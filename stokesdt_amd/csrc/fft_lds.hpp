@@ -1,4 +1,4 @@
-// Device-side radix-2/3/4/5 Stockham FFT on pencils held in LDS.
+// Device-side radix-2/3/4/5/8 Stockham FFT on pencils held in LDS.
 //
 // Self-sorting decimation-in-frequency form: a stage of radix r on a
 // sub-sequence length `len` with stride `s` (m = len / r) reads
@@ -26,8 +26,8 @@ struct FFTArgs {
     unsigned str[16], str_m[16];   // sub-sequence stride s = K / len and reciprocal
 };
 
-// floor(n / d) = umulhi(n, floor(2^32 / d) + 1) for n < 2^17 and 2 <= d <
-// 2^15 (the error term n (2^32 mod d) / 2^32 / d stays below 1 / d); the
+// floor(n / d) = umulhi(n, floor(2^32 / d) + 1) for d >= 2 and n d < 2^32
+// (the error term n (d - 2^32 mod d) / 2^32 / d stays below 1 / d); the
 // reciprocal of 1 is encoded as 0.
 inline unsigned fft_magic(unsigned d) {
     return d == 1 ? 0u : (unsigned)((1ull << 32) / d + 1ull);
@@ -83,6 +83,25 @@ __device__ __forceinline__ void small_dft(cplx<T>* a) {
         a[0] = cadd(a[0], t1);
         a[1] = cadd(m, j);
         a[2] = csub(m, j);
+    } else if constexpr (R == 8) {
+        // two 4-point DFTs (even / odd inputs) and the w_8^k combination
+        cplx<T> e[4] = {a[0], a[2], a[4], a[6]};
+        cplx<T> o[4] = {a[1], a[3], a[5], a[7]};
+        small_dft<T, 4, SIGN>(e);
+        small_dft<T, 4, SIGN>(o);
+        const T h = T(0.70710678118654752440084436210485);
+        const T sg = T(SIGN);
+        const cplx<T> t1{h * (o[1].x - sg * o[1].y), h * (o[1].y + sg * o[1].x)};
+        const cplx<T> t2 = rot90<T, SIGN>(o[2]);
+        const cplx<T> t3{h * (-o[3].x - sg * o[3].y), h * (-o[3].y + sg * o[3].x)};
+        a[0] = cadd(e[0], o[0]);
+        a[4] = csub(e[0], o[0]);
+        a[1] = cadd(e[1], t1);
+        a[5] = csub(e[1], t1);
+        a[2] = cadd(e[2], t2);
+        a[6] = csub(e[2], t2);
+        a[3] = cadd(e[3], t3);
+        a[7] = csub(e[3], t3);
     } else if constexpr (R == 5) {
         const T c1 = T(0.30901699437494742410229341718282);    // cos(2pi/5)
         const T c2 = T(-0.80901699437494742410229341718282);   // cos(4pi/5)
@@ -133,13 +152,79 @@ __device__ __forceinline__ void fft_stage(const cplx<T>* __restrict__ src,
     }
 }
 
+// In-place form of the same stage: every thread first computes its (at most
+// MAXB) butterflies into registers, the workgroup synchronises, then all
+// results are stored. One LDS buffer instead of two. Requires
+// npen * K / R <= MAXB * blockDim.x.
+template <typename T, int R, int SIGN, int MAXB>
+__device__ __forceinline__ void fft_stage_inplace(cplx<T>* __restrict__ buf, int npen, int PS,
+                                                  const FFTArgs& fa, int st,
+                                                  const cplx<T>* __restrict__ tw) {
+    const unsigned per = fa.per[st], per_m = fa.per_m[st];
+    const unsigned s = fa.str[st], s_m = fa.str_m[st];
+    const unsigned m = per / s;
+    const unsigned total = (unsigned)npen * per;
+    cplx<T> out[MAXB][R];
+    unsigned dst[MAXB];
+#pragma unroll
+    for (int u = 0; u < MAXB; u++) {
+        const unsigned item = threadIdx.x + u * blockDim.x;
+        dst[u] = 0xffffffffu;
+        if (item < total) {
+            const unsigned pen = fdiv(item, per_m);
+            const unsigned j = item - pen * per;
+            const unsigned p = fdiv(j, s_m);
+            const unsigned q = j - p * s;
+            const cplx<T>* sp = buf + pen * PS;
+#pragma unroll
+            for (int t = 0; t < R; t++) out[u][t] = sp[lpad(q + s * (p + t * m))];
+            small_dft<T, R, SIGN>(out[u]);
+#pragma unroll
+            for (int t = 1; t < R; t++) {
+                cplx<T> w = tw[s * p * t];
+                if (SIGN > 0) w.y = -w.y;
+                out[u][t] = cmul(out[u][t], w);
+            }
+            dst[u] = pen * PS + q + s * R * p;   // unpadded offset of output 0
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < MAXB; u++) {
+        if (dst[u] != 0xffffffffu) {
+            const unsigned pen = fdiv(threadIdx.x + u * blockDim.x, per_m);
+            const unsigned ob = dst[u] - pen * PS;
+            cplx<T>* dp = buf + pen * PS;
+#pragma unroll
+            for (int t = 0; t < R; t++) dp[lpad(ob + s * t)] = out[u][t];
+        }
+    }
+    __syncthreads();
+}
+
+// In-place transform of npen pencils; MAXI2 bounds the radix-2 butterflies
+// per thread (npen * K <= 2 * MAXI2 * blockDim.x), other radices scale.
+template <typename T, int SIGN, int MAXI2>
+__device__ void fft_pencils_inplace(cplx<T>* buf, int npen, int PS, const FFTArgs& fa,
+                                    const cplx<T>* tw) {
+    for (int st = 0; st < fa.nst; st++) {
+        const int r = fa.radix[st];
+        if (r == 8) fft_stage_inplace<T, 8, SIGN, (2 * MAXI2 + 7) / 8>(buf, npen, PS, fa, st, tw);
+        else if (r == 4) fft_stage_inplace<T, 4, SIGN, (2 * MAXI2 + 3) / 4>(buf, npen, PS, fa, st, tw);
+        else if (r == 2) fft_stage_inplace<T, 2, SIGN, MAXI2>(buf, npen, PS, fa, st, tw);
+        else if (r == 3) fft_stage_inplace<T, 3, SIGN, (2 * MAXI2 + 2) / 3>(buf, npen, PS, fa, st, tw);
+        else fft_stage_inplace<T, 5, SIGN, (2 * MAXI2 + 4) / 5>(buf, npen, PS, fa, st, tw);
+    }
+}
+
 // Runs all stages; returns the buffer holding the result (a or b).
 template <typename T, int SIGN>
 __device__ cplx<T>* fft_pencils(cplx<T>* a, cplx<T>* b, int npen, int PS, const FFTArgs& fa,
                                 const cplx<T>* tw) {
     for (int st = 0; st < fa.nst; st++) {
         const int r = fa.radix[st];
-        if (r == 4) fft_stage<T, 4, SIGN>(a, b, npen, PS, fa, st, tw);
+        if (r == 8) fft_stage<T, 8, SIGN>(a, b, npen, PS, fa, st, tw);
+        else if (r == 4) fft_stage<T, 4, SIGN>(a, b, npen, PS, fa, st, tw);
         else if (r == 2) fft_stage<T, 2, SIGN>(a, b, npen, PS, fa, st, tw);
         else if (r == 3) fft_stage<T, 3, SIGN>(a, b, npen, PS, fa, st, tw);
         else fft_stage<T, 5, SIGN>(a, b, npen, PS, fa, st, tw);

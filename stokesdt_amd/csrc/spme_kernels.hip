@@ -8,6 +8,7 @@
 // spreads by owner-computes mesh tiles accumulated in LDS (plain stores, no
 // global atomics, no zero-fill pass), and fuses the influence multiply
 // between the forward and inverse z-transforms.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include "fft_lds.hpp"
@@ -413,56 +414,177 @@ fft_y_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ 
     }
 }
 
+// ------------------------------------------------ fused x + y plane -------
+// One workgroup per z-plane of one mesh: the K x K real plane is read once,
+// transformed along x (two real rows per complex FFT) and along y in LDS
+// (in place, padded pencils), and the K x (K/2+1) half-spectrum plane is
+// written once. The inverse kernel mirrors it. Used when one plane fits LDS.
+// Register bounds are template parameters: MAXI2 radix-2 butterflies and
+// MAXP pack/unpack items per thread (K <= 128: 5 and 8; K <= 180: 8 and 16).
+constexpr int kPlaneThreads = 1024;
+
+template <typename T, int MAXI2, int MAXP>
+__global__ void __launch_bounds__(kPlaneThreads)
+fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict__ spec,
+                     const double* __restrict__ tw_g) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int K = fa.K, Kh = K / 2 + 1, PS = pencil_stride(K), H = K / 2;
+    cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* buf = tw + K;
+    load_twiddles(tw, tw_g, K);
+    const size_t plane = blockIdx.x;
+    const T* src = real + plane * K * K;
+    // row pairs (2j, 2j+1) -> pencil j
+    for (int i = threadIdx.x; i < H * K; i += blockDim.x) {
+        const int j = (int)fdiv(i, fa.K_m), x = i - j * K;
+        buf[j * PS + lpad(x)] = cplx<T>{src[(size_t)(2 * j) * K + x], src[(size_t)(2 * j + 1) * K + x]};
+    }
+    __syncthreads();
+    fft_pencils_inplace<T, -1, MAXI2>(buf, H, PS, fa, tw);
+    // unpack Z_j = A + i B into the half spectra of rows 2j and 2j+1, stored
+    // as y-pencils (pencil kx, element y)
+    {
+        cplx<T> va[MAXP], vb[MAXP];
+        const int nit = H * Kh;
+#pragma unroll
+        for (int u = 0; u < MAXP; u++) {
+            const int i = threadIdx.x + u * blockDim.x;
+            if (i < nit) {
+                const int j = (int)fdiv(i, fa.Kh_m), k = i - j * Kh;
+                const cplx<T> zk = buf[j * PS + lpad(k)];
+                const cplx<T> zm = buf[j * PS + lpad(k == 0 ? 0 : K - k)];
+                const T h = T(0.5);
+                va[u] = cplx<T>{h * (zk.x + zm.x), h * (zk.y - zm.y)};
+                vb[u] = cplx<T>{h * (zk.y + zm.y), h * (zm.x - zk.x)};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < MAXP; u++) {
+            const int i = threadIdx.x + u * blockDim.x;
+            if (i < nit) {
+                const int j = (int)fdiv(i, fa.Kh_m), k = i - j * Kh;
+                buf[k * PS + lpad(2 * j)] = va[u];
+                buf[k * PS + lpad(2 * j + 1)] = vb[u];
+            }
+        }
+        __syncthreads();
+    }
+    fft_pencils_inplace<T, -1, MAXI2>(buf, Kh, PS, fa, tw);
+    cplx<T>* dst = spec + plane * K * Kh;
+    for (int i = threadIdx.x; i < K * Kh; i += blockDim.x) {
+        const int y = (int)fdiv(i, fa.Kh_m), kx = i - y * Kh;
+        dst[i] = buf[kx * PS + lpad(y)];
+    }
+}
+
+template <typename T, int MAXI2, int MAXP>
+__global__ void __launch_bounds__(kPlaneThreads)
+fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict__ real,
+                     const double* __restrict__ tw_g) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int K = fa.K, Kh = K / 2 + 1, PS = pencil_stride(K), H = K / 2;
+    cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* buf = tw + K;
+    load_twiddles(tw, tw_g, K);
+    const size_t plane = blockIdx.x;
+    const cplx<T>* src = spec + plane * K * Kh;
+    for (int i = threadIdx.x; i < K * Kh; i += blockDim.x) {
+        const int y = (int)fdiv(i, fa.Kh_m), kx = i - y * Kh;
+        buf[kx * PS + lpad(y)] = src[i];
+    }
+    __syncthreads();
+    fft_pencils_inplace<T, +1, MAXI2>(buf, Kh, PS, fa, tw);
+    // pack the half spectra of rows 2j, 2j+1 into Z_j = A + i B (full x)
+    {
+        cplx<T> vz[MAXP];
+        const int nit = H * K;
+#pragma unroll
+        for (int u = 0; u < MAXP; u++) {
+            const int i = threadIdx.x + u * blockDim.x;
+            if (i < nit) {
+                const int j = (int)fdiv(i, fa.K_m), k = i - j * K;
+                const int kk = (k <= H) ? k : K - k;
+                cplx<T> A = buf[kk * PS + lpad(2 * j)];
+                cplx<T> B = buf[kk * PS + lpad(2 * j + 1)];
+                if (kk == 0 || 2 * kk == K) { A.y = T(0); B.y = T(0); }
+                vz[u] = (k <= H) ? cplx<T>{A.x - B.y, A.y + B.x} : cplx<T>{A.x + B.y, B.x - A.y};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < MAXP; u++) {
+            const int i = threadIdx.x + u * blockDim.x;
+            if (i < nit) {
+                const int j = (int)fdiv(i, fa.K_m), k = i - j * K;
+                buf[j * PS + lpad(k)] = vz[u];
+            }
+        }
+        __syncthreads();
+    }
+    fft_pencils_inplace<T, +1, MAXI2>(buf, H, PS, fa, tw);
+    T* dst = real + plane * K * K;
+    for (int i = threadIdx.x; i < H * K; i += blockDim.x) {
+        const int j = (int)fdiv(i, fa.K_m), x = i - j * K;
+        const cplx<T> v = buf[j * PS + lpad(x)];
+        dst[(size_t)(2 * j) * K + x] = v.x;
+        dst[(size_t)(2 * j + 1) * K + x] = v.y;
+    }
+}
+
 // ------------------------------------------ z transform + influence -------
-// For NP = 2^lnp consecutive kx of one (rhs, y): forward z-FFT of all
-// components, the RPY influence tensor (src/mob/spme.cc:201-310,
-// spme.h:114-123), inverse z-FFT, store. Mono (NC = 3): B = lm2 (1 - k^2/3)
-// (I - k k^T / k^2) applied to the 3-vector. Poly (NC = 6, spreads of f and
-// a^2 f): the reference's three mesh round trips (flags 1, 2, 2) reduce to
-// two outputs
+// A workgroup takes NP = 2^lnp consecutive (y, kx) positions of the flat
+// K x (K/2+1) half-plane index j = y (K/2+1) + kx (contiguous in memory, so
+// every z-plane read is one NP-element run) for one right-hand side: forward
+// z-FFT of all components in place, the RPY influence tensor
+// (src/mob/spme.cc:201-310, spme.h:114-123), inverse z-FFT, store.
+// Mono (NC = 3): B = lm2 (1 - k^2/3)(I - k k^T / k^2) applied to the
+// 3-vector. Poly (NC = 6, spreads of f and a^2 f): the reference's three
+// mesh round trips (flags 1, 2, 2) reduce to two outputs
 //   U = G (F f - k^2/6 F[a^2 f]),  W = -k^2/6 G F f,  G = lm2 (I - k k^T / k^2)
 // interpolated as interp(U) + a_i^2 interp(W).
+constexpr int kZThreads = 1024;
+constexpr int kZMaxI2 = 8;
+
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kZThreads)
 fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ tw_g,
                        const double* __restrict__ bsp2, int lnp, int NC, double box, double xi) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int K = fa.K, Kh = K / 2 + 1, PS = pencil_stride(K);
     const int NP = 1 << lnp;
+    const int nj = K * Kh;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-    cplx<T>* bufa = tw + K;
-    cplx<T>* bufb = bufa + NC * NP * PS;
+    cplx<T>* buf = tw + K;
     load_twiddles(tw, tw_g, K);
-    const int nkb = (Kh + NP - 1) >> lnp;
-    const int kx0 = (int)(blockIdx.x % nkb) * NP;
-    const int y = (int)((blockIdx.x / nkb) % K);
-    const int rhs = (int)(blockIdx.x / ((unsigned)nkb * K));
-    const size_t mesh_sz = (size_t)K * K * Kh;
-    cplx<T>* base = spec + (size_t)rhs * NC * mesh_sz + (size_t)y * Kh;
-    const size_t zs = (size_t)K * Kh;
+    const int nch = (nj + NP - 1) >> lnp;
+    const int j0 = (int)(blockIdx.x % nch) * NP;
+    const int rhs = (int)(blockIdx.x / nch);
+    const size_t mesh_sz = (size_t)nj * K;
+    cplx<T>* base = spec + (size_t)rhs * NC * mesh_sz + j0;
+    const int nvalid = min(NP, nj - j0);
     for (int i = threadIdx.x; i < NC * NP * K; i += blockDim.x) {
-        const int j = i & (NP - 1);
+        const int jl = i & (NP - 1);
         const int cz = i >> lnp;
         const int c = (int)fdiv(cz, fa.K_m), z = cz - c * K;
-        const int kx = kx0 + j;
-        bufa[(c * NP + j) * PS + lpad(z)] =
-            (kx < Kh) ? base[c * mesh_sz + z * zs + kx] : cplx<T>{T(0), T(0)};
+        buf[(c * NP + jl) * PS + lpad(z)] =
+            (jl < nvalid) ? base[c * mesh_sz + (size_t)z * nj + jl] : cplx<T>{T(0), T(0)};
     }
     __syncthreads();
-    cplx<T>* res = fft_pencils<T, -1>(bufa, bufb, NC * NP, PS, fa, tw);
-    cplx<T>* oth = (res == bufa) ? bufb : bufa;
+    fft_pencils_inplace<T, -1, kZMaxI2>(buf, NC * NP, PS, fa, tw);
     // influence
     const T twopi_l = (T)(2.0 * M_PI / box);
     const T inv_xi2 = (T)(1.0 / (xi * xi));
-    const int my = (y <= K / 2) ? y : y - K;
-    const T ky = twopi_l * (T)my;
     const int cs = NP * PS;    // component stride in LDS
     for (int i = threadIdx.x; i < NP * K; i += blockDim.x) {
-        const int j = (int)fdiv(i, fa.K_m), z = i - j * K;
-        const int kxi = kx0 + j;
-        if (kxi >= Kh) continue;
+        const int jl = (int)fdiv(i, fa.K_m), z = i - jl * K;
+        if (jl >= nvalid) continue;
+        const int j = j0 + jl;
+        const int y = (int)fdiv(j, fa.Kh_m), kxi = j - y * Kh;
+        const int my = (y <= K / 2) ? y : y - K;
         const int mz = (z <= K / 2) ? z : z - K;
         const T kx = twopi_l * (T)kxi;
+        const T ky = twopi_l * (T)my;
         const T kz = twopi_l * (T)mz;
         const T kk = kx * kx + ky * ky + kz * kz;
         T m2 = T(0), ex = T(0), ey = T(0), ez = T(0);
@@ -474,23 +596,23 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
             const T rk = T(1) / sqrt(kk);
             ex = kx * rk; ey = ky * rk; ez = kz * rk;
         }
-        const int o = j * PS + lpad(z);
+        const int o = jl * PS + lpad(z);
         if (NC == 3) {
             const T s = m2 * (T(1) - kk / T(3));
-            cplx<T> g0 = res[o], g1 = res[cs + o], g2 = res[2 * cs + o];
+            cplx<T> g0 = buf[o], g1 = buf[cs + o], g2 = buf[2 * cs + o];
             // (I - e e^T) g, scaled
             const T dr = ex * g0.x + ey * g1.x + ez * g2.x;
             const T di = ex * g0.y + ey * g1.y + ez * g2.y;
-            res[o] = cplx<T>{s * (g0.x - ex * dr), s * (g0.y - ex * di)};
-            res[cs + o] = cplx<T>{s * (g1.x - ey * dr), s * (g1.y - ey * di)};
-            res[2 * cs + o] = cplx<T>{s * (g2.x - ez * dr), s * (g2.y - ez * di)};
+            buf[o] = cplx<T>{s * (g0.x - ex * dr), s * (g0.y - ex * di)};
+            buf[cs + o] = cplx<T>{s * (g1.x - ey * dr), s * (g1.y - ey * di)};
+            buf[2 * cs + o] = cplx<T>{s * (g2.x - ez * dr), s * (g2.y - ez * di)};
         } else {
             const T q = -kk / T(6);
             cplx<T> f[3], a[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                f[c] = res[c * cs + o];
-                a[c] = res[(3 + c) * cs + o];
+                f[c] = buf[c * cs + o];
+                a[c] = buf[(3 + c) * cs + o];
             }
             cplx<T> u[3], w[3];
 #pragma unroll
@@ -505,20 +627,19 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
             const T e3[3] = {ex, ey, ez};
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                res[c * cs + o] = cplx<T>{m2 * (u[c].x - e3[c] * ur), m2 * (u[c].y - e3[c] * ui)};
-                res[(3 + c) * cs + o] =
+                buf[c * cs + o] = cplx<T>{m2 * (u[c].x - e3[c] * ur), m2 * (u[c].y - e3[c] * ui)};
+                buf[(3 + c) * cs + o] =
                     cplx<T>{m2 * (w[c].x - e3[c] * wr), m2 * (w[c].y - e3[c] * wi)};
             }
         }
     }
     __syncthreads();
-    const cplx<T>* out = fft_pencils<T, +1>(res, oth, NC * NP, PS, fa, tw);
+    fft_pencils_inplace<T, +1, kZMaxI2>(buf, NC * NP, PS, fa, tw);
     for (int i = threadIdx.x; i < NC * NP * K; i += blockDim.x) {
-        const int j = i & (NP - 1);
+        const int jl = i & (NP - 1);
         const int cz = i >> lnp;
         const int c = (int)fdiv(cz, fa.K_m), z = cz - c * K;
-        const int kx = kx0 + j;
-        if (kx < Kh) base[c * mesh_sz + z * zs + kx] = out[(c * NP + j) * PS + lpad(z)];
+        if (jl < nvalid) base[c * mesh_sz + (size_t)z * nj + jl] = buf[(c * NP + jl) * PS + lpad(z)];
     }
 }
 
@@ -664,6 +785,7 @@ bool make_fft_plan(int K, FFTPlan* plan) {
     plan->K = K;
     plan->nst = 0;
     int m = K;
+    while (m % 8 == 0) { plan->radix[plan->nst++] = 8; m /= 8; }
     while (m % 4 == 0) { plan->radix[plan->nst++] = 4; m /= 4; }
     while (m % 2 == 0) { plan->radix[plan->nst++] = 2; m /= 2; }
     while (m % 3 == 0) { plan->radix[plan->nst++] = 3; m /= 3; }
@@ -804,11 +926,61 @@ static size_t fft_lds_bytes(int K, int npen) {
     return (K + 2 * (size_t)npen * pencil_stride(K)) * sizeof(cplx<T>);
 }
 
+// largest dynamic LDS allocation of one workgroup on the current device
+static size_t max_lds_bytes() {
+    static size_t v = 0;
+    if (!v) {
+        int dev = 0, x = 0;
+        SDT_HIP_CHECK(hipGetDevice(&dev));
+        SDT_HIP_CHECK(hipDeviceGetAttribute(&x, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        v = (size_t)x;
+    }
+    return v;
+}
+
+template <typename T>
+static size_t plane_lds_bytes(int K) {
+    return (K + (size_t)(K / 2 + 1) * pencil_stride(K)) * sizeof(cplx<T>);
+}
+
+// the fused x + y plane kernels apply when one half-spectrum plane fits the
+// LDS of a workgroup and the per-thread butterfly / pack bounds hold:
+// 0 = not applicable, 1 = the K <= 128 variant, 2 = the K <= 180 variant
+template <typename T>
+static int plane_path(int K) {
+    if (plane_lds_bytes<T>(K) > max_lds_bytes()) return 0;
+    const int Kh = K / 2 + 1, H = K / 2, nt = kPlaneThreads;
+    if (Kh * K <= 2 * 5 * nt && H * K <= 8 * nt && H * Kh <= 8 * nt) return 1;
+    if (sizeof(T) == 4 && Kh * K <= 2 * 8 * nt && H * K <= 16 * nt && H * Kh <= 16 * nt) return 2;
+    return 0;
+}
+
+template <typename T>
+static void allow_lds(const void* fn, size_t bytes) {
+    if (bytes > 64 * 1024)
+        SDT_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 template <typename T>
 void SpmeGpu<T>::fft_forward(int b, hipStream_t s) {
     const FFTArgs fa = fft_args(plan);
     const int Kh = K / 2 + 1;
     const long long nrows = (long long)nmesh(b) * K * K;
+    if (const int pv = plane_path<T>(K)) {
+        const size_t lds = plane_lds_bytes<T>(K);
+        const dim3 grid((unsigned)(nmesh(b) * K));
+        if (pv == 1) {
+            allow_lds<T>((const void*)fft_plane_fwd_kernel<T, 5, 8>, lds);
+            hipLaunchKernelGGL((fft_plane_fwd_kernel<T, 5, 8>), grid, dim3(kPlaneThreads), lds, s,
+                               fa, d_real.get(), d_spec.get(), d_tw.get());
+        } else if constexpr (sizeof(T) == 4) {
+            allow_lds<T>((const void*)fft_plane_fwd_kernel<T, 8, 16>, lds);
+            hipLaunchKernelGGL((fft_plane_fwd_kernel<T, 8, 16>), grid, dim3(kPlaneThreads), lds, s,
+                               fa, d_real.get(), d_spec.get(), d_tw.get());
+        }
+        SDT_HIP_CHECK(hipGetLastError());
+        return;
+    }
     const int NR = 2 << pick_lnp<T>(K, 1, 48 * 1024);
     hipLaunchKernelGGL((fft_x_r2c_kernel<T>), dim3(ceil_div(nrows, NR)), dim3(256),
                        fft_lds_bytes<T>(K, NR / 2), s, fa, nrows, d_real.get(), d_spec.get(),
@@ -826,10 +998,22 @@ void SpmeGpu<T>::influence(int b, hipStream_t s) {
     const FFTArgs fa = fft_args(plan);
     const int Kh = K / 2 + 1;
     const int NC = poly ? 6 : 3;
-    const int lnp = pick_lnp<T>(K, NC, 64 * 1024);
-    hipLaunchKernelGGL((fft_z_influence_kernel<T>), dim3(ceil_div(Kh, 1 << lnp) * b * K),
-                       dim3(256), fft_lds_bytes<T>(K, NC << lnp), s, fa, d_spec.get(), d_tw.get(),
-                       d_bsp2.get(), lnp, NC, box, xi);
+    const int nj = K * Kh;
+    // pencils per workgroup: at most 16, one padded buffer of NC * NP pencils
+    // within ~110 KB (two workgroups per CU) and the butterfly bound
+    const size_t budget = std::min<size_t>(max_lds_bytes(), 112 * 1024);
+    int lnp = 4;
+    auto lds_of = [&](int l) {
+        return (K + (size_t)NC * ((size_t)1 << l) * pencil_stride(K)) * sizeof(cplx<T>);
+    };
+    while (lnp > 0 && (lds_of(lnp) > budget || NC * (K << lnp) > 2 * kZMaxI2 * kZThreads)) lnp--;
+    SDT_REQUIRE(lds_of(lnp) <= max_lds_bytes() && NC * K <= 2 * kZMaxI2 * kZThreads, SDT_EINVAL,
+                "z transform: mesh dimension too large");
+    const size_t lds = lds_of(lnp);
+    allow_lds<T>((const void*)fft_z_influence_kernel<T>, lds);
+    const unsigned nch = ceil_div(nj, (size_t)1 << lnp);
+    hipLaunchKernelGGL((fft_z_influence_kernel<T>), dim3(nch * (unsigned)b), dim3(kZThreads), lds,
+                       s, fa, d_spec.get(), d_tw.get(), d_bsp2.get(), lnp, NC, box, xi);
     SDT_HIP_CHECK(hipGetLastError());
 }
 
@@ -838,6 +1022,21 @@ void SpmeGpu<T>::fft_backward(int b, hipStream_t s) {
     const FFTArgs fa = fft_args(plan);
     const int Kh = K / 2 + 1;
     const long long nrows = (long long)nmesh(b) * K * K;
+    if (const int pv = plane_path<T>(K)) {
+        const size_t lds = plane_lds_bytes<T>(K);
+        const dim3 grid((unsigned)(nmesh(b) * K));
+        if (pv == 1) {
+            allow_lds<T>((const void*)fft_plane_inv_kernel<T, 5, 8>, lds);
+            hipLaunchKernelGGL((fft_plane_inv_kernel<T, 5, 8>), grid, dim3(kPlaneThreads), lds, s,
+                               fa, d_spec.get(), d_real.get(), d_tw.get());
+        } else if constexpr (sizeof(T) == 4) {
+            allow_lds<T>((const void*)fft_plane_inv_kernel<T, 8, 16>, lds);
+            hipLaunchKernelGGL((fft_plane_inv_kernel<T, 8, 16>), grid, dim3(kPlaneThreads), lds, s,
+                               fa, d_spec.get(), d_real.get(), d_tw.get());
+        }
+        SDT_HIP_CHECK(hipGetLastError());
+        return;
+    }
     const int lnp = pick_lnp<T>(K, 1, 48 * 1024);
     hipLaunchKernelGGL((fft_y_kernel<T, +1>), dim3(ceil_div(Kh, 1 << lnp) * nmesh(b) * K),
                        dim3(256), fft_lds_bytes<T>(K, 1 << lnp), s, fa, d_spec.get(), d_tw.get(),

@@ -197,3 +197,19 @@ def test_workload_size_matches_oracle(sdt, orc, prec, tol):
         got = getattr(g, mode + "_mul_vector")(f)
         ref = o.mul_vector(f, mode=mode)
         assert orc.frob_rel_err(ref, got) < tol, mode
+
+
+@pytest.mark.parametrize("K", [160, 192])
+def test_large_mesh_fft_paths(sdt, orc, K):
+    # K = 160: FP32 uses the wide fused-plane FFT variant, FP64 the separate
+    # x / y passes (a plane no longer fits LDS); K = 192: both separate
+    n, box = 400, 40.0
+    g, o, pos, rdi = _pair(sdt, orc, n, box, 0.9, 6.0, K, 6, False, 21, sdt.FP64)
+    f = np.random.default_rng(22).standard_normal((3 * n, 3))
+    ref = o.mul_vector(f, mode="recip")
+    got = g.recip_mul_vector(f)
+    assert orc.frob_rel_err(ref, got) < 1e-10
+    g32 = sdt.MobSpme(n, rdi, box, 0.9, 6.0, K, 6, sdt.FP32)
+    assert g32.init()
+    g32.update(pos)
+    assert orc.frob_rel_err(got, g32.recip_mul_vector(f)) < 1e-5
