@@ -110,6 +110,8 @@ void MobSpmeGpu::update(const double* pos, const double* rdi, bool dev) {
     SDT_HIP_CHECK(hipGetLastError());
     // real-space pairs in sorted order: rows by perm, columns through iperm
     real_nnz_ = pairs_.build(d_pos_.get(), d_perm_.get(), d_iperm_.get(), false, s);
+    ovl_.build(npos_, pairs_.d_rowptr.get(), pairs_.d_colidx.get(), d_pos_s_.get(),
+               d_rdi_s_.get(), box_, s);
     updated_ = true;
 }
 
@@ -133,6 +135,7 @@ void MobSpmeGpu::apply_packed(int mode, int b, double alpha, const double* X, do
         else
             real_matvec<double>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha,
                                 X, rb, Y, s);
+        real_overlap(ovl_, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha, X, Y, s);
     }
 }
 

@@ -53,6 +53,7 @@ class MobSpmeGpu {
     std::unique_ptr<SpmeGpu<float>> spme32_;
     std::unique_ptr<SpmeGpu<double>> spme64_;
     PairListGpu pairs_;
+    RealOverlap ovl_;
     SortScratch sc_;
     DevBuf<double> d_pos_, d_rdi_, d_pos_s_, d_rdi_s_;
     DevBuf<int> d_key_, d_perm_, d_iperm_, d_tstart_;

@@ -7,6 +7,21 @@ namespace sdt {
 // Y = beta Y + alpha * M_real X over a CSR pair list (self included) whose
 // rows and columns index the same particle order as pos/rdi and the packed
 // vectors X, Y (row-major 3N x b).
+// Overlapping pairs (r < a_i + a_j) of a CSR pair list, built once per
+// update: per-row CSR over the rows that have any (rows[]).
+struct RealOverlap {
+    DevBuf<int> cnt, has, rowptr, hrp, col, rows, scratch;
+    long long npairs = 0;
+    int nrows = 0;
+    void build(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
+               const double* d_rdi, double box, hipStream_t s);
+};
+
+// Y += alpha * (overlapping-pair blocks) X, in FP64
+void real_overlap(const RealOverlap& ov, const double* d_pos, const double* d_rdi, double box,
+                  double xi, int b, double alpha, const double* X, double* Y, hipStream_t s);
+
+// without the overlapping pairs (those are added by real_overlap)
 template <typename T>
 void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
                  const double* d_rdi, double box, double xi, int b, double alpha, const double* X,

@@ -8,8 +8,15 @@
 // thread, so a matvec streams only the column indices and the input vector.
 #include <cmath>
 #include "real_gpu.hpp"
+#include "sort_gpu.hpp"
 
 namespace sdt {
+
+// squared min-image distance with explicit fused operations, so that every
+// kernel that classifies a pair as overlapping rounds identically
+__device__ __forceinline__ double pair_rr(double dx, double dy, double dz) {
+    return fma(dx, dx, fma(dy, dy, dz * dz));
+}
 
 template <typename T>
 __device__ __forceinline__ void ewald_scalars(T xi, T r, T aa, T ab, T* xa, T* ya) {
@@ -31,20 +38,25 @@ __device__ __forceinline__ void ewald_scalars(T xi, T r, T aa, T ab, T* xa, T* y
     *xa = T(0.75) * (b0 + b2 * s6) + *ya;
 }
 
-// Y[(3 i + c) b + r] = beta Y + alpha sum_j B_ij X_j for r in [r0, r0 + RC).
-// LPR consecutive lanes share a row, each taking every LPR-th pair, so a
-// row's gathers are in flight together; partial sums are combined with
-// cross-lane shuffles.
+// Y[(3 i + c) b + r] = beta Y + alpha sum_j B_ij X_j for r in [r0, r0 + RC),
+// over the self block and the non-overlapping pairs (r >= a_i + a_j);
+// overlapping pairs are added by real_overlap_kernel. LPR consecutive lanes
+// share a row, each taking every LPR-th pair, so a row's gathers are in
+// flight together; partial sums are combined with cross-lane shuffles.
 template <typename T, int RC, int LPR>
 __global__ void __launch_bounds__(256)
 real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restrict__ colidx,
                    const double* __restrict__ pos, const double* __restrict__ rdi, double box,
                    double xi, int b, double alpha, const double* __restrict__ X, double beta,
                    double* __restrict__ Y) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    // rows are in spatially sorted order: XCD-contiguous block ranges keep
+    // the gathered neighbour rows of X within one L2
+    const unsigned lid = xcd_logical_id(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int bxl = (int)(lid % gridDim.x), byl = (int)(lid / gridDim.x);
+    const int tid = bxl * blockDim.x + threadIdx.x;
     const int i = tid / LPR, sub = tid % LPR;
     const bool valid = i < npos;
-    const int r0 = blockIdx.y * RC;
+    const int r0 = byl * RC;
     const int rc = min(RC, b - r0);
     T acc[3][RC];
 #pragma unroll
@@ -82,33 +94,19 @@ real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restri
                 const double dy = remainder(pos[3 * j + 1] - y1, box);
                 const double dz = remainder(pos[3 * j + 2] - z1, box);
                 const double ab_d = rdi[j];
-                const double rr_d = dx * dx + dy * dy + dz * dz;
-                if (rr_d < (aa_d + ab_d) * (aa_d + ab_d)) {
-                    // overlapping pair: the overlap correction cancels the
-                    // 1/r^3 Ewald terms, so this rare block is formed in FP64
-                    const double r = sqrt(rr_d);
-                    double xa, ya;
-                    ewald_scalars<double>(xi, r, aa_d, ab_d, &xa, &ya);
-                    const double s2 = aa_d * aa_d + ab_d * ab_d;
-                    xa += -(1.5 - s2 * 0.5 / rr_d) / r + 2.0 / (aa_d + ab_d) - 3.0 * r / 8.0 / s2;
-                    ya += -(0.75 + s2 * 0.25 / rr_d) / r + 2.0 / (aa_d + ab_d) - 9.0 * r / 16.0 / s2;
-                    const double ex = dx / r, ey = dy / r, ez = dz / r;
-                    const double d = xa - ya;
-                    bxx = (T)(d * ex * ex + ya); bxy = (T)(d * ex * ey); bxz = (T)(d * ex * ez);
-                    byy = (T)(d * ey * ey + ya); byz = (T)(d * ey * ez); bzz = (T)(d * ez * ez + ya);
-                } else {
-                    const T rx = (T)dx, ry = (T)dy, rz = (T)dz;
-                    const T rr = rx * rx + ry * ry + rz * rz;
-                    const T r = sqrt(rr);
-                    const T ab = (T)ab_d;
-                    T xa, ya;
-                    ewald_scalars<T>(txi, r, aa, ab, &xa, &ya);
-                    const T ir = T(1) / r;
-                    const T ex = rx * ir, ey = ry * ir, ez = rz * ir;
-                    const T d = xa - ya;
-                    bxx = d * ex * ex + ya; bxy = d * ex * ey; bxz = d * ex * ez;
-                    byy = d * ey * ey + ya; byz = d * ey * ez; bzz = d * ez * ez + ya;
-                }
+                const double sab = aa_d + ab_d;
+                if (pair_rr(dx, dy, dz) < sab * sab) continue;   // overlap pass
+                const T rx = (T)dx, ry = (T)dy, rz = (T)dz;
+                const T rr = rx * rx + ry * ry + rz * rz;
+                const T r = sqrt(rr);
+                const T ab = (T)ab_d;
+                T xa, ya;
+                ewald_scalars<T>(txi, r, aa, ab, &xa, &ya);
+                const T ir = T(1) / r;
+                const T ex = rx * ir, ey = ry * ir, ez = rz * ir;
+                const T d = xa - ya;
+                bxx = d * ex * ex + ya; bxy = d * ex * ey; bxz = d * ex * ez;
+                byy = d * ey * ey + ya; byz = d * ey * ez; bzz = d * ez * ez + ya;
             }
 #pragma unroll
             for (int r = 0; r < RC; r++) {
@@ -142,7 +140,13 @@ void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const doubl
                  const double* d_rdi, double box, double xi, int b, double alpha, const double* X,
                  double beta, double* Y, hipStream_t s) {
     constexpr int LPR = 8;
-    if (b >= 8) {
+    if (sizeof(T) == 4 && b >= 16) {
+        // 16 columns: one 128-byte line per component row of a neighbour
+        constexpr int RC = sizeof(T) == 4 ? 16 : 8;
+        dim3 grid(ceil_div((size_t)npos * LPR, 256), ceil_div(b, RC));
+        hipLaunchKernelGGL((real_matvec_kernel<T, RC, LPR>), grid, dim3(256), 0, s, npos,
+                           d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X, beta, Y);
+    } else if (b >= 8) {
         constexpr int RC = 8;
         dim3 grid(ceil_div((size_t)npos * LPR, 256), ceil_div(b, RC));
         hipLaunchKernelGGL((real_matvec_kernel<T, RC, LPR>), grid, dim3(256), 0, s, npos,
@@ -153,6 +157,122 @@ void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const doubl
         hipLaunchKernelGGL((real_matvec_kernel<T, RC, LPR>), grid, dim3(256), 0, s, npos,
                            d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X, beta, Y);
     }
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ overlaps -----
+// Pairs closer than a_i + a_j carry the overlap correction of
+// src/mob/mob_spme.cc:170-175, which cancels the 1/r^3 Ewald terms: they are
+// listed once per update and always evaluated in FP64.
+__device__ __forceinline__ bool overlapping(int i, int j, const double* pos, const double* rdi,
+                                            double box) {
+    if (i == j) return false;
+    const double dx = remainder(pos[3 * j] - pos[3 * i], box);
+    const double dy = remainder(pos[3 * j + 1] - pos[3 * i + 1], box);
+    const double dz = remainder(pos[3 * j + 2] - pos[3 * i + 2], box);
+    const double sab = rdi[i] + rdi[j];
+    return pair_rr(dx, dy, dz) < sab * sab;
+}
+
+__global__ void overlap_count_kernel(int npos, const int* __restrict__ rowptr,
+                                     const int* __restrict__ colidx, const double* __restrict__ pos,
+                                     const double* __restrict__ rdi, double box,
+                                     int* __restrict__ cnt, int* __restrict__ has) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npos) return;
+    int c = 0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) c += overlapping(i, colidx[k], pos, rdi, box);
+    cnt[i] = c;
+    has[i] = c > 0;
+}
+
+__global__ void overlap_fill_kernel(int npos, const int* __restrict__ rowptr,
+                                    const int* __restrict__ colidx, const double* __restrict__ pos,
+                                    const double* __restrict__ rdi, double box,
+                                    const int* __restrict__ orp, const int* __restrict__ hrp,
+                                    int* __restrict__ ocol, int* __restrict__ orows) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npos) return;
+    int o = orp[i];
+    if (o == orp[i + 1]) return;
+    orows[hrp[i]] = i;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        const int j = colidx[k];
+        if (overlapping(i, j, pos, rdi, box)) ocol[o++] = j;
+    }
+}
+
+void RealOverlap::build(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
+                        const double* d_rdi, double box, hipStream_t s) {
+    cnt.reserve((size_t)npos);
+    has.reserve((size_t)npos);
+    rowptr.reserve((size_t)npos + 1);
+    hrp.reserve((size_t)npos + 1);
+    const unsigned g = ceil_div((size_t)npos, 256);
+    hipLaunchKernelGGL(overlap_count_kernel, dim3(g), dim3(256), 0, s, npos, d_rowptr, d_colidx,
+                       d_pos, d_rdi, box, cnt.get(), has.get());
+    SDT_HIP_CHECK(hipGetLastError());
+    exclusive_scan(cnt.get(), npos, rowptr.get(), scratch, s);
+    exclusive_scan(has.get(), npos, hrp.get(), scratch, s);
+    int tot[2] = {0, 0};
+    SDT_HIP_CHECK(hipMemcpyAsync(&tot[0], rowptr.get() + npos, sizeof(int), hipMemcpyDeviceToHost, s));
+    SDT_HIP_CHECK(hipMemcpyAsync(&tot[1], hrp.get() + npos, sizeof(int), hipMemcpyDeviceToHost, s));
+    SDT_HIP_CHECK(hipStreamSynchronize(s));
+    npairs = tot[0];
+    nrows = tot[1];
+    if (nrows == 0) return;
+    col.reserve((size_t)npairs);
+    rows.reserve((size_t)nrows);
+    hipLaunchKernelGGL(overlap_fill_kernel, dim3(g), dim3(256), 0, s, npos, d_rowptr, d_colidx,
+                       d_pos, d_rdi, box, rowptr.get(), hrp.get(), col.get(), rows.get());
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
+// Y[(3 i + c) b + r] += alpha sum over the overlapping pairs of row i, one
+// thread per (listed row, column); runs after real_matvec_kernel.
+__global__ void real_overlap_kernel(int nrows, const int* __restrict__ rows,
+                                    const int* __restrict__ orp, const int* __restrict__ ocol,
+                                    const double* __restrict__ pos, const double* __restrict__ rdi,
+                                    double box, double xi, int b, double alpha,
+                                    const double* __restrict__ X, double* __restrict__ Y) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nrows * b) return;
+    const int i = rows[t / b];
+    const int r = (int)(t % b);
+    const double aa = rdi[i];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = orp[i]; k < orp[i + 1]; k++) {
+        const int j = ocol[k];
+        const double dx = remainder(pos[3 * j] - pos[3 * i], box);
+        const double dy = remainder(pos[3 * j + 1] - pos[3 * i + 1], box);
+        const double dz = remainder(pos[3 * j + 2] - pos[3 * i + 2], box);
+        const double ab = rdi[j];
+        const double rr = dx * dx + dy * dy + dz * dz;
+        const double rv = sqrt(rr);
+        double xa, ya;
+        ewald_scalars<double>(xi, rv, aa, ab, &xa, &ya);
+        const double s2 = aa * aa + ab * ab;
+        xa += -(1.5 - s2 * 0.5 / rr) / rv + 2.0 / (aa + ab) - 3.0 * rv / 8.0 / s2;
+        ya += -(0.75 + s2 * 0.25 / rr) / rv + 2.0 / (aa + ab) - 9.0 * rv / 16.0 / s2;
+        const double ex = dx / rv, ey = dy / rv, ez = dz / rv;
+        const double d = xa - ya;
+        const double* xj = X + (size_t)(3 * j) * b + r;
+        const double v0 = xj[0], v1 = xj[b], v2 = xj[2 * b];
+        acc[0] += (d * ex * ex + ya) * v0 + d * ex * ey * v1 + d * ex * ez * v2;
+        acc[1] += d * ex * ey * v0 + (d * ey * ey + ya) * v1 + d * ey * ez * v2;
+        acc[2] += d * ex * ez * v0 + d * ey * ez * v1 + (d * ez * ez + ya) * v2;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) Y[(size_t)(3 * i + c) * b + r] += alpha * acc[c];
+}
+
+void real_overlap(const RealOverlap& ov, const double* d_pos, const double* d_rdi, double box,
+                  double xi, int b, double alpha, const double* X, double* Y, hipStream_t s) {
+    if (ov.nrows == 0) return;
+    const size_t nt = (size_t)ov.nrows * b;
+    hipLaunchKernelGGL(real_overlap_kernel, dim3(ceil_div(nt, 256)), dim3(256), 0, s, ov.nrows,
+                       ov.rows.get(), ov.rowptr.get(), ov.col.get(), d_pos, d_rdi, box, xi, b,
+                       alpha, X, Y);
     SDT_HIP_CHECK(hipGetLastError());
 }
 

@@ -75,6 +75,17 @@ struct cplx {
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): linear id i runs on XCD i % 8. This maps the linear id to a logical
+// id so that every XCD receives one contiguous range of the logical order
+// (spatial neighbours share that XCD's L2). A bijection on [0, n).
+__device__ __forceinline__ unsigned xcd_logical_id(unsigned i, unsigned n) {
+    constexpr unsigned kXcd = 8;
+    const unsigned q = n / kXcd, r = n % kXcd;
+    const unsigned x = i % kXcd, k = i / kXcd;
+    return x * q + min(x, r) + k;
+}
+
 // FFT plan: radices whose product is K (each in {2,3,4,5}).
 struct FFTPlan {
     int K = 0;

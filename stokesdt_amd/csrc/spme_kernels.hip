@@ -86,7 +86,6 @@ __global__ void spme_gather_kernel(int npos, const int* __restrict__ perm,
 // (base points in the 4+1 x 2 x 2 surrounding tiles, filtered by overlap) are
 // staged in LDS with their weights and forces; every lane then walks the
 // staged list. No atomics: every mesh point is written once, by its owner.
-constexpr int kStageChunk = 128;
 
 // exclusive prefix over the workgroup of a 0/1 predicate; returns this
 // thread's slot and the total (all threads must call)
@@ -120,17 +119,24 @@ spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ 
                      const double* __restrict__ a2, const double* __restrict__ X, int b, int RB,
                      int NC, int cap, T* __restrict__ mesh) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int REC = 3 * P + 3 + RBNC;      // wx[P] wy[P] wz[P] bx by bz f[RBNC]
+    // record: x weights in a zero-padded window wxp[0 .. P+13] (wxp[7 + k] =
+    // wx[k]), wy[P], wz[P], bx, by, bz, f[RBNC]. A wave reads the weight of
+    // its tile point x at wxp[7 - sd + x], sd = base - tile start in
+    // [-(P-1), ts-1]: no per-point selects or range tests.
+    constexpr int OWY = P + 14, OWZ = 2 * P + 14, OB = 3 * P + 14, OF = 3 * P + 17;
+    constexpr int REC = OF + RBNC;
     T* stage = reinterpret_cast<T*>(smem_raw);
     __shared__ int s_wave[4];
     __shared__ int s_off[21];                   // candidate offsets of the base tiles
     __shared__ int s_beg[20];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // block -> (super-tile x, tile y, tile z), rhs chunk
-    const int sxi = blockIdx.x % nsx;
-    const int tyz = blockIdx.x / nsx;
+    // block -> (super-tile x, tile y, tile z), rhs chunk; XCD-contiguous
+    const unsigned lid = xcd_logical_id(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int bxl = (int)(lid % gridDim.x), byl = (int)(lid / gridDim.x);
+    const int sxi = bxl % nsx;
+    const int tyz = bxl / nsx;
     const int ty = tyz % ntile, tz = tyz / ntile;
-    const int r0 = blockIdx.y * RB;
+    const int r0 = byl * RB;
     const int rb = min(RB, b - r0);
     const int rbnc = rb * NC;
     const int stx = sxi * 4;                    // first x tile of the super-tile
@@ -173,30 +179,27 @@ spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ 
         if (!lane_on) return;
         for (int sidx = 0; sidx < n; sidx++) {
             const T* rec = stage + (size_t)sidx * REC;
-            const int bx = (int)rec[3 * P];
+            const int bx = (int)rec[OB], by = (int)rec[OB + 1], bz = (int)rec[OB + 2];
             const int dxw = wrapk(bx - wx0, K);
             if (!(dxw < ts || K - dxw < P)) continue;          // wave-uniform
-            const int by = (int)rec[3 * P + 1], bz = (int)rec[3 * P + 2];
-            T wxr[P], f[RBNC];
-#pragma unroll
-            for (int k = 0; k < P; k++) wxr[k] = rec[k];
-#pragma unroll
-            for (int q = 0; q < RBNC; q++) f[q] = rec[3 * P + 3 + q];
             const int ly = wrapk(yl - by, K), lz = wrapk(zl - bz, K);
             if (ly >= P || lz >= P) continue;
-            const T wyz = rec[P + ly] * rec[2 * P + lz];
+            const int sd = dxw < ts ? dxw : dxw - K;
+            const T* wxp = rec + (7 - sd);
+            // every load of the candidate is issued before the first use;
+            // points x >= ts accumulate window zeros / unused values and are
+            // never stored
+            T f[RBNC], wx8[8];
+#pragma unroll
+            for (int q = 0; q < RBNC; q++) f[q] = rec[OF + q];
+#pragma unroll
+            for (int x = 0; x < 8; x++) wx8[x] = wxp[x];
+            const T wyz = rec[OWY + ly] * rec[OWZ + lz];
 #pragma unroll
             for (int x = 0; x < 8; x++) {
-                if (x >= ts) break;
-                const int lx = wrapk(wx0 + x - bx, K);
-                T w = T(0);
+                const T w = wx8[x] * wyz;
 #pragma unroll
-                for (int k = 0; k < P; k++) w = (lx == k) ? wxr[k] : w;
-                if (lx < P) {
-                    w *= wyz;
-#pragma unroll
-                    for (int q = 0; q < RBNC; q++) acc[x][q] += w * f[q];
-                }
+                for (int q = 0; q < RBNC; q++) acc[x][q] += w * f[q];
             }
         }
     };
@@ -268,16 +271,24 @@ spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ 
                             const double x = X[(size_t)(3 * is + q % 3) * b + r0 + r];
                             fv = (T)(scaled ? x * sc : x);
                         }
-                        rec[3 * P + 3 + q] = fv;
+                        rec[OF + q] = fv;
                     }
                     T w[P];
+                    bspline_weights<T, P>(frac[3 * is], w);
 #pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        bspline_weights<T, P>(frac[3 * is + d], w);
+                    for (int k = 0; k < 7; k++) rec[k] = T(0);
 #pragma unroll
-                        for (int k = 0; k < P; k++) rec[d * P + k] = w[k];
-                        rec[3 * P + d] = (T)b3u[u][d];
-                    }
+                    for (int k = 0; k < P; k++) rec[7 + k] = w[k];
+#pragma unroll
+                    for (int k = 7 + P; k < P + 14; k++) rec[k] = T(0);
+                    bspline_weights<T, P>(frac[3 * is + 1], w);
+#pragma unroll
+                    for (int k = 0; k < P; k++) rec[OWY + k] = w[k];
+                    bspline_weights<T, P>(frac[3 * is + 2], w);
+#pragma unroll
+                    for (int k = 0; k < P; k++) rec[OWZ + k] = w[k];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) rec[OB + d] = (T)b3u[u][d];
                 }
                 nst += take;
                 done += take;
@@ -660,19 +671,25 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
                    const T* __restrict__ mesh, int b, int RB, int NC, double alpha, double beta,
                    double* __restrict__ Y) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int sxi = blockIdx.x % nsx;
-    const int tyz = blockIdx.x / nsx;
+    const unsigned lid = xcd_logical_id(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int bxl = (int)(lid % gridDim.x), byl = (int)(lid / gridDim.x);
+    const int sxi = bxl % nsx;
+    const int tyz = bxl / nsx;
     const int ty = tyz % ntile, tz = tyz / ntile;
-    const int r0 = blockIdx.y * RB;
+    const int r0 = byl * RB;
     const int rb = min(RB, b - r0);
     const int stx = sxi * sxt;
     const int nxt = min(sxt, ntile - stx);
     const int sx0 = stx * ts, sy0 = ty * ts, sz0 = tz * ts;
     const int RX = nxt * ts + P - 1, RY = ts + P - 1, RZ = ts + P - 1;
     const int nreg = RX * RY * RZ;
-    // multiply-high reciprocals of RX and RY (the region index is < 2^17)
-    const unsigned rx_m = 0xffffffffu / (unsigned)RX + 1u;
-    const unsigned ry_m = 0xffffffffu / (unsigned)RY + 1u;
+    // staging: wave-uniform row loop, lane = x offset within the row
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwave = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int nrow = RY * RZ;
+    const bool lane_x = lane < RX;
+    const int gx = wrapk(sx0 + (lane_x ? lane : 0), K);
     // LDS: region [RZ][RY][RX] | weights [CH][3][P] | bases [CH][3] |
     //      partial sums [CH][P] | results [CH][RB][3] (double)
     T* region = reinterpret_cast<T*>(smem_raw);
@@ -706,11 +723,27 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
             for (int q = 0; q < NC; q++) {
                 __syncthreads();
                 const T* m = mesh + (size_t)((r0 + r) * NC + q) * K3;
-                for (int i = threadIdx.x; i < nreg; i += blockDim.x) {
-                    const int row = (int)fdiv(i, rx_m), lx = i - row * RX;
-                    const int lz = (int)fdiv(row, ry_m), ly = row - lz * RY;
-                    region[i] = m[((size_t)wrapk(sz0 + lz, K) * K + wrapk(sy0 + ly, K)) * K +
-                                  wrapk(sx0 + lx, K)];
+                // one region row (RX <= 64 x values) per wave and step, lanes
+                // over x; the row address is wave-uniform (scalar) and 8 rows
+                // are loaded before their LDS stores
+                constexpr int UNR = 8;
+                for (int row0 = wave * UNR; row0 < nrow; row0 += nwave * UNR) {
+                    T v[UNR];
+#pragma unroll
+                    for (int u = 0; u < UNR; u++) {
+                        const int row = row0 + u;
+                        v[u] = T(0);
+                        if (row < nrow && lane_x) {
+                            const int lz = row / RY, ly = row - lz * RY;
+                            const T* src = m + ((size_t)wrapk(sz0 + lz, K) * K + wrapk(sy0 + ly, K)) * K;
+                            v[u] = src[gx];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; u++) {
+                        const int row = row0 + u;
+                        if (row < nrow && lane_x) region[row * RX + lane] = v[u];
+                    }
                 }
                 __syncthreads();
                 for (int item = threadIdx.x; item < cnt * P; item += blockDim.x) {
@@ -892,9 +925,11 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s) {
     const int RBNC = poly ? 12 : 12;
     const int RB = RBNC / NC;
     const int nsx = (ntile + 3) / 4;
-    const int rec = 3 * porder + 3 + RBNC;
-    int cap = (int)(49152 / (rec * sizeof(T)));
-    SDT_REQUIRE(cap >= kStageChunk, SDT_EINVAL, "spread: staging buffer too small");
+    const int rec = 3 * porder + 17 + RBNC;
+    // staging list of up to 64 KB (any capacity works: a full stage is
+    // accumulated and emptied)
+    const int cap = (int)(65536 / (rec * sizeof(T)));
+    SDT_REQUIRE(cap >= 64, SDT_EINVAL, "spread: staging buffer too small");
     const size_t lds = (size_t)cap * rec * sizeof(T);
     dim3 grid(nsx * ntile * ntile, ceil_div(b, RB));
 #define SDT_SPREAD(PP)                                                                        \
@@ -1070,6 +1105,7 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
     }
     const size_t lds = lds_of(sxt, ch);
     SDT_REQUIRE(lds <= 64 * 1024, SDT_EINVAL, "interp: LDS region too large");
+    SDT_REQUIRE(sxt * ts + porder - 1 <= 64, SDT_EINVAL, "interp: region row exceeds a wave");
     const int nsx = (ntile + sxt - 1) / sxt;
     dim3 grid(nsx * ntile * ntile, ceil_div(b, RB));
 #define SDT_INTERP(PP)                                                                        \
