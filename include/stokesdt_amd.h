@@ -38,6 +38,7 @@ typedef struct sdt_brwn_s* sdt_brwn_t;
 typedef struct sdt_pairlist_s* sdt_pairlist_t;
 typedef struct sdt_force_s* sdt_force_t;
 typedef struct sdt_bd_s* sdt_bd_t;
+typedef struct sdt_simbox_s* sdt_simbox_t;
 
 /* Thread-local message of the last failing call on this thread. */
 const char* sdt_last_error(void);
@@ -151,6 +152,40 @@ int sdt_bd_advance(sdt_bd_t bd, int nsteps);
 int sdt_bd_get_positions(sdt_bd_t bd, double* pos);
 int sdt_bd_last_iters(sdt_bd_t bd, int* iters, int cap, int* ngroups);
 int sdt_bd_destroy(sdt_bd_t bd);
+
+/* ---------------------------------------------------------------- SimBox
+ * Replaces BDSimBox(config_file, model_file, xyz_file)
+ * (src/simbox/simbox_bd.cc:31-294) with Init and Advance (:296-366), and the
+ * file formats of MoleculeIO (src/util/molecule_io.cc:25-414):
+ *  - config "key value" lines with the reference keys (mob-method must be
+ *    spme, brownian-method lanczos; spme-xi, spme-rmax, spme-nfft-grids,
+ *    spme-porder, mob-interval, lanczos-maxiters/-maxnrhs/-tol, delta-t,
+ *    traj-interval, traj-startstep, traj-output) plus "precision fp64|fp32"
+ *    and "seed <uint64>";
+ *  - model lines "par type radius charge" and "steric r0 k0" (bond lines
+ *    are rejected);
+ *  - XYZ: npos, "step time Lx Ly Lz [seed]", "type x y z"; the box must be
+ *    cubic.
+ * create parses the files on the host (no device use); init allocates the
+ * device state. Trajectory frames and sdt_simbox_write_xyz keep particle
+ * types and full precision and record the seed, so any frame restarts the
+ * run (a start inside a mobility interval recomputes that interval's
+ * displacements from the restart positions).
+ * sdt_simbox_params: dparams[7] = {spme-xi, spme-rmax, lanczos-tol,
+ * steric r0, steric k0, delta-t, 0}; iparams[9] = {spme-nfft-grids,
+ * spme-porder, mob-interval, lanczos-maxiters, lanczos-maxnrhs,
+ * traj-interval, traj-startstep, precision, seed}. */
+int sdt_simbox_create(const char* config_file, const char* model_file, const char* xyz_file,
+                      sdt_simbox_t* out);
+int sdt_simbox_init(sdt_simbox_t sb);
+int sdt_simbox_advance(sdt_simbox_t sb, int nsteps);
+int sdt_simbox_npos(sdt_simbox_t sb, int* npos);
+int sdt_simbox_state(sdt_simbox_t sb, int64_t* step, double* time);
+int sdt_simbox_params(sdt_simbox_t sb, double* dparams, int64_t* iparams);
+int sdt_simbox_get_positions(sdt_simbox_t sb, double* pos);
+int sdt_simbox_get_radii(sdt_simbox_t sb, double* rdi);
+int sdt_simbox_write_xyz(sdt_simbox_t sb, const char* path);
+int sdt_simbox_destroy(sdt_simbox_t sb);
 
 /* ----------------------------------------------------------------- misc */
 /* Gaussian block of the BD driver (Philox-4x32-10 + Box-Muller, FP64):

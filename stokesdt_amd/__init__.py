@@ -77,6 +77,17 @@ SIGNATURES = {
     "sdt_bd_get_positions": (_I, [_P, _P]),
     "sdt_bd_last_iters": (_I, [_P, _P, _I, _PI]),
     "sdt_bd_destroy": (_I, [_P]),
+    "sdt_simbox_create": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                               ctypes.POINTER(_P)]),
+    "sdt_simbox_init": (_I, [_P]),
+    "sdt_simbox_advance": (_I, [_P, _I]),
+    "sdt_simbox_npos": (_I, [_P, _PI]),
+    "sdt_simbox_state": (_I, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_D)]),
+    "sdt_simbox_params": (_I, [_P, _P, _P]),
+    "sdt_simbox_get_positions": (_I, [_P, _P]),
+    "sdt_simbox_get_radii": (_I, [_P, _P]),
+    "sdt_simbox_write_xyz": (_I, [_P, ctypes.c_char_p]),
+    "sdt_simbox_destroy": (_I, [_P]),
     "sdt_gaussian": (_I, [ctypes.c_uint64, ctypes.c_uint32, _I64, _P]),
     "sdt_device_synchronize": (_I, []),
     "sdt_debug_gram": (_I, [_I64, _I, _P, _P, _P]),
@@ -358,6 +369,63 @@ class StericForce:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             lib().sdt_steric_destroy(self.h)
+            self.h = None
+
+
+class SimBox:
+    """BDSimBox(config_file, model_file, xyz_file) (src/simbox/simbox_bd.cc:31-366)
+    with the reference's .cfg / .mod / XYZ formats, on the GPU BD step.
+    Construction parses the files only; init() allocates the device state."""
+
+    DKEYS = ("spme_xi", "spme_rmax", "lanczos_tol", "steric_r0", "steric_k0", "delta_t")
+    IKEYS = ("spme_dim", "spme_porder", "mob_interval", "lanczos_maxiters", "lanczos_maxnrhs",
+             "traj_interval", "traj_start", "precision", "seed")
+
+    def __init__(self, config_file, model_file, xyz_file):
+        h = _P()
+        _check(lib().sdt_simbox_create(str(config_file).encode(), str(model_file).encode(),
+                                       str(xyz_file).encode(), ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_int(0)
+        _check(lib().sdt_simbox_npos(self.h, ctypes.byref(n)))
+        self.npos = n.value
+
+    def init(self):
+        _check(lib().sdt_simbox_init(self.h))
+
+    def advance(self, nsteps):
+        _check(lib().sdt_simbox_advance(self.h, int(nsteps)))
+
+    def state(self):
+        st, t = ctypes.c_int64(0), ctypes.c_double(0.0)
+        _check(lib().sdt_simbox_state(self.h, ctypes.byref(st), ctypes.byref(t)))
+        return st.value, t.value
+
+    def params(self):
+        d = np.zeros(7)
+        i = np.zeros(9, dtype=np.int64)
+        _check(lib().sdt_simbox_params(self.h, _ptr(d), _ptr(i)))
+        out = dict(zip(self.DKEYS, d.tolist()))
+        out.update(zip(self.IKEYS, i.tolist()))
+        out["seed"] = int(np.uint64(i[8]))
+        return out
+
+    def positions(self):
+        out = np.zeros(3 * self.npos)
+        _check(lib().sdt_simbox_get_positions(self.h, _ptr(out)))
+        return out
+
+    def radii(self):
+        out = np.zeros(self.npos)
+        _check(lib().sdt_simbox_get_radii(self.h, _ptr(out)))
+        return out
+
+    def write_xyz(self, path):
+        _check(lib().sdt_simbox_write_xyz(self.h, str(path).encode()))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_simbox_destroy(self.h)
             self.h = None
 
 

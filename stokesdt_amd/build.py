@@ -1,4 +1,5 @@
-"""Builds stokesdt_amd/libstokesdt_amd.so for gfx950 with hipcc (in-tree)."""
+"""Builds stokesdt_amd/libstokesdt_amd.so for gfx950 with hipcc (in-tree) and the
+command-line BD driver stokesdt_amd/stokesdt_bd linked against it."""
 import concurrent.futures
 import os
 import subprocess
@@ -8,6 +9,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libstokesdt_amd.so")
+PKG = HERE
+EXE = os.path.join(HERE, "stokesdt_bd")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SDT_OFFLOAD_ARCH", "gfx950")
 
@@ -19,9 +22,10 @@ HIP_SOURCES = [
     ("real_kernels.hip", []),
     ("block_kernels.hip", []),
     ("mob_gpu.hip", []),
+    ("simbox.hip", []),
     ("capi.hip", []),
 ]
-HOST_SOURCES = [("host_linalg.cpp", []), ("host_util.cpp", [])]
+HOST_SOURCES = [("host_linalg.cpp", []), ("host_util.cpp", []), ("molecule_io.cpp", [])]
 
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
@@ -59,6 +63,11 @@ def build(verbose=False, jobs=8):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # command-line BD driver over the C ABI
+    exe_tmp = EXE + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", os.path.join(CSRC, "bd_main.cpp"), "-o", exe_tmp,
+          "-L" + PKG, "-lstokesdt_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe_tmp, EXE)
     return LIB
 
 

@@ -5,6 +5,7 @@
 #include "../../include/stokesdt_amd.h"
 #include "../../include/stokesdt_amd_debug.h"
 #include "mob_gpu.hpp"
+#include "simbox.hpp"
 
 namespace sdt {
 long long rsa_positions(int n, double box, double a, uint64_t seed, long long max_attempts,
@@ -27,6 +28,7 @@ struct sdt_pairlist_s {
 };
 struct sdt_force_s { StericGpu* f; hipStream_t s; };
 struct sdt_bd_s { BDGpu* bd; };
+struct sdt_simbox_s { SimBoxGpu* sb; };
 
 template <typename F>
 static int guard(F&& fn) {
@@ -399,6 +401,71 @@ int sdt_gaussian(uint64_t seed, uint32_t stream, int64_t n, double* out) {
 }
 int sdt_device_synchronize(void) {
     return guard([&] { SDT_HIP_CHECK(hipDeviceSynchronize()); });
+}
+
+// ---------------------------------------------------------------- SimBox --
+int sdt_simbox_create(const char* config_file, const char* model_file, const char* xyz_file,
+                      sdt_simbox_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] { *out = new sdt_simbox_s{new SimBoxGpu(config_file, model_file, xyz_file)}; });
+}
+int sdt_simbox_init(sdt_simbox_t sb) {
+    SDT_CHECK_HANDLE(sb);
+    return guard([&] { sb->sb->init(); });
+}
+int sdt_simbox_advance(sdt_simbox_t sb, int nsteps) {
+    SDT_CHECK_HANDLE(sb);
+    return guard([&] { sb->sb->advance(nsteps); });
+}
+int sdt_simbox_npos(sdt_simbox_t sb, int* npos) {
+    SDT_CHECK_HANDLE(sb);
+    if (!npos) { set_error("null output"); return SDT_EINVAL; }
+    *npos = sb->sb->npos();
+    return SDT_OK;
+}
+int sdt_simbox_state(sdt_simbox_t sb, int64_t* step, double* time) {
+    SDT_CHECK_HANDLE(sb);
+    if (step) *step = sb->sb->step();
+    if (time) *time = sb->sb->time();
+    return SDT_OK;
+}
+int sdt_simbox_params(sdt_simbox_t sb, double* dparams, int64_t* iparams) {
+    SDT_CHECK_HANDLE(sb);
+    const SimBoxParams& p = sb->sb->params();
+    if (dparams) {
+        const double d[7] = {p.spme_xi, p.spme_rmax, p.lanczos_tol, p.steric_r0, p.steric_k0,
+                             p.delta_t, 0.0};
+        memcpy(dparams, d, sizeof(d));
+    }
+    if (iparams) {
+        const int64_t v[9] = {p.spme_dim, p.spme_porder, p.mob_interval, p.lanczos_maxiters,
+                              p.lanczos_maxnrhs, p.traj_interval, p.traj_start, p.precision,
+                              (int64_t)p.seed};
+        memcpy(iparams, v, sizeof(v));
+    }
+    return SDT_OK;
+}
+int sdt_simbox_get_positions(sdt_simbox_t sb, double* pos) {
+    SDT_CHECK_HANDLE(sb);
+    if (!pos) { set_error("null output"); return SDT_EINVAL; }
+    return guard([&] { sb->sb->positions(pos); });
+}
+int sdt_simbox_get_radii(sdt_simbox_t sb, double* rdi) {
+    SDT_CHECK_HANDLE(sb);
+    if (!rdi) { set_error("null output"); return SDT_EINVAL; }
+    return guard([&] { sb->sb->radii(rdi); });
+}
+int sdt_simbox_write_xyz(sdt_simbox_t sb, const char* path) {
+    SDT_CHECK_HANDLE(sb);
+    if (!path) { set_error("null path"); return SDT_EINVAL; }
+    return guard([&] { sb->sb->write_xyz(path); });
+}
+int sdt_simbox_destroy(sdt_simbox_t sb) {
+    if (!sb) return SDT_OK;
+    delete sb->sb;
+    delete sb;
+    return SDT_OK;
 }
 
 // ----------------------------------------------------------- debug hooks ---

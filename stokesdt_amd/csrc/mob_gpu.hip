@@ -667,6 +667,7 @@ BDGpu::BDGpu(int npos, const double* pos, const double* rdi, double box, double 
     SDT_REQUIRE(dt > 0.0, SDT_EINVAL, "dt must be positive");
     h_pos0_.assign(pos, pos + 3 * (size_t)npos);
     rdi_.assign(rdi, rdi + npos);
+    steric_on_ = r0 > 0.0 && k0 != 0.0;
     mob_.reset(new MobSpmeGpu(npos, rdi, box, xi, rmax, dim, porder, precision));
     brwn_.reset(new BrwnLanczosGpu(3 * npos, max_iters, max_nrhs, tol));
     steric_.reset(new StericGpu(npos, rdi, box, r0, k0));
@@ -675,7 +676,7 @@ BDGpu::BDGpu(int npos, const double* pos, const double* rdi, double box, double 
 void BDGpu::init() {
     mob_->init();
     brwn_->init();
-    steric_->init();
+    if (steric_on_) steric_->init();
     mob_->set_timer(&timer_);
     brwn_->timer = &timer_;
     steric_->timer = &timer_;
@@ -692,21 +693,30 @@ void BDGpu::advance(int nsteps) {
     hipStream_t s = mob_->stream();
     for (int k = 0; k < nsteps; k++) {
         const int irhs = (int)(istep_ % mob_interval_);
-        if (irhs == 0) {
-            const long long blk = istep_ / mob_interval_;
+        const long long blk = istep_ / mob_interval_;
+        // a new interval, or the first step after a start inside an interval
+        // (then D of this interval is computed from the current positions)
+        if (irhs == 0 || blk != d_block_) {
             gaussian_fill_device(seed_, (unsigned)blk, (long long)mob_interval_ * ldm_, d_Z_.get(), s);
             mob_->update(d_pos_.get(), nullptr, true);
             brwn_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
+            d_block_ = blk;
         }
         double* disp = d_D_.get() + (size_t)irhs * ldm_;
         fill(ldm_, 0.0, d_f_.get(), s);
-        steric_->accumulate(d_pos_.get(), d_f_.get(), true, s);
+        if (steric_on_) steric_->accumulate(d_pos_.get(), d_f_.get(), true, s);
         mob_->mul_vector(0, 1, dt_, ldm_, d_f_.get(), sqrt(2.0 * dt_), ldm_, disp, true);
         axpby(ldm_, 1.0, disp, 1.0, d_pos_.get(), s);
         istep_++;
     }
     SDT_HIP_CHECK(hipStreamSynchronize(s));
     timer_.collect();
+}
+
+void BDGpu::set_step(long long istep) {
+    SDT_REQUIRE(istep >= 0, SDT_EINVAL, "step must be non-negative");
+    istep_ = istep;
+    d_block_ = -1;
 }
 
 void BDGpu::get_positions(double* pos) {

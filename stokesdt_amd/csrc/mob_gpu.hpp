@@ -126,6 +126,10 @@ class BDGpu {
     const std::vector<int>& last_iters() const { return brwn_->iters(); }
     MobSpmeGpu* mob() { return mob_.get(); }
     long long istep() const { return istep_; }
+    // continue from step istep (a restart); a start inside a mobility
+    // interval computes that interval's D from the positions at the start
+    void set_step(long long istep);
+    int npos() const { return npos_; }
     StageTimer& timer() { return timer_; }
 
   private:
@@ -139,6 +143,8 @@ class BDGpu {
     std::unique_ptr<StericGpu> steric_;
     DevBuf<double> d_pos_, d_Z_, d_D_, d_f_;
     long long istep_ = 0;
+    long long d_block_ = -1;   // interval whose displacements d_D_ holds
+    bool steric_on_ = true;    // r0 > 0 and k0 != 0
     int ldm_ = 0;
     StageTimer timer_;
 };

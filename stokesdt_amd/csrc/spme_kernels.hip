@@ -311,6 +311,189 @@ spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ 
     }
 }
 
+// Point-owner form (K % 4 == 0): a workgroup of 8 waves owns a 32 x 4 x 4
+// block of mesh points (wave w: x in [4w, 4w+4)), lane l one point
+// (x = l % 4, y = (l / 4) % 4, z = l / 16), and NQ meshes (NQ / NC
+// right-hand sides) accumulated in registers. Candidates (base points whose
+// stencil reaches the block) are staged in LDS in candidate order; every wave
+// lists the candidates that reach its 4 x 4 x 4 sub-block, and each lane
+// adds w = wx[x - bx] wy[y - by] wz[z - bz] (0 outside the stencil) times the
+// NQ forces. Deterministic, no atomics, small per-candidate overhead.
+constexpr int kPtThreads = 512;
+constexpr int kPtMaxTiles = 200;   // contributor bucket tiles of one block (host-checked)
+
+template <typename T, int P, int NQ>
+__global__ void __launch_bounds__(kPtThreads)
+spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile_start,
+                 const T* __restrict__ frac, const int* __restrict__ base,
+                 const double* __restrict__ a2, const double* __restrict__ X, int b, int RB,
+                 int NC, int cap, T* __restrict__ mesh) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    // record: wx[P] 0, wy[P] 0, wz[P] 0, bx by bz, pad, f[NQ] (16-byte aligned)
+    constexpr int OWY = P + 1, OWZ = 2 * (P + 1), OB = 3 * (P + 1);
+    constexpr int OF = (OB + 3 + 3) & ~3;
+    constexpr int REC = OF + NQ;
+    static_assert(NQ % 4 == 0, "NQ must be a multiple of 4");
+    T* stage = reinterpret_cast<T*>(smem_raw);
+    int* s_list = reinterpret_cast<int*>(stage + (size_t)cap * REC);   // [8][cap]
+    int* s_pid = s_list + 8 * cap;                                       // [cap]
+    __shared__ int s_wave[kPtThreads / 64];
+    __shared__ int s_off[kPtMaxTiles + 1];
+    __shared__ int s_beg[kPtMaxTiles];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned lid = xcd_logical_id(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int bxl = (int)(lid % gridDim.x), byl = (int)(lid / gridDim.x);
+    const int K4 = K >> 2;
+    const int sxi = bxl % nsx;
+    const int yz = bxl / nsx;
+    const int x0 = sxi * 32, y0 = (yz % K4) * 4, z0 = (yz / K4) * 4;
+    const int xlen = min(32, K - x0);
+    const int r0 = byl * (NQ / NC);
+    const int rb = min(NQ / NC, b - r0);
+    const int nq = rb * NC;
+    const int sx = x0 + 4 * wave;                  // this wave's sub-block
+    const bool wave_on = 4 * wave < xlen;
+    const int px = sx + (lane & 3), py = y0 + ((lane >> 2) & 3), pz = z0 + (lane >> 4);
+    T acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) acc[q] = T(0);
+
+    // contributor base tiles (distinct, with wrap): bases in [lo, lo + len)
+    if (threadIdx.x == 0) {
+        int first[3], cnt[3];
+        const int lo[3] = {x0 - (P - 1), y0 - (P - 1), z0 - (P - 1)};
+        const int len[3] = {xlen + P - 1, 4 + P - 1, 4 + P - 1};
+        for (int d = 0; d < 3; d++) {
+            if (len[d] >= K) { first[d] = 0; cnt[d] = ntile; continue; }
+            const int a = (lo[d] >= 0) ? lo[d] / ts : -((-lo[d] + ts - 1) / ts);
+            const int e = lo[d] + len[d] - 1;
+            const int bt = (e >= 0) ? e / ts : -((-e + ts - 1) / ts);
+            cnt[d] = min(bt - a + 1, ntile);
+            first[d] = ((a % ntile) + ntile) % ntile;
+        }
+        int off = 0, k = 0;
+        for (int jz = 0; jz < cnt[2]; jz++)
+            for (int jy = 0; jy < cnt[1]; jy++)
+                for (int jx = 0; jx < cnt[0]; jx++) {
+                    const int tt = (((first[2] + jz) % ntile) * ntile + (first[1] + jy) % ntile) *
+                                       ntile + (first[0] + jx) % ntile;
+                    const int pb = tile_start[tt], pe = tile_start[tt + 1];
+                    s_beg[k] = pb;
+                    s_off[k] = off;
+                    off += pe - pb;
+                    k++;
+                }
+        s_off[k] = off;
+        s_wave[0] = k;     // tile count, read below before s_wave is reused
+    }
+    __syncthreads();
+    const int nbt = s_wave[0];
+    const int ncand = s_off[nbt];
+    __syncthreads();
+
+    // accumulate the first n staged candidates
+    auto accumulate = [&](int n) __attribute__((always_inline)) {
+        int* list = s_list + wave * cap;
+        int cnt = 0;
+        if (wave_on) {
+            for (int c0 = 0; c0 < n; c0 += 64) {
+                const int c = c0 + lane;
+                bool hit = false;
+                if (c < n) {
+                    const T* rec = stage + (size_t)c * REC;
+                    const int dx = wrapk((int)rec[OB] - sx, K);
+                    const int dy = wrapk((int)rec[OB + 1] - y0, K);
+                    const int dz = wrapk((int)rec[OB + 2] - z0, K);
+                    hit = (dx < 4 || K - dx < P) && (dy < 4 || K - dy < P) && (dz < 4 || K - dz < P);
+                }
+                const unsigned long long m = __ballot(hit);
+                if (hit) list[cnt + __popcll(m & ((1ull << lane) - 1ull))] = c;
+                cnt += __popcll(m);
+            }
+        }
+        // the list is written and read by this wave only (LDS accesses of a
+        // wave complete in order)
+        __builtin_amdgcn_wave_barrier();
+        for (int k = 0; k < cnt; k++) {
+            const T* rec = stage + (size_t)list[k] * REC;
+            const int dx = min(wrapk(px - (int)rec[OB], K), P);
+            const int dy = min(wrapk(py - (int)rec[OB + 1], K), P);
+            const int dz = min(wrapk(pz - (int)rec[OB + 2], K), P);
+            const T w = rec[dx] * rec[OWY + dy] * rec[OWZ + dz];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) acc[q] += w * rec[OF + q];
+        }
+    };
+
+    int nst = 0;
+    for (int c0 = 0; c0 < ncand; c0 += kPtThreads) {
+        const int ci = c0 + (int)threadIdx.x;
+        int is = -1, b3[3] = {0, 0, 0};
+        bool take = false;
+        if (ci < ncand) {
+            int k = 0;
+            while (k + 1 < nbt && s_off[k + 1] <= ci) k++;
+            is = s_beg[k] + (ci - s_off[k]);
+            b3[0] = base[3 * is]; b3[1] = base[3 * is + 1]; b3[2] = base[3 * is + 2];
+            const int dx = wrapk(b3[0] - x0, K);
+            const int dy = wrapk(b3[1] - y0, K);
+            const int dz = wrapk(b3[2] - z0, K);
+            take = (dx < xlen || K - dx < P) && (dy < 4 || K - dy < P) && (dz < 4 || K - dz < P);
+        }
+        int tot;
+        const int slot = wg_compact(take, s_wave, &tot);
+        for (int done = 0; done < tot;) {
+            if (nst == cap) {
+                __syncthreads();
+                accumulate(nst);
+                __syncthreads();
+                nst = 0;
+            }
+            const int taken = min(cap - nst, tot - done);
+            const int first = nst;
+            if (take && slot >= done && slot < done + taken) {
+                const int sl = first + slot - done;
+                T* rec = stage + (size_t)sl * REC;
+                T w[P];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    bspline_weights<T, P>(frac[3 * is + d], w);
+#pragma unroll
+                    for (int k = 0; k < P; k++) rec[d * (P + 1) + k] = w[k];
+                    rec[d * (P + 1) + P] = T(0);
+                    rec[OB + d] = (T)b3[d];
+                }
+                s_pid[sl] = is;
+            }
+            __syncthreads();
+            // forces of the new slots: slot q holds rhs r, component c (mono:
+            // q = 3 r + c; poly: q = 6 r + c for f, 6 r + 3 + c for a^2 f)
+            for (int idx = threadIdx.x; idx < taken * NQ; idx += kPtThreads) {
+                const int sl = first + idx / NQ, q = idx % NQ;
+                const int p = s_pid[sl];
+                const int r = q / NC, cc = q - r * NC;
+                T fv = T(0);
+                if (r < rb) {
+                    const double x = X[(size_t)(3 * p + cc % 3) * b + r0 + r];
+                    fv = (T)((cc >= 3) ? x * a2[p] : x);
+                }
+                stage[(size_t)sl * REC + OF + q] = fv;
+            }
+            nst += taken;
+            done += taken;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    accumulate(nst);
+    if (!wave_on) return;
+    const size_t K3 = (size_t)K * K * K;
+    const size_t pt = ((size_t)pz * K + py) * K + px;
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+        if (q < nq) mesh[(size_t)(r0 * NC + q) * K3 + pt] = acc[q];
+}
+
 // ------------------------------------------------------ x transforms -------
 // Real rows (2 per complex FFT: z = a + i b). Each workgroup takes NR rows.
 // LDS: K twiddles, then two buffers of NR / 2 pencils of stride PS.
@@ -663,7 +846,7 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
 // particle adds its P slices in fixed order (src/mob/spme.h:219-265 without
 // the stored P/ind tables). Y = beta Y + alpha (sum_U + a^2 sum_W).
 
-template <typename T, int P>
+template <typename T, int P, int RREG>
 __global__ void __launch_bounds__(256)
 interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
                    const int* __restrict__ tile_start, const T* __restrict__ frac,
@@ -683,13 +866,6 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
     const int sx0 = stx * ts, sy0 = ty * ts, sz0 = tz * ts;
     const int RX = nxt * ts + P - 1, RY = ts + P - 1, RZ = ts + P - 1;
     const int nreg = RX * RY * RZ;
-    // staging: wave-uniform row loop, lane = x offset within the row
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int nwave = blockDim.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int nrow = RY * RZ;
-    const bool lane_x = lane < RX;
-    const int gx = wrapk(sx0 + (lane_x ? lane : 0), K);
     // LDS: region [RZ][RY][RX] | weights [CH][3][P] | bases [CH][3] |
     //      partial sums [CH][P] | results [CH][RB][3] (double)
     T* region = reinterpret_cast<T*>(smem_raw);
@@ -699,11 +875,39 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
     double* s_res = reinterpret_cast<double*>(
         (reinterpret_cast<uintptr_t>(s_part + ch * P) + 7) & ~(uintptr_t)7);
     const size_t K3 = (size_t)K * K * K;
+    // offsets within one mesh of this thread's region elements (the same for
+    // every mesh): element u is region index threadIdx.x + 256 u
+    int off[RREG];
+    {
+        const unsigned rx_m = 0xffffffffu / (unsigned)RX + 1u;
+        const unsigned ry_m = 0xffffffffu / (unsigned)RY + 1u;
+#pragma unroll
+        for (int u = 0; u < RREG; u++) {
+            const int i = (int)threadIdx.x + 256 * u;
+            off[u] = -1;
+            if (i < nreg) {
+                const int row = (int)fdiv(i, rx_m), lx = i - row * RX;
+                const int lz = (int)fdiv(row, ry_m), ly = row - lz * RY;
+                off[u] = (wrapk(sz0 + lz, K) * K + wrapk(sy0 + ly, K)) * K + wrapk(sx0 + lx, K);
+            }
+        }
+    }
+    // the region of the next mesh is loaded into registers while the current
+    // one is used (software pipelining over the rb * NC meshes)
+    T v[RREG];
+    auto prefetch = [&](int pi) __attribute__((always_inline)) {
+        const int r = pi / NC, q = pi - r * NC;
+        const T* m = mesh + (size_t)((r0 + r) * NC + q) * K3;
+#pragma unroll
+        for (int u = 0; u < RREG; u++) v[u] = (off[u] >= 0) ? m[off[u]] : T(0);
+    };
     // own particles: base tiles stx .. stx+nxt-1 of row (ty, tz) are contiguous
     const int t_first = (tz * ntile + ty) * ntile + stx;
     const int pb = tile_start[t_first], pe = tile_start[t_first + nxt];
+    const int npass = rb * NC;
     for (int c0 = pb; c0 < pe; c0 += ch) {
         const int cnt = min(ch, pe - c0);
+        prefetch(0);
         for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
             const int is = c0 + t;
             T w[P];
@@ -719,58 +923,40 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
             s_b[t * 3 + 2] = wrapk(base[3 * is + 2] - sz0, K);
             for (int q = 0; q < rb * 3; q++) s_res[t * RB * 3 + q] = 0.0;
         }
-        for (int r = 0; r < rb; r++) {
-            for (int q = 0; q < NC; q++) {
-                __syncthreads();
-                const T* m = mesh + (size_t)((r0 + r) * NC + q) * K3;
-                // one region row (RX <= 64 x values) per wave and step, lanes
-                // over x; the row address is wave-uniform (scalar) and 8 rows
-                // are loaded before their LDS stores
-                constexpr int UNR = 8;
-                for (int row0 = wave * UNR; row0 < nrow; row0 += nwave * UNR) {
-                    T v[UNR];
+        for (int pi = 0; pi < npass; pi++) {
+            const int r = pi / NC, q = pi - r * NC;
+            __syncthreads();          // the previous pass no longer reads the region
 #pragma unroll
-                    for (int u = 0; u < UNR; u++) {
-                        const int row = row0 + u;
-                        v[u] = T(0);
-                        if (row < nrow && lane_x) {
-                            const int lz = row / RY, ly = row - lz * RY;
-                            const T* src = m + ((size_t)wrapk(sz0 + lz, K) * K + wrapk(sy0 + ly, K)) * K;
-                            v[u] = src[gx];
-                        }
-                    }
+            for (int u = 0; u < RREG; u++) {
+                const int i = (int)threadIdx.x + 256 * u;
+                if (i < nreg) region[i] = v[u];
+            }
+            __syncthreads();
+            if (pi + 1 < npass) prefetch(pi + 1);
+            for (int item = threadIdx.x; item < cnt * P; item += blockDim.x) {
+                const int t = item / P, iz = item - t * P;
+                const T* wx = s_w + (t * 3 + 0) * P;
+                const T* wy = s_w + (t * 3 + 1) * P;
+                const T* rp = region + ((size_t)(s_b[t * 3 + 2] + iz) * RY + s_b[t * 3 + 1]) * RX +
+                              s_b[t * 3 + 0];
+                T acc = T(0);
 #pragma unroll
-                    for (int u = 0; u < UNR; u++) {
-                        const int row = row0 + u;
-                        if (row < nrow && lane_x) region[row * RX + lane] = v[u];
-                    }
+                for (int iy = 0; iy < P; iy++) {
+                    T row = T(0);
+#pragma unroll
+                    for (int ix = 0; ix < P; ix++) row += wx[ix] * rp[iy * RX + ix];
+                    acc += wy[iy] * row;
                 }
-                __syncthreads();
-                for (int item = threadIdx.x; item < cnt * P; item += blockDim.x) {
-                    const int t = item / P, iz = item - t * P;
-                    const T* wx = s_w + (t * 3 + 0) * P;
-                    const T* wy = s_w + (t * 3 + 1) * P;
-                    const T* rp = region + ((size_t)(s_b[t * 3 + 2] + iz) * RY + s_b[t * 3 + 1]) * RX +
-                                  s_b[t * 3 + 0];
-                    T acc = T(0);
+                s_part[t * P + iz] = acc * s_w[(t * 3 + 2) * P + iz];
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+                T sum = T(0);
 #pragma unroll
-                    for (int iy = 0; iy < P; iy++) {
-                        T row = T(0);
-#pragma unroll
-                        for (int ix = 0; ix < P; ix++) row += wx[ix] * rp[iy * RX + ix];
-                        acc += wy[iy] * row;
-                    }
-                    s_part[t * P + iz] = acc * s_w[(t * 3 + 2) * P + iz];
-                }
-                __syncthreads();
-                for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-                    T sum = T(0);
-#pragma unroll
-                    for (int iz = 0; iz < P; iz++) sum += s_part[t * P + iz];
-                    double v = (double)sum;
-                    if (q >= 3) v *= a2[c0 + t];
-                    s_res[t * RB * 3 + r * 3 + (q % 3)] += v;
-                }
+                for (int iz = 0; iz < P; iz++) sum += s_part[t * P + iz];
+                double vv = (double)sum;
+                if (q >= 3) vv *= a2[c0 + t];
+                s_res[t * RB * 3 + r * 3 + (q % 3)] += vv;
             }
         }
         __syncthreads();
@@ -921,6 +1107,36 @@ template <typename T>
 void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s) {
     SDT_REQUIRE(b <= bcap, SDT_ECAP, "spread: rhs count exceeds mesh capacity");
     const int NC = poly ? 6 : 3;
+    // contributor bucket tiles of a 32 x 4 x 4 block: bases in intervals of
+    // length min(32, K) + P - 1 (x) and P + 3 (y, z)
+    const int pt_tiles = std::min(ntile, (std::min(32, K) + porder - 1 + ts - 1) / ts + 1) *
+                         std::min(ntile, (porder + 3 + ts - 1) / ts + 1) *
+                         std::min(ntile, (porder + 3 + ts - 1) / ts + 1);
+    if (K % 4 == 0 && K >= 16 && pt_tiles <= kPtMaxTiles) {
+        // point-owner kernel: NQ meshes per workgroup, 12 when few are needed
+        constexpr int NQL = sizeof(T) == 4 ? 48 : 24;
+        const int NQ = (b * NC <= 12) ? 12 : NQL;
+        const int rec = ((3 * (porder + 1) + 3 + 3) & ~3) + NQ;
+        const int cap = (int)(65536 / (rec * sizeof(T) + 9 * sizeof(int)));
+        const size_t lds = (size_t)cap * (rec * sizeof(T) + 9 * sizeof(int));
+        const int nsx = (K + 31) / 32;
+        dim3 grid(nsx * (K / 4) * (K / 4), ceil_div(b, NQ / NC));
+#define SDT_SPREAD_PT(PP, QQ)                                                                  \
+    hipLaunchKernelGGL((spread_pt_kernel<T, PP, QQ>), grid, dim3(kPtThreads), lds, s, K, ts,     \
+                       ntile, nsx, d_tile_start.get(), d_frac.get(), d_base.get(),             \
+                       poly ? d_a2.get() : (double*)nullptr, X, b, NQ / NC, NC, cap,          \
+                       d_real.get())
+#define SDT_SPREAD_P(PP)                                                                       \
+    case PP:                                                                                   \
+        if (NQ == 12) SDT_SPREAD_PT(PP, 12);                                                   \
+        else SDT_SPREAD_PT(PP, NQL);                                                           \
+        break;
+        switch (porder) { SDT_SPREAD_P(4) SDT_SPREAD_P(5) SDT_SPREAD_P(6) SDT_SPREAD_P(7) SDT_SPREAD_P(8) }
+#undef SDT_SPREAD_P
+#undef SDT_SPREAD_PT
+        SDT_HIP_CHECK(hipGetLastError());
+        return;
+    }
     // right-hand sides per workgroup: RB * NC register accumulators per point
     const int RBNC = poly ? 12 : 12;
     const int RB = RBNC / NC;
@@ -1105,17 +1321,24 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
     }
     const size_t lds = lds_of(sxt, ch);
     SDT_REQUIRE(lds <= 64 * 1024, SDT_EINVAL, "interp: LDS region too large");
-    SDT_REQUIRE(sxt * ts + porder - 1 <= 64, SDT_EINVAL, "interp: region row exceeds a wave");
     const int nsx = (ntile + sxt - 1) / sxt;
+    // region elements per thread (registers of the prefetch)
+    const int rreg = (int)ceil_div((size_t)(sxt * ts + R) * (ts + R) * (ts + R), 256);
+    SDT_REQUIRE(rreg <= 36, SDT_EINVAL, "interp: region too large for the prefetch registers");
     dim3 grid(nsx * ntile * ntile, ceil_div(b, RB));
+#define SDT_INTERP_R(PP, RR)                                                                  \
+    hipLaunchKernelGGL((interp_tile_kernel<T, PP, RR>), grid, dim3(256), lds, s, K, ts, ntile,    \
+                       nsx, sxt, ch, d_tile_start.get(), d_frac.get(), d_base.get(),              \
+                       poly ? d_a2.get() : (double*)nullptr, d_real.get(), b, RB, NC, a_eff, beta, \
+                       Y)
 #define SDT_INTERP(PP)                                                                        \
     case PP:                                                                                  \
-        hipLaunchKernelGGL((interp_tile_kernel<T, PP>), grid, dim3(256), lds, s, K, ts, ntile, \
-                           nsx, sxt, ch, d_tile_start.get(), d_frac.get(), d_base.get(),      \
-                           poly ? d_a2.get() : (double*)nullptr, d_real.get(), b, RB, NC,    \
-                           a_eff, beta, Y);                                                   \
+        if (rreg <= 16) SDT_INTERP_R(PP, 16);                                                 \
+        else if (rreg <= 26) SDT_INTERP_R(PP, 26);                                            \
+        else SDT_INTERP_R(PP, 36);                                                            \
         break;
     switch (porder) { SDT_INTERP(4) SDT_INTERP(5) SDT_INTERP(6) SDT_INTERP(7) SDT_INTERP(8) }
+#undef SDT_INTERP_R
 #undef SDT_INTERP
     SDT_HIP_CHECK(hipGetLastError());
 }
