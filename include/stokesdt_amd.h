@@ -135,6 +135,11 @@ int sdt_steric_init(sdt_force_t fo);
 int sdt_steric_compute(sdt_force_t fo, const double* pos, const double* rdi, double* f);
 int sdt_steric_accumulate(sdt_force_t fo, const double* pos, const double* rdi, double* f);
 int sdt_steric_destroy(sdt_force_t fo);
+/* One-shot f = force_steric_mex(pos, rdi, L, r0, k0)
+ * (mex/force_steric_mex.cc:8-50): pos AoS 3 x npos, f 3 x npos (zeroed,
+ * then the steric forces); L, r0 and k0 must be positive. */
+int sdt_force_steric(const double* pos, const double* rdi, int npos, double L, double r0,
+                     double k0, double* f);
 
 /* --------------------------------------------------------------- BD step
  * BDSimBox::Advance (src/simbox/simbox_bd.cc:296-366) with SPME mobility,

@@ -77,6 +77,7 @@ SIGNATURES = {
     "sdt_bd_get_positions": (_I, [_P, _P]),
     "sdt_bd_last_iters": (_I, [_P, _P, _I, _PI]),
     "sdt_bd_destroy": (_I, [_P]),
+    "sdt_force_steric": (_I, [_P, _P, _I, _D, _D, _D, _P]),
     "sdt_simbox_create": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                ctypes.POINTER(_P)]),
     "sdt_simbox_init": (_I, [_P]),
@@ -370,6 +371,17 @@ class StericForce:
         if getattr(self, "h", None) and _lib is not None:
             lib().sdt_steric_destroy(self.h)
             self.h = None
+
+
+def force_steric(pos, rdi, L, r0, k0):
+    """f = force_steric_mex(pos, rdi, L, r0, k0) (mex/force_steric_mex.cc:8-50).
+    pos: (3, npos) or flat AoS; returns the flat AoS force vector."""
+    pos = _f64(np.asarray(pos).T.reshape(-1) if np.ndim(pos) == 2 and np.shape(pos)[0] == 3
+               else np.asarray(pos).reshape(-1))
+    rdi = _f64(rdi).reshape(-1)
+    f = np.zeros(pos.size)
+    _check(lib().sdt_force_steric(_ptr(pos), _ptr(rdi), rdi.size, L, r0, k0, _ptr(f)))
+    return f
 
 
 class SimBox:

@@ -20,6 +20,9 @@ void tsmm(long long n, int b, int nblk, const double* A, long long blk_stride, c
 // [ngroup][nblk][bs][bs] (rows/columns beyond a narrower last group unused)
 void tsmm_groups(long long n, int btot, int bs, int nblk, const double* A, long long blk_stride,
                  const double* d_S, double alpha, double beta, double* C, hipStream_t s);
+// the same with one coefficient block per group (nblk = 1); C may alias A
+void tsmm_groups1(long long n, int btot, int bs, const double* A, const double* d_S, double alpha,
+                  double beta, double* C, hipStream_t s);
 // sum of squares (synchronises)
 double sumsq(long long n, const double* x, DevBuf<double>& part, hipStream_t s);
 // reference layout (column-major 3N x b, leading dimension ld, original

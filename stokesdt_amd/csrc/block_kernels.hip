@@ -127,6 +127,53 @@ void tsmm_groups(long long n, int btot, int bs, int nblk, const double* A, long 
     SDT_HIP_CHECK(hipGetLastError());
 }
 
+// One coefficient block per group, one thread per (row, group): the thread
+// reads the row's b_g values of A, then writes the b_g results, so C may
+// alias A (in-place C = C S).
+__global__ void __launch_bounds__(256)
+tsmm_rowgroup_kernel(long long n, int btot, int bs, const double* A, const double* __restrict__ S,
+                     double alpha, double beta, double* C) {
+    extern __shared__ double s_S[];
+    const int ngroup = (btot + bs - 1) / bs;
+    for (int t = threadIdx.x; t < ngroup * bs * bs; t += blockDim.x) s_S[t] = S[t];
+    __syncthreads();
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * ngroup) return;
+    const long long i = t / ngroup;
+    const int g = (int)(t - i * ngroup);
+    const int c0 = g * bs, bg = min(bs, btot - c0);
+    const double* sg = s_S + (size_t)g * bs * bs;
+    double a[16], out[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) a[j] = (j < bg) ? A[i * btot + c0 + j] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (j < bg && c < bg) acc += a[j] * sg[j * bs + c];
+        out[c] = acc;
+    }
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        if (c < bg) {
+            double* cp = C + i * btot + c0 + c;
+            *cp = (beta == 0.0 ? 0.0 : beta * *cp) + alpha * out[c];
+        }
+    }
+}
+
+void tsmm_groups1(long long n, int btot, int bs, const double* A, const double* d_S, double alpha,
+                  double beta, double* C, hipStream_t s) {
+    SDT_REQUIRE(bs >= 1 && bs <= 16, SDT_EINVAL, "group width must be in [1, 16]");
+    const int ngroup = (btot + bs - 1) / bs;
+    const size_t lds = sizeof(double) * (size_t)ngroup * bs * bs;
+    SDT_REQUIRE(lds <= 64 * 1024, SDT_ECAP, "tsmm: coefficient block too large for LDS");
+    hipLaunchKernelGGL(tsmm_rowgroup_kernel, dim3(ceil_div(n * ngroup, 256)), dim3(256), lds, s, n,
+                       btot, bs, A, d_S, alpha, beta, C);
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
 void tsmm(long long n, int b, int nblk, const double* A, long long blk_stride, const double* d_S,
           double alpha, double beta, double* C, hipStream_t s) {
     tsmm_groups(n, b, b, nblk, A, blk_stride, d_S, alpha, beta, C, s);

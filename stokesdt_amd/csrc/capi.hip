@@ -340,6 +340,21 @@ int sdt_steric_compute(sdt_force_t fo, const double* pos, const double* rdi, dou
         fo->f->accumulate(pos, f, false, fo->s);
     });
 }
+int sdt_force_steric(const double* pos, const double* rdi, int npos, double L, double r0,
+                     double k0, double* f) {
+    // argument checks of mex/force_steric_mex.cc:13-37
+    if (!pos || !rdi || !f || npos <= 0) { set_error("invalid array arguments"); return SDT_EINVAL; }
+    if (L <= 0. || r0 <= 0. || k0 <= 0.) {
+        set_error("Scalar arguments must be positive.");
+        return SDT_EINVAL;
+    }
+    sdt_force_t h = nullptr;
+    int rc = sdt_steric_create(npos, rdi, L, r0, k0, &h);
+    if (!rc) rc = sdt_steric_init(h);
+    if (!rc) rc = sdt_steric_compute(h, pos, rdi, f);
+    sdt_steric_destroy(h);
+    return rc;
+}
 int sdt_steric_destroy(sdt_force_t fo) {
     if (!fo) return SDT_OK;
     return guard([&] {

@@ -129,12 +129,14 @@ void MobSpmeGpu::apply_packed(int mode, int b, double alpha, const double* X, do
     if (mode == 0 || mode == 1) {
         StageScope scope(timer_, 5, s);
         const double rb = (mode == 0) ? 1.0 : beta;
-        if (prec_ == kFP32)
+        if (prec_ == kFP32) {
+            d_x32_.reserve((size_t)3 * npos_ * b);
             real_matvec<float>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha,
-                               X, rb, Y, s);
-        else
+                               X, rb, Y, d_x32_.get(), s);
+        } else {
             real_matvec<double>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha,
-                                X, rb, Y, s);
+                                X, rb, Y, nullptr, s);
+        }
         real_overlap(ovl_, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha, X, Y, s);
     }
 }
@@ -273,7 +275,7 @@ void parallel_for(int n, F f) {
 // W_g = Q_g R_g, R_g = R2 R1 stored into (g.*dst). Done groups are zeroed.
 // Returns false in ok[g] when W_g is numerically rank deficient (the group's
 // columns are then zeroed). trace (optional) receives tr(W_g^T W_g).
-static void cholqr2_groups(long long n, int btot, int bs, double* W, double* tmp,
+static void cholqr2_groups(long long n, int btot, int bs, double* W,
                            std::vector<LzGroup>& grp, std::vector<double> LzGroup::*dst,
                            std::vector<char>& ok, std::vector<double>* trace, DevBuf<double>& dG,
                            DevBuf<double>& dS, DevBuf<double>& part, hipStream_t s,
@@ -317,8 +319,7 @@ static void cholqr2_groups(long long n, int btot, int bs, double* W, double* tmp
         SDT_HIP_CHECK(hipMemcpyAsync(dS.get(), hS.data(), sizeof(double) * hS.size(),
                                      hipMemcpyHostToDevice, s));
         StageScope sc(timer, 6, s);
-        tsmm_groups(n, btot, bs, 1, W, 0, dS.get(), 1.0, 0.0, tmp, s);
-        SDT_HIP_CHECK(hipMemcpyAsync(W, tmp, sizeof(double) * n * btot, hipMemcpyDeviceToDevice, s));
+        tsmm_groups1(n, btot, bs, W, dS.get(), 1.0, 0.0, W, s);   // in place
     }
 }
 
@@ -330,7 +331,6 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
     const int ng = (btot + bs - 1) / bs;
     const size_t blk = (size_t)n * btot;
     d_V_.reserve((size_t)(max_iters_ + 1) * blk);
-    d_C_.reserve(blk);
     d_G_.reserve((size_t)ng * 256);
     double* V = d_V_.get();
     const size_t sset = (size_t)ng * bs * bs;   // one coefficient block per group
@@ -350,7 +350,7 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
     SDT_HIP_CHECK(hipMemcpyAsync(V, Z, sizeof(double) * blk, hipMemcpyDeviceToDevice, s));
     std::vector<char> ok;
     std::vector<double> tr(ng, 0.0);
-    cholqr2_groups(n, btot, bs, V, d_C_.get(), grp, &LzGroup::R, ok, &tr, d_G_, d_S_, d_part_, s,
+    cholqr2_groups(n, btot, bs, V, grp, &LzGroup::R, ok, &tr, d_G_, d_S_, d_part_, s,
                    timer);
     for (int gi = 0; gi < ng; gi++) {
         if (!ok[gi])
@@ -391,8 +391,7 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
                 }
                 upload_S(hS);
                 StageScope sc(timer, 6, s);
-                tsmm_groups(n, btot, bs, 1, V + (size_t)(iter - 1) * blk, 0, d_S_.get(), 1.0, 0.0,
-                            W, s);
+                tsmm_groups1(n, btot, bs, V + (size_t)(iter - 1) * blk, d_S_.get(), 1.0, 0.0, W, s);
                 beta = -1.0;
             }
             // W = M V_i - V_{i-1} B^T, one application for every column
@@ -413,7 +412,7 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
                     }
             }
             upload_S(hS);
-            { StageScope sc(timer, 6, s); tsmm_groups(n, btot, bs, 1, Vi, 0, d_S_.get(), -1.0, 1.0, W, s); }
+            { StageScope sc(timer, 6, s); tsmm_groups1(n, btot, bs, Vi, d_S_.get(), -1.0, 1.0, W, s); }
             if (reorth) {
                 // full reorthogonalisation against V_0 .. V_i (classical
                 // Gram-Schmidt, one pass)
@@ -443,7 +442,7 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
                 }
             }
             // [V_{i+1}, B_i] = QR(W)
-            cholqr2_groups(n, btot, bs, W, d_C_.get(), grp, &LzGroup::B, ok, nullptr, d_G_, d_S_,
+            cholqr2_groups(n, btot, bs, W, grp, &LzGroup::B, ok, nullptr, d_G_, d_S_,
                            d_part_, s, timer);
             // T: H_i on the diagonal, B_i below it (and B_i^T above)
             for (int gi = 0; gi < ng; gi++) {

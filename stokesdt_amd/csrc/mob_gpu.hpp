@@ -54,6 +54,7 @@ class MobSpmeGpu {
     std::unique_ptr<SpmeGpu<double>> spme64_;
     PairListGpu pairs_;
     RealOverlap ovl_;
+    DevBuf<float> d_x32_;     // FP32 copy of the packed input for the real space
     SortScratch sc_;
     DevBuf<double> d_pos_, d_rdi_, d_pos_s_, d_rdi_s_;
     DevBuf<int> d_key_, d_perm_, d_iperm_, d_tstart_;
@@ -90,7 +91,7 @@ class BrwnLanczosGpu {
     double tol_;
     bool inited_ = false;
     std::vector<int> iters_, hint_;
-    DevBuf<double> d_V_, d_G_, d_S_, d_part_, d_io_z_, d_io_y_, d_Zp_, d_Yp_, d_C_;
+    DevBuf<double> d_V_, d_G_, d_S_, d_part_, d_io_z_, d_io_y_, d_Zp_, d_Yp_;
 };
 
 // Steric force (src/force/force_steric.cc) on the S pair list.

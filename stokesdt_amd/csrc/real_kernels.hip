@@ -6,6 +6,7 @@
 // (src/util/sparse.cc:79-153). Here the blocks are recomputed inside the
 // matvec from FP64 position differences, for a chunk of right-hand sides per
 // thread, so a matvec streams only the column indices and the input vector.
+#include <algorithm>
 #include <cmath>
 #include "real_gpu.hpp"
 #include "sort_gpu.hpp"
@@ -43,11 +44,11 @@ __device__ __forceinline__ void ewald_scalars(T xi, T r, T aa, T ab, T* xa, T* y
 // overlapping pairs are added by real_overlap_kernel. LPR consecutive lanes
 // share a row, each taking every LPR-th pair, so a row's gathers are in
 // flight together; partial sums are combined with cross-lane shuffles.
-template <typename T, int RC, int LPR>
+template <typename T, int RC, int LPR, typename XT>
 __global__ void __launch_bounds__(256)
 real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restrict__ colidx,
                    const double* __restrict__ pos, const double* __restrict__ rdi, double box,
-                   double xi, int b, double alpha, const double* __restrict__ X, double beta,
+                   double xi, int b, double alpha, const XT* __restrict__ X, double beta,
                    double* __restrict__ Y) {
     // rows are in spatially sorted order: XCD-contiguous block ranges keep
     // the gathered neighbour rows of X within one L2
@@ -74,11 +75,11 @@ real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restri
         const int kend = rowptr[i + 1];
         for (int k = rowptr[i] + sub; k < kend; k += LPR) {
             const int j = colidx[k];
-            const double* xj = X + (size_t)(3 * j) * b + r0;
-            double xv[3][RC];
+            const XT* xj = X + (size_t)(3 * j) * b + r0;
+            XT xv[3][RC];
 #pragma unroll
             for (int r = 0; r < RC; r++) {
-                xv[0][r] = xv[1][r] = xv[2][r] = 0.0;
+                xv[0][r] = xv[1][r] = xv[2][r] = XT(0);
                 if (r < rc) {
                     xv[0][r] = xj[r];
                     xv[1][r] = xj[b + r];
@@ -135,27 +136,49 @@ real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restri
     }
 }
 
+__global__ void to_float_kernel(long long n, const double* __restrict__ x, float* __restrict__ y) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        y[i] = (float)x[i];
+}
+
+template <typename T, int RC, typename XT>
+static void launch_real(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
+                        const double* d_rdi, double box, double xi, int b, double alpha,
+                        const XT* X, double beta, double* Y, hipStream_t s) {
+    constexpr int LPR = 8;
+    dim3 grid(ceil_div((size_t)npos * LPR, 256), ceil_div(b, RC));
+    hipLaunchKernelGGL((real_matvec_kernel<T, RC, LPR, XT>), grid, dim3(256), 0, s, npos,
+                       d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X, beta, Y);
+}
+
 template <typename T>
 void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
                  const double* d_rdi, double box, double xi, int b, double alpha, const double* X,
-                 double beta, double* Y, hipStream_t s) {
-    constexpr int LPR = 8;
-    if (sizeof(T) == 4 && b >= 16) {
-        // 16 columns: one 128-byte line per component row of a neighbour
-        constexpr int RC = sizeof(T) == 4 ? 16 : 8;
-        dim3 grid(ceil_div((size_t)npos * LPR, 256), ceil_div(b, RC));
-        hipLaunchKernelGGL((real_matvec_kernel<T, RC, LPR>), grid, dim3(256), 0, s, npos,
-                           d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X, beta, Y);
-    } else if (b >= 8) {
-        constexpr int RC = 8;
-        dim3 grid(ceil_div((size_t)npos * LPR, 256), ceil_div(b, RC));
-        hipLaunchKernelGGL((real_matvec_kernel<T, RC, LPR>), grid, dim3(256), 0, s, npos,
-                           d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X, beta, Y);
+                 double beta, double* Y, float* x32, hipStream_t s) {
+    if constexpr (sizeof(T) == 4) {
+        if (x32 != nullptr && b >= 8) {
+            // FP32 arithmetic: gather an FP32 copy of X (half the bytes per
+            // neighbour row), 16 columns per thread
+            const long long n = 3LL * npos * b;
+            hipLaunchKernelGGL(to_float_kernel, dim3(std::min<long long>(ceil_div(n, 256), 8192)),
+                               dim3(256), 0, s, n, X, x32);
+            if (b >= 16)
+                launch_real<T, 16, float>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha,
+                                          x32, beta, Y, s);
+            else
+                launch_real<T, 8, float>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha,
+                                         x32, beta, Y, s);
+            SDT_HIP_CHECK(hipGetLastError());
+            return;
+        }
+    }
+    if (b >= 8) {
+        launch_real<T, 8, double>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X,
+                                  beta, Y, s);
     } else {
-        constexpr int RC = 4;
-        dim3 grid(ceil_div((size_t)npos * LPR, 256), ceil_div(b, RC));
-        hipLaunchKernelGGL((real_matvec_kernel<T, RC, LPR>), grid, dim3(256), 0, s, npos,
-                           d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X, beta, Y);
+        launch_real<T, 4, double>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X,
+                                  beta, Y, s);
     }
     SDT_HIP_CHECK(hipGetLastError());
 }
@@ -277,8 +300,10 @@ void real_overlap(const RealOverlap& ov, const double* d_pos, const double* d_rd
 }
 
 template void real_matvec<float>(int, const int*, const int*, const double*, const double*, double,
-                                 double, int, double, const double*, double, double*, hipStream_t);
+                                 double, int, double, const double*, double, double*, float*,
+                                 hipStream_t);
 template void real_matvec<double>(int, const int*, const int*, const double*, const double*, double,
-                                  double, int, double, const double*, double, double*, hipStream_t);
+                                  double, int, double, const double*, double, double*, float*,
+                                  hipStream_t);
 
 }  // namespace sdt

@@ -122,3 +122,14 @@ def test_steric_matches_oracle_and_golden(sdt, orc):
 def test_steric_rejects_nonpositive_k0(sdt):
     fo = sdt.StericForce(10, np.ones(10), 20.0, 2.0, 0.0)
     assert not fo.init()
+
+
+def test_force_steric_one_shot(sdt, orc):
+    # force_steric_mex semantics: (3, npos) positions, same forces as the
+    # class interface
+    pos, rdi = poly_system(412, 1000.0, 23)
+    f = sdt.force_steric(pos.reshape(-1, 3).T, rdi, 1000.0, 2.3, 125.0)
+    ref = orc.steric(pos, rdi, 1000.0, 2.3, 125.0)
+    assert np.linalg.norm(f - ref) / np.linalg.norm(ref) < 1e-12
+    with pytest.raises(sdt.SdtError):
+        sdt.force_steric(pos, rdi, 1000.0, 2.3, 0.0)
