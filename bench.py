@@ -39,10 +39,12 @@ def box_size(n, phi):
     return (4.0 * np.pi * n / (3.0 * phi)) ** (1.0 / 3.0)
 
 
-def alg_bytes(stage, b, n, K, e, poly=False):
+def alg_bytes(stage, b, n, K, e, poly=False, nnz=0):
     """Algorithmic HBM bytes of one launch group processing b right-hand
     sides (compact meshes, weights recomputed, nothing re-read)."""
     nc = 6 if poly else 3
+    if stage == "real_space":  # CSR, positions and radii, FP64 X in, Y read and written
+        return 4 * nnz + 4 * (n + 1) + 4 * n * 8 + 3 * n * b * 8 * 3
     K3 = K ** 3
     spec = K * K * (K // 2 + 1) * 2 * e
     if stage == "spread":      # particle offsets/bases, FP64 forces in, meshes out
@@ -247,14 +249,21 @@ def main():
     # region). Its launches there: one single-column matvec per step and,
     # when the window holds a mobility update, one fused matvec over the
     # whole Lanczos batch per block iteration.
-    cand = ["spread", "fft_xy_fwd", "fft_z_influence", "fft_xy_inv", "interp"]
+    cand = ["spread", "fft_xy_fwd", "fft_z_influence", "fft_xy_inv", "interp", "real_space"]
     dom = max(cand, key=lambda k: r32["prof"][k][0])
     tot_ms, launches = r32["prof"][dom]
     btot = lanczos_batch_cols()
     n_lz = launches - args.steps
+    nnz = 0
+    if dom == "real_space":
+        m = S.MobSpme(n, rdi, L, c["xi"], c["rmax"], c["K"], c["p"], S.FP32)
+        assert m.init()
+        m.update(pos)
+        nnz = m.real_nnz()
+        del m
     if n_lz >= 0 and tot_ms > 0:
-        tot_bytes = (args.steps * alg_bytes(dom, 1, n, c["K"], 4) +
-                     n_lz * alg_bytes(dom, btot, n, c["K"], 4))
+        tot_bytes = (args.steps * alg_bytes(dom, 1, n, c["K"], 4, nnz=nnz) +
+                     n_lz * alg_bytes(dom, btot, n, c["K"], 4, nnz=nnz))
         achieved = tot_bytes / (tot_ms * 1e-3) / 1e9
         traffic = None
         tr = read_traffic(dom)

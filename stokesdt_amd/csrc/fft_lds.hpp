@@ -102,6 +102,45 @@ __device__ __forceinline__ void small_dft(cplx<T>* a) {
         a[6] = csub(e[2], t2);
         a[3] = cadd(e[3], t3);
         a[7] = csub(e[3], t3);
+    } else if constexpr (R == 16) {
+        // n = 4 n1 + n2: 4-point DFTs over n1, twiddles w_16^(n2 k1), then
+        // 4-point DFTs over n2; X[k1 + 4 k2]
+        cplx<T> b[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; n2++) {
+            b[n2][0] = a[n2]; b[n2][1] = a[4 + n2]; b[n2][2] = a[8 + n2]; b[n2][3] = a[12 + n2];
+            small_dft<T, 4, SIGN>(b[n2]);
+        }
+        const T c1 = T(0.92387953251128675612818318939679);   // cos(pi/8)
+        const T s1 = T(0.38268343236508977172845998403040);   // sin(pi/8)
+        const T h = T(0.70710678118654752440084436210485);
+        const T sg = T(SIGN);
+        // w_16^e = cos(2 pi e / 16) + SIGN i sin(2 pi e / 16), e = n2 k1
+        auto tw16 = [&](cplx<T> v, int e) -> cplx<T> {
+            T cr, ci;
+            switch (e) {
+                case 1: cr = c1; ci = s1; break;
+                case 2: cr = h; ci = h; break;
+                case 3: cr = s1; ci = c1; break;
+                case 4: cr = T(0); ci = T(1); break;
+                case 6: cr = -h; ci = h; break;
+                case 9: cr = -c1; ci = -s1; break;
+                default: return v;
+            }
+            ci *= sg;
+            return cplx<T>{v.x * cr - v.y * ci, v.x * ci + v.y * cr};
+        };
+#pragma unroll
+        for (int n2 = 1; n2 < 4; n2++)
+#pragma unroll
+            for (int k1 = 1; k1 < 4; k1++) b[n2][k1] = tw16(b[n2][k1], n2 * k1);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            cplx<T> c[4] = {b[0][k1], b[1][k1], b[2][k1], b[3][k1]};
+            small_dft<T, 4, SIGN>(c);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++) a[k1 + 4 * k2] = c[k2];
+        }
     } else if constexpr (R == 5) {
         const T c1 = T(0.30901699437494742410229341718282);    // cos(2pi/5)
         const T c2 = T(-0.80901699437494742410229341718282);   // cos(4pi/5)
@@ -205,7 +244,7 @@ __device__ __forceinline__ void fft_stage_inplace(cplx<T>* __restrict__ buf, int
 // In-place transform of npen pencils; MAXI2 bounds the radix-2 butterflies
 // per thread (npen * K <= 2 * MAXI2 * blockDim.x), other radices scale.
 template <typename T, int SIGN, int MAXI2>
-__device__ void fft_pencils_inplace(cplx<T>* buf, int npen, int PS, const FFTArgs& fa,
+__device__ __forceinline__ void fft_pencils_inplace(cplx<T>* buf, int npen, int PS, const FFTArgs& fa,
                                     const cplx<T>* tw) {
     for (int st = 0; st < fa.nst; st++) {
         const int r = fa.radix[st];
@@ -217,9 +256,64 @@ __device__ void fft_pencils_inplace(cplx<T>* buf, int npen, int PS, const FFTArg
     }
 }
 
-// Runs all stages; returns the buffer holding the result (a or b).
+// Two-stage in-place transform for K = R1 R2 (R1, R2 powers of two <= 16):
+// stage 1 (radix R1, stride 1): butterfly j < R2 reads elements j + R2 u,
+// applies w_K^(j u) and writes R1 j + u; stage 2 (radix R2, stride R1, no
+// twiddles): butterfly j < R1 reads and writes elements j + R1 u. Index
+// arithmetic is shifts and masks; stage 2 is in place per thread. Requires
+// npen * R2 <= MAXB * blockDim.x.
+template <typename T, int R1, int R2, int SIGN, int MAXB = 1>
+__device__ __forceinline__ void fft2_pencils(cplx<T>* buf, int npen, int PS,
+                                             const cplx<T>* __restrict__ tw) {
+    static_assert((R1 & (R1 - 1)) == 0 && (R2 & (R2 - 1)) == 0, "powers of two");
+    {
+        cplx<T> a[MAXB][R1];
+#pragma unroll
+        for (int v = 0; v < MAXB; v++) {
+            const int it = threadIdx.x + v * blockDim.x;
+            if (it < npen * R2) {
+                const int pen = it / R2, j = it % R2;
+                const cplx<T>* base = buf + pen * PS;
+#pragma unroll
+                for (int u = 0; u < R1; u++) a[v][u] = base[lpad(j + R2 * u)];
+                small_dft<T, R1, SIGN>(a[v]);
+#pragma unroll
+                for (int u = 1; u < R1; u++) {
+                    cplx<T> w = tw[j * u];
+                    if (SIGN > 0) w.y = -w.y;
+                    a[v][u] = cmul(a[v][u], w);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < MAXB; v++) {
+            const int it = threadIdx.x + v * blockDim.x;
+            if (it < npen * R2) {
+                const int pen = it / R2, j = it % R2;
+                cplx<T>* base = buf + pen * PS;
+#pragma unroll
+                for (int u = 0; u < R1; u++) base[lpad(R1 * j + u)] = a[v][u];
+            }
+        }
+        __syncthreads();
+    }
+    for (int it = threadIdx.x; it < npen * R1; it += blockDim.x) {
+        const int pen = it / R1, j = it % R1;
+        cplx<T>* base = buf + pen * PS;
+        cplx<T> a[R2];
+#pragma unroll
+        for (int u = 0; u < R2; u++) a[u] = base[lpad(j + R1 * u)];
+        small_dft<T, R2, SIGN>(a);
+#pragma unroll
+        for (int u = 0; u < R2; u++) base[lpad(j + R1 * u)] = a[u];
+    }
+    __syncthreads();
+}
+
+// Runs all stages; returns the buffer holding the result (a or b).// Runs all stages; returns the buffer holding the result (a or b).
 template <typename T, int SIGN>
-__device__ cplx<T>* fft_pencils(cplx<T>* a, cplx<T>* b, int npen, int PS, const FFTArgs& fa,
+__device__ __forceinline__ cplx<T>* fft_pencils(cplx<T>* a, cplx<T>* b, int npen, int PS, const FFTArgs& fa,
                                 const cplx<T>* tw) {
     for (int st = 0; st < fa.nst; st++) {
         const int r = fa.radix[st];

@@ -608,6 +608,15 @@ fft_y_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ 
     }
 }
 
+// K = R1 R2 (both powers of two): the two-stage transform; otherwise the
+// planned radix-8/4/2/3/5 stages
+template <typename T, int SIGN, int MAXI2, int R1, int R2>
+__device__ __forceinline__ void fft_any(cplx<T>* buf, int npen, int PS, const FFTArgs& fa,
+                                        const cplx<T>* tw) {
+    if constexpr (R1 > 0) fft2_pencils<T, R1, R2, SIGN, (R2 >= 16 ? 2 : 1)>(buf, npen, PS, tw);
+    else fft_pencils_inplace<T, SIGN, MAXI2>(buf, npen, PS, fa, tw);
+}
+
 // ------------------------------------------------ fused x + y plane -------
 // One workgroup per z-plane of one mesh: the K x K real plane is read once,
 // transformed along x (two real rows per complex FFT) and along y in LDS
@@ -617,7 +626,7 @@ fft_y_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ 
 // MAXP pack/unpack items per thread (K <= 128: 5 and 8; K <= 180: 8 and 16).
 constexpr int kPlaneThreads = 1024;
 
-template <typename T, int MAXI2, int MAXP>
+template <typename T, int MAXI2, int MAXP, int R1, int R2>
 __global__ void __launch_bounds__(kPlaneThreads)
 fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict__ spec,
                      const double* __restrict__ tw_g) {
@@ -634,7 +643,7 @@ fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict
         buf[j * PS + lpad(x)] = cplx<T>{src[(size_t)(2 * j) * K + x], src[(size_t)(2 * j + 1) * K + x]};
     }
     __syncthreads();
-    fft_pencils_inplace<T, -1, MAXI2>(buf, H, PS, fa, tw);
+    fft_any<T, -1, MAXI2, R1, R2>(buf, H, PS, fa, tw);
     // unpack Z_j = A + i B into the half spectra of rows 2j and 2j+1, stored
     // as y-pencils (pencil kx, element y)
     {
@@ -664,7 +673,7 @@ fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict
         }
         __syncthreads();
     }
-    fft_pencils_inplace<T, -1, MAXI2>(buf, Kh, PS, fa, tw);
+    fft_any<T, -1, MAXI2, R1, R2>(buf, Kh, PS, fa, tw);
     cplx<T>* dst = spec + plane * K * Kh;
     for (int i = threadIdx.x; i < K * Kh; i += blockDim.x) {
         const int y = (int)fdiv(i, fa.Kh_m), kx = i - y * Kh;
@@ -672,7 +681,7 @@ fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict
     }
 }
 
-template <typename T, int MAXI2, int MAXP>
+template <typename T, int MAXI2, int MAXP, int R1, int R2>
 __global__ void __launch_bounds__(kPlaneThreads)
 fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict__ real,
                      const double* __restrict__ tw_g) {
@@ -688,7 +697,7 @@ fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict
         buf[kx * PS + lpad(y)] = src[i];
     }
     __syncthreads();
-    fft_pencils_inplace<T, +1, MAXI2>(buf, Kh, PS, fa, tw);
+    fft_any<T, +1, MAXI2, R1, R2>(buf, Kh, PS, fa, tw);
     // pack the half spectra of rows 2j, 2j+1 into Z_j = A + i B (full x)
     {
         cplx<T> vz[MAXP];
@@ -716,7 +725,7 @@ fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict
         }
         __syncthreads();
     }
-    fft_pencils_inplace<T, +1, MAXI2>(buf, H, PS, fa, tw);
+    fft_any<T, +1, MAXI2, R1, R2>(buf, H, PS, fa, tw);
     T* dst = real + plane * K * K;
     for (int i = threadIdx.x; i < H * K; i += blockDim.x) {
         const int j = (int)fdiv(i, fa.K_m), x = i - j * K;
@@ -740,7 +749,7 @@ fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict
 constexpr int kZThreads = 1024;
 constexpr int kZMaxI2 = 8;
 
-template <typename T>
+template <typename T, int R1, int R2>
 __global__ void __launch_bounds__(kZThreads)
 fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ tw_g,
                        const double* __restrict__ bsp2, int lnp, int NC, double box, double xi) {
@@ -765,7 +774,7 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
             (jl < nvalid) ? base[c * mesh_sz + (size_t)z * nj + jl] : cplx<T>{T(0), T(0)};
     }
     __syncthreads();
-    fft_pencils_inplace<T, -1, kZMaxI2>(buf, NC * NP, PS, fa, tw);
+    fft_any<T, -1, kZMaxI2, R1, R2>(buf, NC * NP, PS, fa, tw);
     // influence
     const T twopi_l = (T)(2.0 * M_PI / box);
     const T inv_xi2 = (T)(1.0 / (xi * xi));
@@ -828,7 +837,7 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
         }
     }
     __syncthreads();
-    fft_pencils_inplace<T, +1, kZMaxI2>(buf, NC * NP, PS, fa, tw);
+    fft_any<T, +1, kZMaxI2, R1, R2>(buf, NC * NP, PS, fa, tw);
     for (int i = threadIdx.x; i < NC * NP * K; i += blockDim.x) {
         const int jl = i & (NP - 1);
         const int cz = i >> lnp;
@@ -1220,13 +1229,22 @@ void SpmeGpu<T>::fft_forward(int b, hipStream_t s) {
     if (const int pv = plane_path<T>(K)) {
         const size_t lds = plane_lds_bytes<T>(K);
         const dim3 grid((unsigned)(nmesh(b) * K));
-        if (pv == 1) {
-            allow_lds<T>((const void*)fft_plane_fwd_kernel<T, 5, 8>, lds);
-            hipLaunchKernelGGL((fft_plane_fwd_kernel<T, 5, 8>), grid, dim3(kPlaneThreads), lds, s,
+        if (sizeof(T) == 4 && K == 128) {
+            // two-stage 16 x 8 transforms (FP32: 16 complex registers)
+            allow_lds<T>((const void*)fft_plane_fwd_kernel<T, 5, 8, 16, 8>, lds);
+            hipLaunchKernelGGL((fft_plane_fwd_kernel<T, 5, 8, 16, 8>), grid, dim3(kPlaneThreads), lds, s,
+                               fa, d_real.get(), d_spec.get(), d_tw.get());
+        } else if (K == 64) {
+            allow_lds<T>((const void*)fft_plane_fwd_kernel<T, 5, 8, 8, 8>, lds);
+            hipLaunchKernelGGL((fft_plane_fwd_kernel<T, 5, 8, 8, 8>), grid, dim3(kPlaneThreads), lds, s,
+                               fa, d_real.get(), d_spec.get(), d_tw.get());
+        } else if (pv == 1) {
+            allow_lds<T>((const void*)fft_plane_fwd_kernel<T, 5, 8, 0, 0>, lds);
+            hipLaunchKernelGGL((fft_plane_fwd_kernel<T, 5, 8, 0, 0>), grid, dim3(kPlaneThreads), lds, s,
                                fa, d_real.get(), d_spec.get(), d_tw.get());
         } else if constexpr (sizeof(T) == 4) {
-            allow_lds<T>((const void*)fft_plane_fwd_kernel<T, 8, 16>, lds);
-            hipLaunchKernelGGL((fft_plane_fwd_kernel<T, 8, 16>), grid, dim3(kPlaneThreads), lds, s,
+            allow_lds<T>((const void*)fft_plane_fwd_kernel<T, 8, 16, 0, 0>, lds);
+            hipLaunchKernelGGL((fft_plane_fwd_kernel<T, 8, 16, 0, 0>), grid, dim3(kPlaneThreads), lds, s,
                                fa, d_real.get(), d_spec.get(), d_tw.get());
         }
         SDT_HIP_CHECK(hipGetLastError());
@@ -1261,10 +1279,22 @@ void SpmeGpu<T>::influence(int b, hipStream_t s) {
     SDT_REQUIRE(lds_of(lnp) <= max_lds_bytes() && NC * K <= 2 * kZMaxI2 * kZThreads, SDT_EINVAL,
                 "z transform: mesh dimension too large");
     const size_t lds = lds_of(lnp);
-    allow_lds<T>((const void*)fft_z_influence_kernel<T>, lds);
     const unsigned nch = ceil_div(nj, (size_t)1 << lnp);
-    hipLaunchKernelGGL((fft_z_influence_kernel<T>), dim3(nch * (unsigned)b), dim3(kZThreads), lds,
-                       s, fa, d_spec.get(), d_tw.get(), d_bsp2.get(), lnp, NC, box, xi);
+    const dim3 grid(nch * (unsigned)b);
+    // two-stage transforms need one stage-1 butterfly per thread
+    if (sizeof(T) == 4 && K == 128 && (NC << lnp) * 8 <= kZThreads) {
+        allow_lds<T>((const void*)fft_z_influence_kernel<T, 16, 8>, lds);
+        hipLaunchKernelGGL((fft_z_influence_kernel<T, 16, 8>), grid, dim3(kZThreads), lds, s, fa,
+                           d_spec.get(), d_tw.get(), d_bsp2.get(), lnp, NC, box, xi);
+    } else if (K == 64 && (NC << lnp) * 8 <= kZThreads) {
+        allow_lds<T>((const void*)fft_z_influence_kernel<T, 8, 8>, lds);
+        hipLaunchKernelGGL((fft_z_influence_kernel<T, 8, 8>), grid, dim3(kZThreads), lds, s, fa,
+                           d_spec.get(), d_tw.get(), d_bsp2.get(), lnp, NC, box, xi);
+    } else {
+        allow_lds<T>((const void*)fft_z_influence_kernel<T, 0, 0>, lds);
+        hipLaunchKernelGGL((fft_z_influence_kernel<T, 0, 0>), grid, dim3(kZThreads), lds, s, fa,
+                           d_spec.get(), d_tw.get(), d_bsp2.get(), lnp, NC, box, xi);
+    }
     SDT_HIP_CHECK(hipGetLastError());
 }
 
@@ -1276,13 +1306,22 @@ void SpmeGpu<T>::fft_backward(int b, hipStream_t s) {
     if (const int pv = plane_path<T>(K)) {
         const size_t lds = plane_lds_bytes<T>(K);
         const dim3 grid((unsigned)(nmesh(b) * K));
-        if (pv == 1) {
-            allow_lds<T>((const void*)fft_plane_inv_kernel<T, 5, 8>, lds);
-            hipLaunchKernelGGL((fft_plane_inv_kernel<T, 5, 8>), grid, dim3(kPlaneThreads), lds, s,
+        if (sizeof(T) == 4 && K == 128) {
+            // two-stage 16 x 8 transforms (FP32: 16 complex registers)
+            allow_lds<T>((const void*)fft_plane_inv_kernel<T, 5, 8, 16, 8>, lds);
+            hipLaunchKernelGGL((fft_plane_inv_kernel<T, 5, 8, 16, 8>), grid, dim3(kPlaneThreads), lds, s,
+                               fa, d_spec.get(), d_real.get(), d_tw.get());
+        } else if (K == 64) {
+            allow_lds<T>((const void*)fft_plane_inv_kernel<T, 5, 8, 8, 8>, lds);
+            hipLaunchKernelGGL((fft_plane_inv_kernel<T, 5, 8, 8, 8>), grid, dim3(kPlaneThreads), lds, s,
+                               fa, d_spec.get(), d_real.get(), d_tw.get());
+        } else if (pv == 1) {
+            allow_lds<T>((const void*)fft_plane_inv_kernel<T, 5, 8, 0, 0>, lds);
+            hipLaunchKernelGGL((fft_plane_inv_kernel<T, 5, 8, 0, 0>), grid, dim3(kPlaneThreads), lds, s,
                                fa, d_spec.get(), d_real.get(), d_tw.get());
         } else if constexpr (sizeof(T) == 4) {
-            allow_lds<T>((const void*)fft_plane_inv_kernel<T, 8, 16>, lds);
-            hipLaunchKernelGGL((fft_plane_inv_kernel<T, 8, 16>), grid, dim3(kPlaneThreads), lds, s,
+            allow_lds<T>((const void*)fft_plane_inv_kernel<T, 8, 16, 0, 0>, lds);
+            hipLaunchKernelGGL((fft_plane_inv_kernel<T, 8, 16, 0, 0>), grid, dim3(kPlaneThreads), lds, s,
                                fa, d_spec.get(), d_real.get(), d_tw.get());
         }
         SDT_HIP_CHECK(hipGetLastError());

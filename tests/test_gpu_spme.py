@@ -24,6 +24,7 @@ CASES = [
     (300, 20.0, 0.9, 5.0, 32, 7, True, 2),
     (250, 20.0, 0.8, 8.3, 48, 8, False, 17),
     (200, 16.0, 1.0, 4.0, 40, 6, True, 4),
+    (300, 20.0, 0.9, 5.0, 64, 6, True, 2),
     # K not a multiple of 4: the row-owner spread kernel
     (300, 20.0, 0.9, 5.0, 30, 6, False, 3),
     (300, 20.0, 0.9, 5.0, 50, 5, True, 2),
