@@ -130,12 +130,13 @@ void MobSpmeGpu::apply_packed(int mode, int b, double alpha, const double* X, do
         StageScope scope(timer_, 5, s);
         const double rb = (mode == 0) ? 1.0 : beta;
         if (prec_ == kFP32) {
-            d_x32_.reserve((size_t)3 * npos_ * b);
+            d_xblk_.reserve(ceil_div(real_scratch_bytes<float>(npos_, b), sizeof(double)));
             real_matvec<float>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha,
-                               X, rb, Y, d_x32_.get(), s);
+                               X, rb, Y, d_xblk_.get(), s);
         } else {
+            d_xblk_.reserve(ceil_div(real_scratch_bytes<double>(npos_, b), sizeof(double)));
             real_matvec<double>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha,
-                                X, rb, Y, nullptr, s);
+                                X, rb, Y, d_xblk_.get(), s);
         }
         real_overlap(ovl_, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha, X, Y, s);
     }

@@ -54,7 +54,7 @@ class MobSpmeGpu {
     std::unique_ptr<SpmeGpu<double>> spme64_;
     PairListGpu pairs_;
     RealOverlap ovl_;
-    DevBuf<float> d_x32_;     // FP32 copy of the packed input for the real space
+    DevBuf<double> d_xblk_;   // column-blocked copy of the packed input (real space)
     SortScratch sc_;
     DevBuf<double> d_pos_, d_rdi_, d_pos_s_, d_rdi_s_;
     DevBuf<int> d_key_, d_perm_, d_iperm_, d_tstart_;

@@ -21,12 +21,14 @@ struct RealOverlap {
 void real_overlap(const RealOverlap& ov, const double* d_pos, const double* d_rdi, double box,
                   double xi, int b, double alpha, const double* X, double* Y, hipStream_t s);
 
-// without the overlapping pairs (those are added by real_overlap). With
-// T = float and x32 (3N b floats of scratch) the kernel gathers an FP32 copy
-// of X.
+// without the overlapping pairs (those are added by real_overlap), in
+// arithmetic T. For b >= 8 the kernel gathers a column-blocked copy of X (in
+// T) written to scratch, real_scratch_bytes<T>(npos, b) bytes (0 for b < 8).
+template <typename T>
+size_t real_scratch_bytes(int npos, int b);
 template <typename T>
 void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
                  const double* d_rdi, double box, double xi, int b, double alpha, const double* X,
-                 double beta, double* Y, float* x32, hipStream_t s);
+                 double beta, double* Y, void* scratch, hipStream_t s);
 
 }  // namespace sdt

@@ -44,7 +44,10 @@ __device__ __forceinline__ void ewald_scalars(T xi, T r, T aa, T ab, T* xa, T* y
 // overlapping pairs are added by real_overlap_kernel. LPR consecutive lanes
 // share a row, each taking every LPR-th pair, so a row's gathers are in
 // flight together; partial sums are combined with cross-lane shuffles.
-template <typename T, int RC, int LPR, typename XT>
+// BLK: X is the column-blocked copy Xb[cb][j][c][r] (RC columns per block,
+// zero-padded), so the 3 RC values a pair gathers are one contiguous run of
+// 12 RC or 24 RC bytes instead of three pieces of rows b values apart.
+template <typename T, int RC, int LPR, typename XT, bool BLK>
 __global__ void __launch_bounds__(256)
 real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restrict__ colidx,
                    const double* __restrict__ pos, const double* __restrict__ rdi, double box,
@@ -75,15 +78,25 @@ real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restri
         const int kend = rowptr[i + 1];
         for (int k = rowptr[i] + sub; k < kend; k += LPR) {
             const int j = colidx[k];
-            const XT* xj = X + (size_t)(3 * j) * b + r0;
             XT xv[3][RC];
+            if constexpr (BLK) {
+                const XT* xj = X + ((size_t)byl * npos + j) * (3 * RC);
 #pragma unroll
-            for (int r = 0; r < RC; r++) {
-                xv[0][r] = xv[1][r] = xv[2][r] = XT(0);
-                if (r < rc) {
+                for (int r = 0; r < RC; r++) {
                     xv[0][r] = xj[r];
-                    xv[1][r] = xj[b + r];
-                    xv[2][r] = xj[2 * b + r];
+                    xv[1][r] = xj[RC + r];
+                    xv[2][r] = xj[2 * RC + r];
+                }
+            } else {
+                const XT* xj = X + (size_t)(3 * j) * b + r0;
+#pragma unroll
+                for (int r = 0; r < RC; r++) {
+                    xv[0][r] = xv[1][r] = xv[2][r] = XT(0);
+                    if (r < rc) {
+                        xv[0][r] = xj[r];
+                        xv[1][r] = xj[b + r];
+                        xv[2][r] = xj[2 * b + r];
+                    }
                 }
             }
             T bxx, bxy, bxz, byy, byz, bzz;
@@ -136,51 +149,82 @@ real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restri
     }
 }
 
-__global__ void to_float_kernel(long long n, const double* __restrict__ x, float* __restrict__ y) {
+// Xb[cb][j][c][r] = X[(3 j + c) b + cb RC + r], 0 beyond column b: the
+// column-blocked copy the gather kernel reads (OT = float: FP32 arithmetic,
+// half the bytes per gathered neighbour)
+template <typename OT, int RC>
+__global__ void block_copy_kernel(int npos, int b, int nblk, const double* __restrict__ x,
+                                  OT* __restrict__ y) {
+    const long long n = (long long)nblk * npos * 3 * RC;
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        y[i] = (float)x[i];
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        const int r = (int)(e % RC);
+        const long long q = e / RC;            // (cb npos + j) 3 + c
+        const long long row = q % (3LL * npos);
+        const int cb = (int)(q / (3LL * npos));
+        const int col = cb * RC + r;
+        y[e] = col < b ? (OT)x[row * b + col] : OT(0);
+    }
 }
 
-template <typename T, int RC, typename XT>
+template <typename T, int RC, typename XT, bool BLK>
 static void launch_real(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
                         const double* d_rdi, double box, double xi, int b, double alpha,
                         const XT* X, double beta, double* Y, hipStream_t s) {
     constexpr int LPR = 8;
     dim3 grid(ceil_div((size_t)npos * LPR, 256), ceil_div(b, RC));
-    hipLaunchKernelGGL((real_matvec_kernel<T, RC, LPR, XT>), grid, dim3(256), 0, s, npos,
+    hipLaunchKernelGGL((real_matvec_kernel<T, RC, LPR, XT, BLK>), grid, dim3(256), 0, s, npos,
                        d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X, beta, Y);
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, int RC, typename XT>
+static void blocked_real(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
+                         const double* d_rdi, double box, double xi, int b, double alpha,
+                         const double* X, double beta, double* Y, XT* xb, hipStream_t s) {
+    const int nblk = ceil_div(b, RC);
+    const long long n = (long long)nblk * npos * 3 * RC;
+    hipLaunchKernelGGL((block_copy_kernel<XT, RC>),
+                       dim3(std::min<long long>(ceil_div(n, 256), 8192)), dim3(256), 0, s, npos,
+                       b, nblk, X, xb);
+    launch_real<T, RC, XT, true>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, xb,
+                                 beta, Y, s);
+}
+
+// columns per block of the blocked copy (0: X is gathered in place)
+template <typename T>
+static int real_block_cols(int b) {
+    if (b < 8) return 0;
+    return (sizeof(T) == 4 && b >= 16) ? 16 : 8;
+}
+
+template <typename T>
+size_t real_scratch_bytes(int npos, int b) {
+    const int rc = real_block_cols<T>(b);
+    if (rc == 0) return 0;
+    return (size_t)ceil_div(b, rc) * rc * 3 * npos * sizeof(T);
 }
 
 template <typename T>
 void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
                  const double* d_rdi, double box, double xi, int b, double alpha, const double* X,
-                 double beta, double* Y, float* x32, hipStream_t s) {
-    if constexpr (sizeof(T) == 4) {
-        if (x32 != nullptr && b >= 8) {
-            // FP32 arithmetic: gather an FP32 copy of X (half the bytes per
-            // neighbour row), 16 columns per thread
-            const long long n = 3LL * npos * b;
-            hipLaunchKernelGGL(to_float_kernel, dim3(std::min<long long>(ceil_div(n, 256), 8192)),
-                               dim3(256), 0, s, n, X, x32);
-            if (b >= 16)
-                launch_real<T, 16, float>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha,
-                                          x32, beta, Y, s);
-            else
-                launch_real<T, 8, float>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha,
-                                         x32, beta, Y, s);
-            SDT_HIP_CHECK(hipGetLastError());
-            return;
-        }
-    }
-    if (b >= 8) {
-        launch_real<T, 8, double>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X,
-                                  beta, Y, s);
+                 double beta, double* Y, void* scratch, hipStream_t s) {
+    const int rc = real_block_cols<T>(b);
+    SDT_REQUIRE(rc == 0 || scratch != nullptr, SDT_EINVAL, "real space: scratch buffer missing");
+    if (rc == 16) {
+        if constexpr (sizeof(T) == 4)
+            blocked_real<T, 16, T>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X,
+                                   beta, Y, static_cast<T*>(scratch), s);
+    } else if (rc == 8) {
+        blocked_real<T, 8, T>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X, beta,
+                              Y, static_cast<T*>(scratch), s);
+    } else if (b == 1) {
+        launch_real<T, 1, double, false>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha,
+                                         X, beta, Y, s);
     } else {
-        launch_real<T, 4, double>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X,
-                                  beta, Y, s);
+        launch_real<T, 4, double, false>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha,
+                                         X, beta, Y, s);
     }
-    SDT_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------------------ overlaps -----
@@ -299,11 +343,13 @@ void real_overlap(const RealOverlap& ov, const double* d_pos, const double* d_rd
     SDT_HIP_CHECK(hipGetLastError());
 }
 
+template size_t real_scratch_bytes<float>(int, int);
+template size_t real_scratch_bytes<double>(int, int);
 template void real_matvec<float>(int, const int*, const int*, const double*, const double*, double,
-                                 double, int, double, const double*, double, double*, float*,
+                                 double, int, double, const double*, double, double*, void*,
                                  hipStream_t);
 template void real_matvec<double>(int, const int*, const int*, const double*, const double*, double,
-                                  double, int, double, const double*, double, double*, float*,
+                                  double, int, double, const double*, double, double*, void*,
                                   hipStream_t);
 
 }  // namespace sdt
