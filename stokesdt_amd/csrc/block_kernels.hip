@@ -1,6 +1,7 @@
 // Tall-skinny FP64 block kernels for block Lanczos (packed row-major n x b
 // blocks) and vector layout conversion.
 #include "block_gpu.hpp"
+#include "fft_lds.hpp"
 
 namespace sdt {
 
@@ -26,8 +27,11 @@ gram_partial_kernel(long long n, int btot, int bs, int ngroup, const double* __r
     __shared__ double s_red[kGramThreads / 64][256];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int g = blockIdx.x % ngroup;
-    const int chunk = blockIdx.x / ngroup;
+    // the groups of one row chunk run on one XCD, so the chunk's rows come
+    // from HBM into that XCD's L2 once
+    const unsigned lid = xcd_logical_id(blockIdx.x, gridDim.x);
+    const int g = (int)(lid % ngroup);
+    const int chunk = (int)(lid / ngroup);
     const int c0 = g * bs;
     const int bg = min(bs, btot - c0);
     const long long row0 = (long long)chunk * kGramRows;
@@ -39,14 +43,23 @@ gram_partial_kernel(long long n, int btot, int bs, int ngroup, const double* __r
     const int mcol = lane & 15;
     const int krow = lane >> 4;
     d4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (long long r = row0 + wave * 4; r < rend; r += 4 * (kGramThreads / 64)) {
-        const long long rr = r + krow;
-        double a = 0.0, bb = 0.0;
-        if (rr < rend && mcol < bg) {
-            a = A[rr * btot + c0 + mcol];
-            bb = B[rr * btot + c0 + mcol];
+    // 32 rows per wave and step: the 16 loads of a step are in flight together
+    constexpr int U = 8;
+    for (long long r = row0 + wave * 4 * U; r < rend; r += 4 * U * (kGramThreads / 64)) {
+        double a[U], bb[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long long rr = r + 4 * u + krow;
+            a[u] = 0.0;
+            bb[u] = 0.0;
+            if (rr < rend && mcol < bg) {
+                a[u] = A[rr * btot + c0 + mcol];
+                bb[u] = B[rr * btot + c0 + mcol];
+            }
         }
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bb[u], acc, 0, 0, 0);
     }
     // write this wave's 16x16 tile to LDS, then sum waves
 #pragma unroll
@@ -56,16 +69,26 @@ gram_partial_kernel(long long n, int btot, int bs, int ngroup, const double* __r
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < kGramThreads / 64; w++) s += s_red[w][t];
-    part[(size_t)blockIdx.x * 256 + t] = s;
+    part[(size_t)lid * 256 + t] = s;
 }
 
-// G[g][16][16] = sum over chunks of the partial tiles of group g
+// G[g][16][16] = sum over chunks of the partial tiles of group g (four
+// interleaved partial sums combined in a fixed order: deterministic)
 __global__ void gram_reduce_kernel(int nchunk, int ngroup, const double* __restrict__ part,
                                    double* __restrict__ G) {
     const int g = blockIdx.x, t = threadIdx.x;
-    double s = 0.0;
-    for (int p = 0; p < nchunk; p++) s += part[((size_t)p * ngroup + g) * 256 + t];
-    G[(size_t)g * 256 + t] = s;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    const size_t st = (size_t)ngroup * 256;
+    const double* p0 = part + (size_t)g * 256 + t;
+    int p = 0;
+    for (; p + 4 <= nchunk; p += 4) {
+        s0 += p0[(size_t)p * st];
+        s1 += p0[(size_t)(p + 1) * st];
+        s2 += p0[(size_t)(p + 2) * st];
+        s3 += p0[(size_t)(p + 3) * st];
+    }
+    for (; p < nchunk; p++) s0 += p0[(size_t)p * st];
+    G[(size_t)g * 256 + t] = (s0 + s1) + (s2 + s3);
 }
 
 void gram_groups(long long n, int btot, int bs, const double* A, const double* B, double* d_G,
@@ -89,17 +112,110 @@ void gram(long long n, int b, const double* A, const double* B, double* d_G, Dev
 // ------------------------------------------------------- small products ----
 // C[i][c] = beta C[i][c] + alpha sum_k sum_j A_k[i][c0 + j] S[g][k][j][c - c0]
 // for column c of group g (columns c0 .. c0 + b_g - 1). A is nblk blocks of
-// n x btot (block k at A + k * blk_stride); S is [ngroup][nblk][bs][bs] on
-// the device, staged in LDS.
+// n x btot (block k at A + k * blk_stride); S is [ngroup][nblk][bs][bs].
+//
+// A workgroup owns kTsRows consecutive rows: for each block k it copies the
+// rows (one contiguous run of kTsRows btot values) into LDS with coalesced
+// loads, then thread (c, h) adds, for its column c and rows h, h + 2, ...,
+// the products with column c - c0 of S_gk held in registers. Every value of
+// A is read from HBM once and C is written in whole rows. In LDS each group
+// is followed by one unused slot (row length btot + ngroup), so the groups a
+// wave reads at once fall on distinct banks. C may alias A when nblk = 1
+// (in-place C = C S): a tile is in LDS before any of its rows is written.
+constexpr int kTsRows = 32;
+constexpr int kTsCols = 128;   // widest packed block of the tile kernel
+constexpr int kTsLoads = kTsRows * kTsCols / 256;   // tile elements per thread
+
+__global__ void __launch_bounds__(256)
+tsmm_tile_kernel(long long n, int btot, int bs, unsigned bmag, int nblk, const double* A,
+                 long long blk_stride, const double* __restrict__ S, double alpha, double beta,
+                 double* C) {
+    extern __shared__ double tile[];   // [kTsRows][ld]
+    const int ld = btot + (btot + bs - 1) / bs;
+    const long long row0 = (long long)blockIdx.x * kTsRows;
+    const int nr = (int)min((long long)kTsRows, n - row0);
+    const int c = threadIdx.x % kTsCols, h = threadIdx.x / kTsCols;
+    const bool on = c < btot;
+    const int g = c / bs, cl = c - g * bs, c0 = g * bs;
+    const int bg = min(bs, btot - c0);
+    const int gb = c0 + g;             // LDS offset of the group in a row
+    double acc[kTsRows / 2];
+#pragma unroll
+    for (int m = 0; m < kTsRows / 2; m++) acc[m] = 0.0;
+    // flat coalesced copies, all loads of a thread in flight together;
+    // (row, column) of element e advance by (dq, dr) per step, the group of
+    // a column comes from a magic divide
+    const int ne = nr * btot;
+    double v[kTsLoads];
+    auto load = [&](int k) __attribute__((always_inline)) {
+        const double* src = A + k * blk_stride + row0 * btot;
+#pragma unroll
+        for (int u = 0; u < kTsLoads; u++) {
+            const int e = threadIdx.x + u * 256;
+            v[u] = e < ne ? src[e] : 0.0;
+        }
+    };
+    for (int k = 0; k < nblk; k++) {
+        if (k > 0) __syncthreads();
+        load(k);
+        {
+            int r = threadIdx.x / btot, cc = threadIdx.x - r * btot;
+            const int dq = 256 / btot, dr = 256 - dq * btot;
+#pragma unroll
+            for (int u = 0; u < kTsLoads; u++) {
+                if (threadIdx.x + u * 256 < ne) tile[r * ld + cc + (int)fdiv((unsigned)cc, bmag)] = v[u];
+                r += dq;
+                cc += dr;
+                if (cc >= btot) {
+                    cc -= btot;
+                    r++;
+                }
+            }
+        }
+        __syncthreads();
+        if (on) {
+            double sv[16];
+            const double* sk = S + ((size_t)(g * nblk + k) * bs) * bs + cl;
+#pragma unroll
+            for (int j = 0; j < 16; j++) sv[j] = (j < bg) ? sk[(size_t)j * bs] : 0.0;
+#pragma unroll
+            for (int m = 0; m < kTsRows / 2; m++) {
+                const int r = h + 2 * m;
+                if (r < nr) {
+                    const double* t = tile + r * ld + gb;
+                    double a = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 16; j++)
+                        if (j < bg) a += t[j] * sv[j];
+                    acc[m] += a;
+                }
+            }
+        }
+    }
+    if (!on) return;
+    double* cc = C + (row0 + h) * btot + c;
+    if (beta != 0.0) {
+        // every old value is loaded before the first store
+        double old[kTsRows / 2];
+#pragma unroll
+        for (int m = 0; m < kTsRows / 2; m++)
+            old[m] = (h + 2 * m < nr) ? cc[(size_t)(2 * m) * btot] : 0.0;
+#pragma unroll
+        for (int m = 0; m < kTsRows / 2; m++) acc[m] = beta * old[m] + alpha * acc[m];
+    } else {
+#pragma unroll
+        for (int m = 0; m < kTsRows / 2; m++) acc[m] = alpha * acc[m];
+    }
+#pragma unroll
+    for (int m = 0; m < kTsRows / 2; m++)
+        if (h + 2 * m < nr) cc[(size_t)(2 * m) * btot] = acc[m];
+}
+
+// wider blocks: one thread per element
 __global__ void __launch_bounds__(256)
 tsmm_kernel(long long n, int btot, int bs, int nblk, const double* __restrict__ A,
             long long blk_stride, const double* __restrict__ S, double alpha, double beta,
             double* __restrict__ C) {
-    extern __shared__ double s_S[];
-    const int ngroup = (btot + bs - 1) / bs;
-    const int ns = ngroup * nblk * bs * bs;
-    for (int t = threadIdx.x; t < ns; t += blockDim.x) s_S[t] = S[t];
-    __syncthreads();
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n * btot) return;
     const long long i = gid / btot;
@@ -109,69 +225,42 @@ tsmm_kernel(long long n, int btot, int bs, int nblk, const double* __restrict__ 
     double acc = 0.0;
     for (int k = 0; k < nblk; k++) {
         const double* a = A + k * blk_stride + i * btot + c0;
-        const double* sk = s_S + ((size_t)(g * nblk + k) * bs) * bs + cl;
+        const double* sk = S + ((size_t)(g * nblk + k) * bs) * bs + cl;
         for (int j = 0; j < bg; j++) acc += a[j] * sk[(size_t)j * bs];
     }
     double* cp = C + i * btot + c;
     *cp = (beta == 0.0 ? 0.0 : beta * *cp) + alpha * acc;
 }
 
-void tsmm_groups(long long n, int btot, int bs, int nblk, const double* A, long long blk_stride,
-                 const double* d_S, double alpha, double beta, double* C, hipStream_t s) {
-    const int ngroup = (btot + bs - 1) / bs;
-    const size_t lds = sizeof(double) * (size_t)ngroup * nblk * bs * bs;
-    SDT_REQUIRE(lds <= 64 * 1024, SDT_ECAP, "tsmm: coefficient block too large for LDS");
-    SDT_REQUIRE(n * btot < (1ll << 40), SDT_EINVAL, "tsmm: block too large");
-    hipLaunchKernelGGL(tsmm_kernel, dim3(ceil_div(n * btot, 256)), dim3(256), lds, s, n, btot, bs,
-                       nblk, A, blk_stride, d_S, alpha, beta, C);
+static void launch_tsmm(long long n, int btot, int bs, int nblk, const double* A,
+                        long long blk_stride, const double* d_S, double alpha, double beta,
+                        double* C, hipStream_t s) {
+    if (btot <= kTsCols) {
+        const size_t lds = sizeof(double) * kTsRows * (btot + (btot + bs - 1) / bs);
+        hipLaunchKernelGGL(tsmm_tile_kernel, dim3(ceil_div(n, kTsRows)), dim3(256), lds, s, n,
+                           btot, bs, fft_magic((unsigned)bs), nblk, A, blk_stride, d_S, alpha,
+                           beta, C);
+    } else {
+        // rows are read and written by different threads: no aliasing
+        SDT_REQUIRE(C != A, SDT_EINVAL, "tsmm: in-place product needs at most 128 columns");
+        hipLaunchKernelGGL(tsmm_kernel, dim3(ceil_div(n * btot, 256)), dim3(256), 0, s, n, btot,
+                           bs, nblk, A, blk_stride, d_S, alpha, beta, C);
+    }
     SDT_HIP_CHECK(hipGetLastError());
 }
 
-// One coefficient block per group, one thread per (row, group): the thread
-// reads the row's b_g values of A, then writes the b_g results, so C may
-// alias A (in-place C = C S).
-__global__ void __launch_bounds__(256)
-tsmm_rowgroup_kernel(long long n, int btot, int bs, const double* A, const double* __restrict__ S,
-                     double alpha, double beta, double* C) {
-    extern __shared__ double s_S[];
-    const int ngroup = (btot + bs - 1) / bs;
-    for (int t = threadIdx.x; t < ngroup * bs * bs; t += blockDim.x) s_S[t] = S[t];
-    __syncthreads();
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * ngroup) return;
-    const long long i = t / ngroup;
-    const int g = (int)(t - i * ngroup);
-    const int c0 = g * bs, bg = min(bs, btot - c0);
-    const double* sg = s_S + (size_t)g * bs * bs;
-    double a[16], out[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) a[j] = (j < bg) ? A[i * btot + c0 + j] : 0.0;
-#pragma unroll
-    for (int c = 0; c < 16; c++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int j = 0; j < 16; j++)
-            if (j < bg && c < bg) acc += a[j] * sg[j * bs + c];
-        out[c] = acc;
-    }
-#pragma unroll
-    for (int c = 0; c < 16; c++) {
-        if (c < bg) {
-            double* cp = C + i * btot + c0 + c;
-            *cp = (beta == 0.0 ? 0.0 : beta * *cp) + alpha * out[c];
-        }
-    }
+void tsmm_groups(long long n, int btot, int bs, int nblk, const double* A, long long blk_stride,
+                 const double* d_S, double alpha, double beta, double* C, hipStream_t s) {
+    SDT_REQUIRE(bs >= 1 && bs <= 16, SDT_EINVAL, "group width must be in [1, 16]");
+    SDT_REQUIRE(n * btot < (1ll << 40), SDT_EINVAL, "tsmm: block too large");
+    SDT_REQUIRE(nblk == 1 || C != A, SDT_EINVAL, "tsmm: in-place product needs one block");
+    launch_tsmm(n, btot, bs, nblk, A, blk_stride, d_S, alpha, beta, C, s);
 }
 
 void tsmm_groups1(long long n, int btot, int bs, const double* A, const double* d_S, double alpha,
                   double beta, double* C, hipStream_t s) {
     SDT_REQUIRE(bs >= 1 && bs <= 16, SDT_EINVAL, "group width must be in [1, 16]");
-    const int ngroup = (btot + bs - 1) / bs;
-    const size_t lds = sizeof(double) * (size_t)ngroup * bs * bs;
-    SDT_REQUIRE(lds <= 64 * 1024, SDT_ECAP, "tsmm: coefficient block too large for LDS");
-    hipLaunchKernelGGL(tsmm_rowgroup_kernel, dim3(ceil_div(n * ngroup, 256)), dim3(256), lds, s, n,
-                       btot, bs, A, d_S, alpha, beta, C);
-    SDT_HIP_CHECK(hipGetLastError());
+    launch_tsmm(n, btot, bs, 1, A, 0LL, d_S, alpha, beta, C, s);
 }
 
 void tsmm(long long n, int b, int nblk, const double* A, long long blk_stride, const double* d_S,

@@ -605,9 +605,7 @@ void StericGpu::accumulate(const double* pos, double* f, bool dev, hipStream_t s
         df = d_f_.get();
     }
     StageScope scope(timer, 7, s);
-    pairs_.build(dp, nullptr, nullptr, false, s);
-    steric_accumulate(npos_, dp, pairs_.d_rdi.get(), box_, r0_, k0_, pairs_.d_rowptr.get(),
-                      pairs_.d_colidx.get(), df, s);
+    pairs_.steric_direct(dp, r0_, k0_, df, s);
     if (!dev) {
         SDT_HIP_CHECK(hipMemcpyAsync(f, d_f_.get(), sizeof(double) * 3 * npos_,
                                      hipMemcpyDeviceToHost, s));

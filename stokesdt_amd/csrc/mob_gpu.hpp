@@ -101,7 +101,6 @@ class StericGpu {
     void init();
     // f := 0 then accumulate (ForceBase::Compute); accumulate adds
     void accumulate(const double* pos, double* f, bool dev, hipStream_t s);
-    long long last_nnz() const { return pairs_.nnz; }
     int npos() const { return npos_; }
     StageTimer* timer = nullptr;
 

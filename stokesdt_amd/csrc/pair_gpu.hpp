@@ -24,6 +24,7 @@ struct PairListGpu {
     DevBuf<int> d_cell_perm, d_cell_start;
     DevBuf<int> d_cnt, d_rowptr, d_colidx;
     DevBuf<int> d_nnz;          // 1 int (host-readable copy in h_nnz)
+    DevBuf<int> d_dcidx, d_dperm, d_dstart;   // contact-distance cells (steric_direct)
     SortScratch sc;
     int nnz = 0;
     long long colcap = 0;
@@ -36,6 +37,11 @@ struct PairListGpu {
     // ascending. Returns nnz (synchronises once to size the column array).
     int build(const double* d_pos, const int* row_map, const int* col_map, bool sort_rows,
               hipStream_t s);
+    // steric mode: f += the steric forces of the pairs build() lists, from
+    // cells as wide as the contact distance (no CSR, no host
+    // synchronisation); equal to build + steric_accumulate up to the order
+    // of each particle's FP64 sum
+    void steric_direct(const double* d_pos, double r0, double k0, double* d_f, hipStream_t s);
 };
 
 // Steric forces (src/force/force_steric.cc:14-53) on a pair list in

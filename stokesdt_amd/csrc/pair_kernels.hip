@@ -259,6 +259,90 @@ __global__ void steric_kernel(int npos, const double* __restrict__ pos, const do
     f[3 * i + 2] += f2;
 }
 
+// Steric forces straight from a cell list whose cells are at least as wide
+// as the largest contact distance cutoff * max_radius, so every partner lies
+// in the 27 cells around a particle's own. A pair is taken when
+// 2 r / (a_i + a_j) < cutoff with r from the wrapped positions exactly as in
+// find_pairs_kernel<'S'> (pair_s.cc:148-178; the reference's whole-cell
+// shortcuts only accept pairs that pass this test), and the force of every
+// taken j != i is the one steric_kernel adds; partners are visited cell by
+// cell in ascending particle order. No pair list is stored and the host
+// never waits for its size.
+__global__ void steric_cell_kernel(int npos, const double* __restrict__ pos,
+                                   const double* __restrict__ pos0,
+                                   const double* __restrict__ rdi, const int* __restrict__ cidx,
+                                   const int* __restrict__ cell_start,
+                                   const int* __restrict__ cell_perm, int nc1, double box,
+                                   double cutoff, double r0, double k0, double* __restrict__ f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npos) return;
+    const int c1 = cidx[i];
+    const int ix1 = c1 / (nc1 * nc1), iy1 = (c1 / nc1) % nc1, iz1 = c1 % nc1;
+    const double x1 = pos0[3 * i], y1 = pos0[3 * i + 1], z1 = pos0[3 * i + 2];
+    const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    const double r1 = rdi[i];
+    const double half = box / 2.0;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dz = -1; dz <= 1; dz++) {
+                const int c2 = (wrapc(ix1 + dx, nc1) * nc1 + wrapc(iy1 + dy, nc1)) * nc1 +
+                               wrapc(iz1 + dz, nc1);
+                const int jb = cell_start[c2], je = cell_start[c2 + 1];
+                for (int t = jb; t < je; t++) {
+                    const int j = cell_perm[t];
+                    if (j == i) continue;
+                    const double r2 = rdi[j];
+                    double rx = fabs(pos0[3 * j] - x1);
+                    double ry = fabs(pos0[3 * j + 1] - y1);
+                    double rz = fabs(pos0[3 * j + 2] - z1);
+                    rx = (rx > half ? rx - box : rx);
+                    ry = (ry > half ? ry - box : ry);
+                    rz = (rz > half ? rz - box : rz);
+                    double rr = rx * rx + ry * ry + rz * rz;
+                    if (rr == 0.0) rr = 1.0e-10;
+                    const double sv = 2.0 * sqrt(rr) / (r1 + r2);
+                    if (!(sv < cutoff)) continue;
+                    const double qx = remainder(px - pos[3 * j], box);
+                    const double qy = remainder(py - pos[3 * j + 1], box);
+                    const double qz = remainder(pz - pos[3 * j + 2], box);
+                    const double r = sqrt(qx * qx + qy * qy + qz * qz);
+                    const double force = k0 * ((r1 + r2) * r0 / 2.0 - r);
+                    f0 += force * (qx / r);
+                    f1 += force * (qy / r);
+                    f2 += force * (qz / r);
+                }
+            }
+    f[3 * i] += f0;
+    f[3 * i + 1] += f1;
+    f[3 * i + 2] += f2;
+}
+
+void PairListGpu::steric_direct(const double* d_pos, double r0, double k0, double* d_f,
+                                hipStream_t s) {
+    SDT_REQUIRE(mode == 'S', SDT_EINVAL, "steric_direct needs a steric ('S') cell list");
+    const int ncd = (int)(box / (cutoff * max_radius));
+    if (ncd < 3) {
+        // fewer than 3 cells per side would visit a cell twice: use the list
+        build(d_pos, nullptr, nullptr, false, s);
+        steric_accumulate(npos, d_pos, d_rdi.get(), box, r0, k0, d_rowptr.get(), d_colidx.get(),
+                          d_f, s);
+        return;
+    }
+    const unsigned g = ceil_div(npos, 256);
+    const int nc3 = ncd * ncd * ncd;
+    d_dcidx.reserve(npos);
+    d_dperm.reserve(npos);
+    d_dstart.reserve((size_t)nc3 + 1);
+    hipLaunchKernelGGL(cell_assign_kernel, dim3(g), dim3(256), 0, s, npos, d_pos, box, ncd,
+                       d_pos0.get(), d_dcidx.get());
+    bucket_sort(d_dcidx.get(), npos, nc3, d_dperm.get(), d_dstart.get(), nullptr, sc, s);
+    hipLaunchKernelGGL(steric_cell_kernel, dim3(g), dim3(256), 0, s, npos, d_pos, d_pos0.get(),
+                       d_rdi.get(), d_dcidx.get(), d_dstart.get(), d_dperm.get(), ncd, box, cutoff,
+                       r0, k0, d_f);
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
 void steric_accumulate(int npos, const double* d_pos, const double* d_rdi, double box, double r0,
                        double k0, const int* d_rowptr, const int* d_colidx, double* d_f,
                        hipStream_t s) {

@@ -133,3 +133,23 @@ def test_force_steric_one_shot(sdt, orc):
     assert np.linalg.norm(f - ref) / np.linalg.norm(ref) < 1e-12
     with pytest.raises(sdt.SdtError):
         sdt.force_steric(pos, rdi, 1000.0, 2.3, 0.0)
+
+
+@pytest.mark.parametrize("n,phi,r0,ncells", [(20_000, 0.3, 2.4, "many"), (60, 0.05, 6.0, "fallback")])
+def test_steric_contact_cells_match_oracle(sdt, orc, n, phi, r0, ncells):
+    # a dense random-sequential-addition system with contacts inside r0
+    # (cells of the contact distance, 27 neighbours each), and a box under
+    # three contact distances wide, where the force falls back to the pair list
+    box = (4.0 * np.pi * n / (3.0 * phi)) ** (1.0 / 3.0)
+    assert (box / r0 >= 3) == (ncells == "many")
+    pos = sdt.rsa_positions(n, box, 1.0, 5)
+    pos = pos + np.random.default_rng(6).uniform(-0.05, 0.05, pos.size)
+    rdi = np.ones(n)
+    fo = sdt.StericForce(n, rdi, box, r0, 125.0)
+    assert fo.init()
+    f = fo.compute(pos)
+    ref = orc.steric(pos, rdi, box, r0, 125.0)
+    assert np.linalg.norm(ref) > 0
+    assert np.linalg.norm(f - ref) / np.linalg.norm(ref) < 1e-12
+    # repeat calls give identical bits
+    assert np.array_equal(fo.compute(pos), f)
