@@ -115,31 +115,42 @@ void MobSpmeGpu::update(const double* pos, const double* rdi, bool dev) {
     updated_ = true;
 }
 
-void MobSpmeGpu::apply_packed(int mode, int b, double alpha, const double* X, double beta,
-                              double* Y) {
-    SDT_REQUIRE(updated_, SDT_ESTATE, "mulvector called before update");
-    SDT_REQUIRE(b >= 1, SDT_EINVAL, "at least one right-hand side");
+template <typename T>
+void MobSpmeGpu::apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const double* X,
+                          double beta, double* Y) {
     hipStream_t s = stream_;
     const int* rp = pairs_.d_rowptr.get();
     const int* ci = pairs_.d_colidx.get();
+    // the column-blocked copy of X the real space gathers; the spread reads
+    // the same copy when its blocks are as wide
+    const int rc = real_block_cols<T>(b);
+    if (rc > 0) d_xblk_.reserve(ceil_div(real_scratch_bytes<T>(npos_, b), sizeof(double)));
+    const T* xb = reinterpret_cast<const T*>(d_xblk_.get());
+    bool copied = false;
     if (mode == 0 || mode == 2) {
-        if (prec_ == kFP32) spme32_->apply(b, alpha, X, beta, Y, s);
-        else spme64_->apply(b, alpha, X, beta, Y, s);
+        const bool share = rc > 0 && spme->blocked_cols(b) == rc;
+        if (share) {
+            StageScope scope(timer_, 0, s);
+            real_block_copy<T>(npos_, b, X, d_xblk_.get(), s);
+            copied = true;
+        }
+        spme->apply(b, alpha, X, beta, Y, s, share ? xb : nullptr);
     }
     if (mode == 0 || mode == 1) {
         StageScope scope(timer_, 5, s);
         const double rb = (mode == 0) ? 1.0 : beta;
-        if (prec_ == kFP32) {
-            d_xblk_.reserve(ceil_div(real_scratch_bytes<float>(npos_, b), sizeof(double)));
-            real_matvec<float>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha,
-                               X, rb, Y, d_xblk_.get(), s);
-        } else {
-            d_xblk_.reserve(ceil_div(real_scratch_bytes<double>(npos_, b), sizeof(double)));
-            real_matvec<double>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha,
-                                X, rb, Y, d_xblk_.get(), s);
-        }
+        real_matvec<T>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha, X, rb, Y,
+                       d_xblk_.get(), copied, s);
         real_overlap(ovl_, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha, X, Y, s);
     }
+}
+
+void MobSpmeGpu::apply_packed(int mode, int b, double alpha, const double* X, double beta,
+                              double* Y) {
+    SDT_REQUIRE(updated_, SDT_ESTATE, "mulvector called before update");
+    SDT_REQUIRE(b >= 1, SDT_EINVAL, "at least one right-hand side");
+    if (prec_ == kFP32) apply_as<float>(spme32_.get(), mode, b, alpha, X, beta, Y);
+    else apply_as<double>(spme64_.get(), mode, b, alpha, X, beta, Y);
 }
 
 void MobSpmeGpu::mul_vector(int mode, int nrhs, double alpha, int ldf, const double* f,

@@ -45,6 +45,9 @@ class MobSpmeGpu {
     StageTimer* timer() const { return timer_; }
 
   private:
+    template <typename T>
+    void apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const double* X, double beta,
+                  double* Y);
     int npos_, K_, porder_, prec_;
     double box_, xi_, rmax_;
     std::vector<double> rdi_;

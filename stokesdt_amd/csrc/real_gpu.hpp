@@ -22,13 +22,20 @@ void real_overlap(const RealOverlap& ov, const double* d_pos, const double* d_rd
                   double xi, int b, double alpha, const double* X, double* Y, hipStream_t s);
 
 // without the overlapping pairs (those are added by real_overlap), in
-// arithmetic T. For b >= 8 the kernel gathers a column-blocked copy of X (in
-// T) written to scratch, real_scratch_bytes<T>(npos, b) bytes (0 for b < 8).
+// arithmetic T. For b >= 8 the kernel gathers a column-blocked copy of X in
+// T, Xb[cb][j][c][r] with real_block_cols<T>(b) columns per block
+// (zero-padded), held in scratch (real_scratch_bytes<T>(npos, b) bytes, 0
+// for b < 8): written by real_block_copy, or by real_matvec itself unless
+// copied is set.
+template <typename T>
+int real_block_cols(int b);
 template <typename T>
 size_t real_scratch_bytes(int npos, int b);
 template <typename T>
+void real_block_copy(int npos, int b, const double* X, void* scratch, hipStream_t s);
+template <typename T>
 void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
                  const double* d_rdi, double box, double xi, int b, double alpha, const double* X,
-                 double beta, double* Y, void* scratch, hipStream_t s);
+                 double beta, double* Y, void* scratch, bool copied, hipStream_t s);
 
 }  // namespace sdt

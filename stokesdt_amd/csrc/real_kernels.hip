@@ -178,22 +178,9 @@ static void launch_real(int npos, const int* d_rowptr, const int* d_colidx, cons
     SDT_HIP_CHECK(hipGetLastError());
 }
 
-template <typename T, int RC, typename XT>
-static void blocked_real(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
-                         const double* d_rdi, double box, double xi, int b, double alpha,
-                         const double* X, double beta, double* Y, XT* xb, hipStream_t s) {
-    const int nblk = ceil_div(b, RC);
-    const long long n = (long long)nblk * npos * 3 * RC;
-    hipLaunchKernelGGL((block_copy_kernel<XT, RC>),
-                       dim3(std::min<long long>(ceil_div(n, 256), 8192)), dim3(256), 0, s, npos,
-                       b, nblk, X, xb);
-    launch_real<T, RC, XT, true>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, xb,
-                                 beta, Y, s);
-}
-
 // columns per block of the blocked copy (0: X is gathered in place)
 template <typename T>
-static int real_block_cols(int b) {
+int real_block_cols(int b) {
     if (b < 8) return 0;
     return (sizeof(T) == 4 && b >= 16) ? 16 : 8;
 }
@@ -206,18 +193,35 @@ size_t real_scratch_bytes(int npos, int b) {
 }
 
 template <typename T>
+void real_block_copy(int npos, int b, const double* X, void* scratch, hipStream_t s) {
+    const int rc = real_block_cols<T>(b);
+    SDT_REQUIRE(rc > 0 && scratch != nullptr, SDT_EINVAL, "real space: no blocked copy for b");
+    const int nblk = ceil_div(b, rc);
+    const long long n = (long long)nblk * npos * 3 * rc;
+    const dim3 grid(std::min<long long>(ceil_div(n, 256), 8192));
+    if (rc == 16)
+        hipLaunchKernelGGL((block_copy_kernel<T, 16>), grid, dim3(256), 0, s, npos, b, nblk, X,
+                           static_cast<T*>(scratch));
+    else
+        hipLaunchKernelGGL((block_copy_kernel<T, 8>), grid, dim3(256), 0, s, npos, b, nblk, X,
+                           static_cast<T*>(scratch));
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
 void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
                  const double* d_rdi, double box, double xi, int b, double alpha, const double* X,
-                 double beta, double* Y, void* scratch, hipStream_t s) {
+                 double beta, double* Y, void* scratch, bool copied, hipStream_t s) {
     const int rc = real_block_cols<T>(b);
     SDT_REQUIRE(rc == 0 || scratch != nullptr, SDT_EINVAL, "real space: scratch buffer missing");
+    if (rc > 0 && !copied) real_block_copy<T>(npos, b, X, scratch, s);
     if (rc == 16) {
         if constexpr (sizeof(T) == 4)
-            blocked_real<T, 16, T>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X,
-                                   beta, Y, static_cast<T*>(scratch), s);
+            launch_real<T, 16, T, true>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha,
+                                        static_cast<const T*>(scratch), beta, Y, s);
     } else if (rc == 8) {
-        blocked_real<T, 8, T>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X, beta,
-                              Y, static_cast<T*>(scratch), s);
+        launch_real<T, 8, T, true>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha,
+                                   static_cast<const T*>(scratch), beta, Y, s);
     } else if (b == 1) {
         launch_real<T, 1, double, false>(npos, d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha,
                                          X, beta, Y, s);
@@ -343,13 +347,17 @@ void real_overlap(const RealOverlap& ov, const double* d_pos, const double* d_rd
     SDT_HIP_CHECK(hipGetLastError());
 }
 
+template int real_block_cols<float>(int);
+template int real_block_cols<double>(int);
 template size_t real_scratch_bytes<float>(int, int);
 template size_t real_scratch_bytes<double>(int, int);
+template void real_block_copy<float>(int, int, const double*, void*, hipStream_t);
+template void real_block_copy<double>(int, int, const double*, void*, hipStream_t);
 template void real_matvec<float>(int, const int*, const int*, const double*, const double*, double,
-                                 double, int, double, const double*, double, double*, void*,
+                                 double, int, double, const double*, double, double*, void*, bool,
                                  hipStream_t);
 template void real_matvec<double>(int, const int*, const int*, const double*, const double*, double,
-                                  double, int, double, const double*, double, double*, void*,
+                                  double, int, double, const double*, double, double*, void*, bool,
                                   hipStream_t);
 
 }  // namespace sdt

@@ -40,12 +40,17 @@ struct SpmeGpu {
     // after the shared sort: gather per-particle data into sorted order
     void gather_sorted(const int* d_perm, const double* d_pos, const double* d_rdi,
                        const int* d_tile_start_in, hipStream_t s);
-    // Y = alpha * M_recip X + beta * Y, packed FP64 vectors, b RHS
+    // Y = alpha * M_recip X + beta * Y, packed FP64 vectors, b RHS; xb:
+    // optional column-blocked copy of X in T with blocked_cols(b) columns
+    // per block (Xb[cb][j][c][r], zero-padded), which the spread then reads
     void apply(int b, double alpha, const double* X, double beta, double* Y,
-               hipStream_t s);
+               hipStream_t s, const T* xb = nullptr);
+    // columns per block of the copy the spread can read for b (0: none)
+    int blocked_cols(int b) const;
+    bool point_owner() const;
 
     // stages (exposed for tests/benchmarks)
-    void spread(int b, const double* X, hipStream_t s);
+    void spread(int b, const double* X, hipStream_t s, const T* xb = nullptr);
     void fft_forward(int b, hipStream_t s);
     void influence(int b, hipStream_t s);
     void fft_backward(int b, hipStream_t s);

@@ -322,12 +322,18 @@ spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ 
 constexpr int kPtThreads = 512;
 constexpr int kPtMaxTiles = 200;   // contributor bucket tiles of one block (host-checked)
 
-template <typename T, int P, int NQ>
+//
+// BLK: the forces come from the column-blocked copy Xb[cb][j][c][r] of the
+// packed input (NQ / 3 columns per block, arithmetic type T, monodisperse):
+// a candidate's NQ values are one contiguous run, staged with vector loads,
+// and accumulator q holds component q / (NQ / 3) of column q % (NQ / 3).
+template <typename T, int P, int NQ, bool BLK>
 __global__ void __launch_bounds__(kPtThreads)
 spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile_start,
                  const T* __restrict__ frac, const int* __restrict__ base,
-                 const double* __restrict__ a2, const double* __restrict__ X, int b, int RB,
-                 int NC, int cap, T* __restrict__ mesh) {
+                 const double* __restrict__ a2, const double* __restrict__ X,
+                 const T* __restrict__ Xb, int npos, int b, int RB, int NC, int cap,
+                 T* __restrict__ mesh) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // record: wx[P] 0, wy[P] 0, wz[P] 0, bx by bz, pad, f[NQ] (16-byte aligned)
     constexpr int OWY = P + 1, OWZ = 2 * (P + 1), OB = 3 * (P + 1);
@@ -466,18 +472,62 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                 s_pid[sl] = is;
             }
             __syncthreads();
-            // forces of the new slots: slot q holds rhs r, component c (mono:
-            // q = 3 r + c; poly: q = 6 r + c for f, 6 r + 3 + c for a^2 f)
-            for (int idx = threadIdx.x; idx < taken * NQ; idx += kPtThreads) {
-                const int sl = first + idx / NQ, q = idx % NQ;
-                const int p = s_pid[sl];
-                const int r = q / NC, cc = q - r * NC;
-                T fv = T(0);
-                if (r < rb) {
-                    const double x = X[(size_t)(3 * p + cc % 3) * b + r0 + r];
-                    fv = (T)((cc >= 3) ? x * a2[p] : x);
+            if constexpr (BLK) {
+                // NQ contiguous values per slot, 16 bytes per load, the loads
+                // of a thread in flight together
+                constexpr int VW = 16 / sizeof(T);
+                constexpr int NV = NQ / VW;
+                constexpr int U = 4;
+                typedef T VT __attribute__((ext_vector_type(16 / sizeof(T))));
+                const VT* src = reinterpret_cast<const VT*>(Xb + (size_t)byl * npos * NQ);
+                for (int e0 = threadIdx.x; e0 < taken * NV; e0 += kPtThreads * U) {
+                    VT v[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int e = e0 + u * kPtThreads;
+                        v[u] = VT(0);
+                        if (e < taken * NV) {
+                            const int sl = first + e / NV;
+                            v[u] = src[(size_t)s_pid[sl] * NV + (e - (e / NV) * NV)];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int e = e0 + u * kPtThreads;
+                        if (e < taken * NV) {
+                            const int sl = first + e / NV, vv = e - (e / NV) * NV;
+                            *reinterpret_cast<VT*>(stage + (size_t)sl * REC + OF + vv * VW) = v[u];
+                        }
+                    }
                 }
-                stage[(size_t)sl * REC + OF + q] = fv;
+            } else {
+                // forces of the new slots: slot q holds rhs r, component c
+                // (mono: q = 3 r + c; poly: q = 6 r + c for f, 6 r + 3 + c
+                // for a^2 f); the loads of a thread in flight together
+                constexpr int U = 4;
+                for (int e0 = threadIdx.x; e0 < taken * NQ; e0 += kPtThreads * U) {
+                    T fv[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int idx = e0 + u * kPtThreads;
+                        fv[u] = T(0);
+                        if (idx < taken * NQ) {
+                            const int sl = first + idx / NQ, q = idx % NQ;
+                            const int p = s_pid[sl];
+                            const int r = q / NC, cc = q - r * NC;
+                            if (r < rb) {
+                                const double x = X[(size_t)(3 * p + cc % 3) * b + r0 + r];
+                                fv[u] = (T)((cc >= 3) ? x * a2[p] : x);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int idx = e0 + u * kPtThreads;
+                        if (idx < taken * NQ)
+                            stage[(size_t)(first + idx / NQ) * REC + OF + idx % NQ] = fv[u];
+                    }
+                }
             }
             nst += taken;
             done += taken;
@@ -489,9 +539,18 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     if (!wave_on) return;
     const size_t K3 = (size_t)K * K * K;
     const size_t pt = ((size_t)pz * K + py) * K + px;
+    if constexpr (BLK) {
+        constexpr int RBB = NQ / 3;
 #pragma unroll
-    for (int q = 0; q < NQ; q++)
-        if (q < nq) mesh[(size_t)(r0 * NC + q) * K3 + pt] = acc[q];
+        for (int q = 0; q < NQ; q++) {
+            const int c = q / RBB, r = q % RBB;
+            if (r < rb) mesh[((size_t)(r0 + r) * 3 + c) * K3 + pt] = acc[q];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            if (q < nq) mesh[(size_t)(r0 * NC + q) * K3 + pt] = acc[q];
+    }
 }
 
 // ------------------------------------------------------ x transforms -------
@@ -1113,32 +1172,52 @@ void SpmeGpu<T>::gather_sorted(const int* d_perm, const double* d_pos, const dou
 }
 
 template <typename T>
-void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s) {
-    SDT_REQUIRE(b <= bcap, SDT_ECAP, "spread: rhs count exceeds mesh capacity");
-    const int NC = poly ? 6 : 3;
+bool SpmeGpu<T>::point_owner() const {
     // contributor bucket tiles of a 32 x 4 x 4 block: bases in intervals of
     // length min(32, K) + P - 1 (x) and P + 3 (y, z)
     const int pt_tiles = std::min(ntile, (std::min(32, K) + porder - 1 + ts - 1) / ts + 1) *
                          std::min(ntile, (porder + 3 + ts - 1) / ts + 1) *
                          std::min(ntile, (porder + 3 + ts - 1) / ts + 1);
-    if (K % 4 == 0 && K >= 16 && pt_tiles <= kPtMaxTiles) {
-        // point-owner kernel: NQ meshes per workgroup, 12 when few are needed
+    return K % 4 == 0 && K >= 16 && pt_tiles <= kPtMaxTiles;
+}
+
+// point-owner meshes per workgroup: 12 when few are needed
+template <typename T>
+static int pt_meshes(int b, int NC) {
+    return (b * NC <= 12) ? 12 : (sizeof(T) == 4 ? 48 : 24);
+}
+
+template <typename T>
+int SpmeGpu<T>::blocked_cols(int b) const {
+    if (poly || !point_owner()) return 0;
+    const int NQ = pt_meshes<T>(b, 3);
+    return NQ == 12 ? 0 : NQ / 3;
+}
+
+template <typename T>
+void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
+    SDT_REQUIRE(b <= bcap, SDT_ECAP, "spread: rhs count exceeds mesh capacity");
+    const int NC = poly ? 6 : 3;
+    SDT_REQUIRE(xb == nullptr || blocked_cols(b) > 0, SDT_EINVAL,
+                "spread: no blocked input for this configuration");
+    if (point_owner()) {
         constexpr int NQL = sizeof(T) == 4 ? 48 : 24;
-        const int NQ = (b * NC <= 12) ? 12 : NQL;
+        const int NQ = pt_meshes<T>(b, NC);
         const int rec = ((3 * (porder + 1) + 3 + 3) & ~3) + NQ;
         const int cap = (int)(65536 / (rec * sizeof(T) + 9 * sizeof(int)));
         const size_t lds = (size_t)cap * (rec * sizeof(T) + 9 * sizeof(int));
         const int nsx = (K + 31) / 32;
         dim3 grid(nsx * (K / 4) * (K / 4), ceil_div(b, NQ / NC));
-#define SDT_SPREAD_PT(PP, QQ)                                                                  \
-    hipLaunchKernelGGL((spread_pt_kernel<T, PP, QQ>), grid, dim3(kPtThreads), lds, s, K, ts,     \
-                       ntile, nsx, d_tile_start.get(), d_frac.get(), d_base.get(),             \
-                       poly ? d_a2.get() : (double*)nullptr, X, b, NQ / NC, NC, cap,          \
+#define SDT_SPREAD_PT(PP, QQ, BB)                                                              \
+    hipLaunchKernelGGL((spread_pt_kernel<T, PP, QQ, BB>), grid, dim3(kPtThreads), lds, s, K,    \
+                       ts, ntile, nsx, d_tile_start.get(), d_frac.get(), d_base.get(),         \
+                       poly ? d_a2.get() : (double*)nullptr, X, xb, npos, b, NQ / NC, NC, cap, \
                        d_real.get())
 #define SDT_SPREAD_P(PP)                                                                       \
     case PP:                                                                                   \
-        if (NQ == 12) SDT_SPREAD_PT(PP, 12);                                                   \
-        else SDT_SPREAD_PT(PP, NQL);                                                           \
+        if (NQ == 12) SDT_SPREAD_PT(PP, 12, false);                                            \
+        else if (xb) SDT_SPREAD_PT(PP, NQL, true);                                             \
+        else SDT_SPREAD_PT(PP, NQL, false);                                                    \
         break;
         switch (porder) { SDT_SPREAD_P(4) SDT_SPREAD_P(5) SDT_SPREAD_P(6) SDT_SPREAD_P(7) SDT_SPREAD_P(8) }
 #undef SDT_SPREAD_P
@@ -1384,9 +1463,9 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
 
 template <typename T>
 void SpmeGpu<T>::apply(int b, double alpha, const double* X, double beta, double* Y,
-                       hipStream_t s) {
+                       hipStream_t s, const T* xb) {
     reserve_rhs(b);
-    { StageScope sc(timer, 0, s); spread(b, X, s); }
+    { StageScope sc(timer, 0, s); spread(b, X, s, xb); }
     { StageScope sc(timer, 1, s); fft_forward(b, s); }
     { StageScope sc(timer, 2, s); influence(b, s); }
     { StageScope sc(timer, 3, s); fft_backward(b, s); }

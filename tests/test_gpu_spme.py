@@ -217,3 +217,16 @@ def test_large_mesh_fft_paths(sdt, orc, K):
     assert g32.init()
     g32.update(pos)
     assert orc.frob_rel_err(got, g32.recip_mul_vector(f)) < 1e-5
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-5)])
+def test_blocked_input_paths_match_oracle(sdt, orc, prec, tol):
+    # 40 columns run as blocks of 16, 16 and 8: the spread and the real space
+    # share one column-blocked copy of the input where their block widths
+    # agree (FP32 16, FP64 8) and the spread reads the packed input otherwise
+    n, box = 1500, 28.0
+    g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, 48, 6, False, 31, prec)
+    f = np.random.default_rng(32).standard_normal((3 * n, 40))
+    for mode, fn in (("full", g.mul_vector), ("recip", g.recip_mul_vector)):
+        ref = o.mul_vector(f, mode=mode)
+        assert orc.frob_rel_err(ref, fn(f)) < tol, mode
