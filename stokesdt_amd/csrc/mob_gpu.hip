@@ -125,16 +125,12 @@ void MobSpmeGpu::apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const
     // the same copy when its blocks are as wide
     const int rc = real_block_cols<T>(b);
     if (rc > 0) d_xblk_.reserve(ceil_div(real_scratch_bytes<T>(npos_, b), sizeof(double)));
-    const T* xb = reinterpret_cast<const T*>(d_xblk_.get());
+    T* xb = reinterpret_cast<T*>(d_xblk_.get());
     bool copied = false;
     if (mode == 0 || mode == 2) {
         const bool share = rc > 0 && spme->blocked_cols(b) == rc;
-        if (share) {
-            StageScope scope(timer_, 0, s);
-            real_block_copy<T>(npos_, b, X, d_xblk_.get(), s);
-            copied = true;
-        }
-        spme->apply(b, alpha, X, beta, Y, s, share ? xb : nullptr);
+        spme->apply(b, alpha, X, beta, Y, s, share ? xb : nullptr, share);
+        copied = share;
     }
     if (mode == 0 || mode == 1) {
         StageScope scope(timer_, 5, s);

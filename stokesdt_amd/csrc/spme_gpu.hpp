@@ -43,8 +43,9 @@ struct SpmeGpu {
     // Y = alpha * M_recip X + beta * Y, packed FP64 vectors, b RHS; xb:
     // optional column-blocked copy of X in T with blocked_cols(b) columns
     // per block (Xb[cb][j][c][r], zero-padded), which the spread then reads
+    // (written first, within the spread stage, when fill_xb is set)
     void apply(int b, double alpha, const double* X, double beta, double* Y,
-               hipStream_t s, const T* xb = nullptr);
+               hipStream_t s, T* xb = nullptr, bool fill_xb = false);
     // columns per block of the copy the spread can read for b (0: none)
     int blocked_cols(int b) const;
     bool point_owner() const;

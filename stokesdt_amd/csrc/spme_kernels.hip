@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include "fft_lds.hpp"
+#include "real_gpu.hpp"
 #include "spme_gpu.hpp"
 
 namespace sdt {
@@ -1463,9 +1464,13 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
 
 template <typename T>
 void SpmeGpu<T>::apply(int b, double alpha, const double* X, double beta, double* Y,
-                       hipStream_t s, const T* xb) {
+                       hipStream_t s, T* xb, bool fill_xb) {
     reserve_rhs(b);
-    { StageScope sc(timer, 0, s); spread(b, X, s, xb); }
+    {
+        StageScope sc(timer, 0, s);
+        if (fill_xb) real_block_copy<T>(npos, b, X, xb, s);
+        spread(b, X, s, xb);
+    }
     { StageScope sc(timer, 1, s); fft_forward(b, s); }
     { StageScope sc(timer, 2, s); influence(b, s); }
     { StageScope sc(timer, 3, s); fft_backward(b, s); }
