@@ -907,21 +907,29 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
 }
 
 // ------------------------------------------------------------- interp ------
-// A workgroup takes the particles whose base points lie in one super-tile
-// (4 ts x ts x ts tiles along x) for RB right-hand sides. For every
-// (rhs, component) it stages the mesh region their stencils cover,
-// (4 ts + P - 1) x (ts + P - 1) x (ts + P - 1) points, in LDS; items
-// (particle, z line) sum their P x P stencil slice from LDS, and each
-// particle adds its P slices in fixed order (src/mob/spme.h:219-265 without
-// the stored P/ind tables). Y = beta Y + alpha (sum_U + a^2 sum_W).
+// A workgroup takes the particles whose base points lie in a strip of sxt
+// tiles along x (ts x ts x ts each) for RB right-hand sides, i.e. the
+// consecutive meshes r0 NC .. (r0 + rb) NC - 1. The meshes go through LDS MV
+// at a time (one MV-vector per mesh point): each pass
+// stages the region their stencils cover, (sxt ts + P - 1) x (ts + P - 1) x
+// (ts + P - 1) points, while the next pass's region is loaded into
+// registers. Items (particle, z line) sum their P x P stencil slice of all MV
+// meshes with vector LDS reads, and each particle adds its P slices in fixed
+// order (src/mob/spme.h:219-265 without the stored P/ind tables).
+// Y = beta Y + alpha (sum_U + a^2 sum_W).
+template <typename T, int MV>
+struct InterpVec {
+    typedef T type __attribute__((ext_vector_type(MV)));
+};
 
-template <typename T, int P, int RREG>
+template <typename T, int P, int RREG, int MV>
 __global__ void __launch_bounds__(256)
 interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
                    const int* __restrict__ tile_start, const T* __restrict__ frac,
                    const int* __restrict__ base, const double* __restrict__ a2,
                    const T* __restrict__ mesh, int b, int RB, int NC, double alpha, double beta,
                    double* __restrict__ Y) {
+    typedef typename InterpVec<T, MV>::type VT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const unsigned lid = xcd_logical_id(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
     const int bxl = (int)(lid % gridDim.x), byl = (int)(lid / gridDim.x);
@@ -930,50 +938,57 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
     const int ty = tyz % ntile, tz = tyz / ntile;
     const int r0 = byl * RB;
     const int rb = min(RB, b - r0);
+    const int nmesh = rb * NC;
     const int stx = sxi * sxt;
     const int nxt = min(sxt, ntile - stx);
     const int sx0 = stx * ts, sy0 = ty * ts, sz0 = tz * ts;
     const int RX = nxt * ts + P - 1, RY = ts + P - 1, RZ = ts + P - 1;
     const int nreg = RX * RY * RZ;
-    // LDS: region [RZ][RY][RX] | weights [CH][3][P] | bases [CH][3] |
-    //      partial sums [CH][P] | results [CH][RB][3] (double)
-    T* region = reinterpret_cast<T*>(smem_raw);
-    T* s_w = region + (size_t)(sxt * ts + P - 1) * RY * RZ;
+    // LDS: region VT[RZ][RY][RX] | partial sums VT[ch][P] | weights T[ch][3][P] |
+    //      bases int[ch][3] | results double[ch][RB][3]
+    VT* region = reinterpret_cast<VT*>(smem_raw);
+    VT* s_part = region + (size_t)(sxt * ts + P - 1) * RY * RZ;
+    T* s_w = reinterpret_cast<T*>(s_part + (size_t)ch * P);
     int* s_b = reinterpret_cast<int*>(s_w + ch * 3 * P);
-    T* s_part = reinterpret_cast<T*>(s_b + ch * 3);
     double* s_res = reinterpret_cast<double*>(
-        (reinterpret_cast<uintptr_t>(s_part + ch * P) + 7) & ~(uintptr_t)7);
+        (reinterpret_cast<uintptr_t>(s_b + ch * 3) + 7) & ~(uintptr_t)7);
     const size_t K3 = (size_t)K * K * K;
-    // offsets within one mesh of this thread's region elements (the same for
-    // every mesh): element u is region index threadIdx.x + 256 u
-    int off[RREG];
+    // byte offsets within one mesh of this thread's region points (the same
+    // for every mesh; 32-bit, added to the wave-uniform mesh base): point u is
+    // region index threadIdx.x + 256 u
+    unsigned offb[RREG];
     {
         const unsigned rx_m = 0xffffffffu / (unsigned)RX + 1u;
         const unsigned ry_m = 0xffffffffu / (unsigned)RY + 1u;
 #pragma unroll
         for (int u = 0; u < RREG; u++) {
-            const int i = (int)threadIdx.x + 256 * u;
-            off[u] = -1;
-            if (i < nreg) {
-                const int row = (int)fdiv(i, rx_m), lx = i - row * RX;
-                const int lz = (int)fdiv(row, ry_m), ly = row - lz * RY;
-                off[u] = (wrapk(sz0 + lz, K) * K + wrapk(sy0 + ly, K)) * K + wrapk(sx0 + lx, K);
-            }
+            const int i = min((int)threadIdx.x + 256 * u, nreg - 1);
+            const int row = (int)fdiv(i, rx_m), lx = i - row * RX;
+            const int lz = (int)fdiv(row, ry_m), ly = row - lz * RY;
+            offb[u] = (unsigned)((wrapk(sz0 + lz, K) * K + wrapk(sy0 + ly, K)) * K +
+                                 wrapk(sx0 + lx, K)) * (unsigned)sizeof(T);
         }
     }
-    // the region of the next mesh is loaded into registers while the current
-    // one is used (software pipelining over the rb * NC meshes)
-    T v[RREG];
+    const unsigned char* mbase =
+        reinterpret_cast<const unsigned char*>(mesh + (size_t)r0 * NC * K3);
+    VT v[RREG];
     auto prefetch = [&](int pi) __attribute__((always_inline)) {
-        const int r = pi / NC, q = pi - r * NC;
-        const T* m = mesh + (size_t)((r0 + r) * NC + q) * K3;
 #pragma unroll
-        for (int u = 0; u < RREG; u++) v[u] = (off[u] >= 0) ? m[off[u]] : T(0);
+        for (int j = 0; j < MV; j++) {
+            const int q = pi * MV + j;
+            const unsigned char* m = mbase + (size_t)min(q, nmesh - 1) * K3 * sizeof(T);
+            const bool on = q < nmesh;
+#pragma unroll
+            for (int u = 0; u < RREG; u++) {
+                const T x = *reinterpret_cast<const T*>(m + offb[u]);
+                v[u][j] = on ? x : T(0);
+            }
+        }
     };
     // own particles: base tiles stx .. stx+nxt-1 of row (ty, tz) are contiguous
     const int t_first = (tz * ntile + ty) * ntile + stx;
     const int pb = tile_start[t_first], pe = tile_start[t_first + nxt];
-    const int npass = rb * NC;
+    const int npass = (nmesh + MV - 1) / MV;
     for (int c0 = pb; c0 < pe; c0 += ch) {
         const int cnt = min(ch, pe - c0);
         prefetch(0);
@@ -993,7 +1008,6 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
             for (int q = 0; q < rb * 3; q++) s_res[t * RB * 3 + q] = 0.0;
         }
         for (int pi = 0; pi < npass; pi++) {
-            const int r = pi / NC, q = pi - r * NC;
             __syncthreads();          // the previous pass no longer reads the region
 #pragma unroll
             for (int u = 0; u < RREG; u++) {
@@ -1006,12 +1020,13 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
                 const int t = item / P, iz = item - t * P;
                 const T* wx = s_w + (t * 3 + 0) * P;
                 const T* wy = s_w + (t * 3 + 1) * P;
-                const T* rp = region + ((size_t)(s_b[t * 3 + 2] + iz) * RY + s_b[t * 3 + 1]) * RX +
-                              s_b[t * 3 + 0];
-                T acc = T(0);
-#pragma unroll
+                const VT* rp = region + ((size_t)(s_b[t * 3 + 2] + iz) * RY + s_b[t * 3 + 1]) * RX +
+                               s_b[t * 3 + 0];
+                VT acc = VT(0);
+                // one stencil row at a time keeps the live vector reads to P
+#pragma unroll 1
                 for (int iy = 0; iy < P; iy++) {
-                    T row = T(0);
+                    VT row = VT(0);
 #pragma unroll
                     for (int ix = 0; ix < P; ix++) row += wx[ix] * rp[iy * RX + ix];
                     acc += wy[iy] * row;
@@ -1020,12 +1035,19 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
             }
             __syncthreads();
             for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-                T sum = T(0);
+                VT sum = VT(0);
 #pragma unroll
                 for (int iz = 0; iz < P; iz++) sum += s_part[t * P + iz];
-                double vv = (double)sum;
-                if (q >= 3) vv *= a2[c0 + t];
-                s_res[t * RB * 3 + r * 3 + (q % 3)] += vv;
+#pragma unroll
+                for (int j = 0; j < MV; j++) {
+                    const int q = pi * MV + j;
+                    if (q < nmesh) {
+                        const int r = q / NC, c = q - r * NC;
+                        double vv = (double)sum[j];
+                        if (c >= 3) vv *= a2[c0 + t];
+                        s_res[t * RB * 3 + r * 3 + (c % 3)] += vv;
+                    }
+                }
             }
         }
         __syncthreads();
@@ -1426,38 +1448,47 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
     const double a_eff = alpha / (box * box * box);
     const int RB = 4;
     const int R = porder - 1;
-    // super-tile width (tiles along x) and particle chunk: fit 64 KB LDS
-    int sxt = 4, ch = 128;
+    // meshes per LDS pass, strip width (tiles along x) and particle chunk
+    // (measured at K = 128, p = 6): FP32 one mesh per pass over 4-tile
+    // strips; FP64 two meshes (16-byte LDS vectors) over 2-tile strips
+    constexpr int MV = sizeof(T) == 8 ? 2 : 1;
+    int sxt = sizeof(T) == 8 ? 2 : 4, ch = sizeof(T) == 8 ? 64 : 128;
     auto lds_of = [&](int sx, int c) {
         const size_t nreg = (size_t)(sx * ts + R) * (ts + R) * (ts + R);
-        return nreg * sizeof(T) + (size_t)c * 3 * porder * sizeof(T) + (size_t)c * 3 * sizeof(int) +
-               (size_t)c * porder * sizeof(T) + 8 + (size_t)c * RB * 3 * sizeof(double);
+        return nreg * MV * sizeof(T) + (size_t)c * porder * MV * sizeof(T) +
+               (size_t)c * 3 * porder * sizeof(T) + (size_t)c * 3 * sizeof(int) + 8 +
+               (size_t)c * RB * 3 * sizeof(double);
     };
-    while (lds_of(sxt, ch) > 64 * 1024 && (sxt > 1 || ch > 32)) {
-        if (ch > 64) ch /= 2;
-        else if (sxt > 1) sxt /= 2;
+    while (lds_of(sxt, ch) > 80 * 1024 && (sxt > 1 || ch > 32)) {
+        if (sxt > 1) sxt /= 2;
         else ch /= 2;
     }
     const size_t lds = lds_of(sxt, ch);
-    SDT_REQUIRE(lds <= 64 * 1024, SDT_EINVAL, "interp: LDS region too large");
+    SDT_REQUIRE(lds <= 80 * 1024, SDT_EINVAL, "interp: LDS region too large");
     const int nsx = (ntile + sxt - 1) / sxt;
-    // region elements per thread (registers of the prefetch)
+    // region points per thread (registers of the prefetch)
     const int rreg = (int)ceil_div((size_t)(sxt * ts + R) * (ts + R) * (ts + R), 256);
-    SDT_REQUIRE(rreg <= 36, SDT_EINVAL, "interp: region too large for the prefetch registers");
+    SDT_REQUIRE(rreg <= 26, SDT_EINVAL, "interp: region too large for the prefetch registers");
     dim3 grid(nsx * ntile * ntile, ceil_div(b, RB));
 #define SDT_INTERP_R(PP, RR)                                                                  \
-    hipLaunchKernelGGL((interp_tile_kernel<T, PP, RR>), grid, dim3(256), lds, s, K, ts, ntile,    \
+    if constexpr (sizeof(T) == 8) SDT_INTERP_M(PP, RR, 2);                                    \
+    else SDT_INTERP_M(PP, RR, 1);
+#define SDT_INTERP_M(PP, RR, MM)                                                              \
+    do {                                                                                      \
+    allow_lds<T>((const void*)interp_tile_kernel<T, PP, RR, MM>, lds);                            \
+    hipLaunchKernelGGL((interp_tile_kernel<T, PP, RR, MM>), grid, dim3(256), lds, s, K, ts, ntile, \
                        nsx, sxt, ch, d_tile_start.get(), d_frac.get(), d_base.get(),              \
                        poly ? d_a2.get() : (double*)nullptr, d_real.get(), b, RB, NC, a_eff, beta, \
-                       Y)
+                       Y);                                                                \
+    } while (0)
 #define SDT_INTERP(PP)                                                                        \
     case PP:                                                                                  \
-        if (rreg <= 16) SDT_INTERP_R(PP, 16);                                                 \
-        else if (rreg <= 26) SDT_INTERP_R(PP, 26);                                            \
-        else SDT_INTERP_R(PP, 36);                                                            \
+        if (rreg <= 16) { SDT_INTERP_R(PP, 16); }                                             \
+        else { SDT_INTERP_R(PP, 26); }                                                        \
         break;
     switch (porder) { SDT_INTERP(4) SDT_INTERP(5) SDT_INTERP(6) SDT_INTERP(7) SDT_INTERP(8) }
 #undef SDT_INTERP_R
+#undef SDT_INTERP_M
 #undef SDT_INTERP
     SDT_HIP_CHECK(hipGetLastError());
 }
