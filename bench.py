@@ -243,17 +243,21 @@ def main():
     res["step_latency_ms_p99"] = round(float(np.percentile(lat, 99)), 4)
     res["lanczos_iters_fp32"] = r32["iters"]
     stage = {k: dict(total_ms=round(v[0], 4), launches=v[1]) for k, v in r32["prof"].items()}
+    for k, v in r32["prof"].items():
+        if v[2]:
+            stage[k]["columns"] = v[2]
     res["stage_device_ms"] = stage
 
-    # roofline of the dominant kernel (largest device time in the timed
+    # roofline of the dominant stage (largest device time in the timed
     # region). Its launches there: one single-column matvec per step and,
-    # when the window holds a mobility update, one fused matvec over the
-    # whole Lanczos batch per block iteration.
+    # when the window holds a mobility update, one fused matvec per block
+    # iteration over the columns of the groups still iterating (the timer
+    # records the columns of every launch; algorithmic and measured bytes are
+    # affine in the launch width).
     cand = ["spread", "fft_xy_fwd", "fft_z_influence", "fft_xy_inv", "interp", "real_space"]
     dom = max(cand, key=lambda k: r32["prof"][k][0])
-    tot_ms, launches = r32["prof"][dom]
+    tot_ms, launches, ncols = r32["prof"][dom]
     btot = lanczos_batch_cols()
-    n_lz = launches - args.steps
     nnz = 0
     if dom == "real_space":
         m = S.MobSpme(n, rdi, L, c["xi"], c["rmax"], c["K"], c["p"], S.FP32)
@@ -261,25 +265,26 @@ def main():
         m.update(pos)
         nnz = m.real_nnz()
         del m
-    if n_lz >= 0 and tot_ms > 0:
-        tot_bytes = (args.steps * alg_bytes(dom, 1, n, c["K"], 4, nnz=nnz) +
-                     n_lz * alg_bytes(dom, btot, n, c["K"], 4, nnz=nnz))
+    if launches > 0 and tot_ms > 0:
+        a0 = alg_bytes(dom, 0, n, c["K"], 4, nnz=nnz)
+        a1 = alg_bytes(dom, 1, n, c["K"], 4, nnz=nnz) - a0
+        tot_bytes = launches * a0 + a1 * ncols
         achieved = tot_bytes / (tot_ms * 1e-3) / 1e9
         traffic = None
         tr = read_traffic(dom)
         if 1 in tr and btot in tr:
-            traffic = round((args.steps * tr[1] + n_lz * tr[btot]) / launches)
+            per_col = (tr[btot] - tr[1]) / (btot - 1)
+            traffic = round((launches * (tr[1] - per_col) + per_col * ncols) / launches)
         res["roofline"] = dict(kernel=dom, bound="hbm", achieved=round(achieved, 2),
                                peak=HBM_PEAK_GBS, unit="GB/s",
                                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,
-                               launches=launches, launch_columns={"1": args.steps,
-                                                                  str(btot): n_lz},
+                               launches=launches, columns=ncols,
                                avg_launch_ms=round(tot_ms / launches, 5),
                                alg_bytes_per_launch=round(tot_bytes / launches))
     else:
         res["roofline"] = dict(kernel=dom, bound="hbm", achieved=None, peak=HBM_PEAK_GBS,
                                unit="GB/s", frac=None, traffic=None,
-                               note="unexpected launch count %d" % launches)
+                               note="no launches of %s recorded" % dom)
     O = None
     if not args.no_cpu or not args.no_err:
         try:

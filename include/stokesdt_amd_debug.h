@@ -49,6 +49,9 @@ int sdt_bench_bd_steps(sdt_bd_t bd, int nsteps, double* ms_total);
  * clears the sums; sdt_bd_profile returns sums (ms) and counts. */
 int sdt_bd_set_profiling(sdt_bd_t bd, int on);
 int sdt_bd_profile(sdt_bd_t bd, double* sum_ms, int64_t* counts, int nstages);
+/* right-hand sides processed per stage over the recorded launches (stages 0-5
+ * record their launch width; the others 0) */
+int sdt_bd_profile_columns(sdt_bd_t bd, int64_t* columns, int nstages);
 /* Random sequential addition of n non-overlapping spheres of radius a in a
  * periodic box of side L (seeded); returns SDT_ECAP if max_attempts ran out
  * (then *placed < n). Synthetic-input generator for benchmarks. */

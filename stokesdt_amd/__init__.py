@@ -100,6 +100,7 @@ SIGNATURES = {
     "sdt_bench_bd_steps": (_I, [_P, _I, ctypes.POINTER(_D)]),
     "sdt_bd_set_profiling": (_I, [_P, _I]),
     "sdt_bd_profile": (_I, [_P, _P, _P, _I]),
+    "sdt_bd_profile_columns": (_I, [_P, _P, _I]),
     "sdt_util_rsa_positions": (_I, [_I, _D, _D, ctypes.c_uint64, _I64, _P, _PI64]),
 }
 
@@ -482,11 +483,14 @@ class BDSim:
         _check(lib().sdt_bd_set_profiling(self.h, 1 if on else 0))
 
     def profile(self):
-        """{stage: (total ms, launches)} recorded since profiling was enabled."""
+        """{stage: (total ms, launches, right-hand sides over the launches)}
+        recorded since profiling was enabled."""
         ms = np.zeros(len(STAGES))
         cnt = np.zeros(len(STAGES), dtype=np.int64)
+        cols = np.zeros(len(STAGES), dtype=np.int64)
         _check(lib().sdt_bd_profile(self.h, _ptr(ms), _ptr(cnt), len(STAGES)))
-        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(STAGES)}
+        _check(lib().sdt_bd_profile_columns(self.h, _ptr(cols), len(STAGES)))
+        return {name: (float(ms[i]), int(cnt[i]), int(cols[i])) for i, name in enumerate(STAGES)}
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

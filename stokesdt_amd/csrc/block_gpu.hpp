@@ -29,6 +29,10 @@ double sumsq(long long n, const double* x, DevBuf<double>& part, hipStream_t s);
 // particle order) <-> packed layout (row-major 3N x b, sorted order)
 void pack(int npos, int b, const int* perm, int ldf, const double* f, double* X, hipStream_t s);
 void unpack(int npos, int b, const int* perm, int ldv, const double* X, double* v, hipStream_t s);
+// columns cols[0 .. nc) of a packed block (nrow rows, leading dimension ld)
+// to a compact nrow x nc block (to_compact), or back
+void column_subset(long long nrow, int ld, int nc, const int* d_cols, const double* src,
+                   double* dst, bool to_compact, hipStream_t s);
 void axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s);
 void fill(long long n, double v, double* p, hipStream_t s);
 

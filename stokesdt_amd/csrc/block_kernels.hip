@@ -320,6 +320,28 @@ __global__ void unpack_kernel(int npos, int b, const int* __restrict__ perm, int
     v[(size_t)r * ldv + 3 * (size_t)i + c] = X[gid];
 }
 
+// dst[row][k] = src[row][cols[k]] (to_packed) or dst[row][cols[k]] =
+// src[row][k] (back): moves a subset of columns between a packed block of
+// leading dimension ld and a compact one of nc columns
+__global__ void column_subset_kernel(long long nrow, int ld, int nc, const int* __restrict__ cols,
+                                     const double* __restrict__ src, double* __restrict__ dst,
+                                     int to_compact) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nrow * nc) return;
+    const long long row = gid / nc;
+    const int k = (int)(gid - row * nc);
+    if (to_compact) dst[gid] = src[row * ld + cols[k]];
+    else dst[row * ld + cols[k]] = src[gid];
+}
+
+void column_subset(long long nrow, int ld, int nc, const int* d_cols, const double* src,
+                   double* dst, bool to_compact, hipStream_t s) {
+    const long long n = nrow * nc;
+    hipLaunchKernelGGL(column_subset_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, nrow, ld, nc,
+                       d_cols, src, dst, to_compact ? 1 : 0);
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
 void pack(int npos, int b, const int* perm, int ldf, const double* f, double* X, hipStream_t s) {
     const long long n = (long long)npos * 3 * b;
     hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, npos, b, perm, ldf, f, X);

@@ -692,6 +692,14 @@ int sdt_bd_profile(sdt_bd_t bd, double* sum_ms, int64_t* counts, int nstages) {
     return SDT_OK;
 }
 
+int sdt_bd_profile_columns(sdt_bd_t bd, int64_t* columns, int nstages) {
+    SDT_CHECK_HANDLE(bd);
+    StageTimer& t = bd->bd->timer();
+    for (int i = 0; i < nstages && i < StageTimer::kStages; i++)
+        if (columns) columns[i] = t.cols[i];
+    return SDT_OK;
+}
+
 int sdt_util_rsa_positions(int n, double box, double a, uint64_t seed, int64_t max_attempts,
                            double* pos, int64_t* placed) {
     if (n <= 0 || box <= 0.0 || a <= 0.0 || !pos) { set_error("invalid arguments"); return SDT_EINVAL; }

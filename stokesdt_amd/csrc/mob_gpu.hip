@@ -133,7 +133,7 @@ void MobSpmeGpu::apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const
         copied = share;
     }
     if (mode == 0 || mode == 1) {
-        StageScope scope(timer_, 5, s);
+        StageScope scope(timer_, 5, s, b);
         const double rb = (mode == 0) ? 1.0 : beta;
         real_matvec<T>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha, X, rb, Y,
                        d_xblk_.get(), copied, s);
@@ -402,8 +402,27 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
                 tsmm_groups1(n, btot, bs, V + (size_t)(iter - 1) * blk, d_S_.get(), 1.0, 0.0, W, s);
                 beta = -1.0;
             }
-            // W = M V_i - V_{i-1} B^T, one application for every column
-            mob->apply_packed(0, btot, 1.0, Vi, beta, W);
+            // W = M V_i - V_{i-1} B^T, one application for the columns of
+            // the groups still iterating (the columns of finished groups stay
+            // zero: their coefficient blocks are zero from here on)
+            std::vector<int> act;
+            for (const LzGroup& g : grp)
+                if (!g.done)
+                    for (int c = 0; c < g.b; c++) act.push_back(g.c0 + c);
+            if ((int)act.size() == btot || 4 * (int)act.size() > 3 * btot) {
+                mob->apply_packed(0, btot, 1.0, Vi, beta, W);
+            } else {
+                const int na = (int)act.size();
+                d_cols_.reserve(na);
+                d_Xa_.reserve((size_t)n * na);
+                d_Wa_.reserve((size_t)n * na);
+                SDT_HIP_CHECK(hipMemcpyAsync(d_cols_.get(), act.data(), sizeof(int) * na,
+                                             hipMemcpyHostToDevice, s));
+                column_subset(n, btot, na, d_cols_.get(), Vi, d_Xa_.get(), true, s);
+                if (beta != 0.0) column_subset(n, btot, na, d_cols_.get(), W, d_Wa_.get(), true, s);
+                mob->apply_packed(0, na, 1.0, d_Xa_.get(), beta, d_Wa_.get());
+                column_subset(n, btot, na, d_cols_.get(), d_Wa_.get(), W, false, s);
+            }
             // H = V_i^T W ; W -= V_i H
             { StageScope sc(timer, 6, s); gram_groups(n, btot, bs, Vi, W, d_G_.get(), d_part_, s); }
             SDT_HIP_CHECK(hipMemcpyAsync(hG.data(), d_G_.get(), sizeof(double) * hG.size(),

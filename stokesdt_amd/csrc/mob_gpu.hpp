@@ -95,6 +95,8 @@ class BrwnLanczosGpu {
     bool inited_ = false;
     std::vector<int> iters_, hint_;
     DevBuf<double> d_V_, d_G_, d_S_, d_part_, d_io_z_, d_io_y_, d_Zp_, d_Yp_;
+    DevBuf<double> d_Xa_, d_Wa_;   // active columns of a partly finished batch
+    DevBuf<int> d_cols_;
 };
 
 // Steric force (src/force/force_steric.cc) on the S pair list.

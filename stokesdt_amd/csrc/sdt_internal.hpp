@@ -107,7 +107,8 @@ struct StageTimer {
     bool on = false;
     double sum_ms[kStages] = {0};
     long long count[kStages] = {0};
-    struct Rec { int stage; hipEvent_t a, b; };
+    long long cols[kStages] = {0};   // right-hand sides over the launches
+    struct Rec { int stage; long long cols; hipEvent_t a, b; };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;
     hipEvent_t take() {
@@ -125,10 +126,10 @@ struct StageTimer {
         SDT_HIP_CHECK(hipEventRecord(e, s));
         return e;
     }
-    void end(int stage, hipEvent_t a, hipStream_t s) {
+    void end(int stage, hipEvent_t a, hipStream_t s, long long ncols) {
         hipEvent_t b = take();
         SDT_HIP_CHECK(hipEventRecord(b, s));
-        pending.push_back(Rec{stage, a, b});
+        pending.push_back(Rec{stage, ncols, a, b});
     }
     // call after the stream has been synchronised
     void collect() {
@@ -137,13 +138,14 @@ struct StageTimer {
             SDT_HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
             sum_ms[r.stage] += ms;
             count[r.stage] += 1;
+            cols[r.stage] += r.cols;
             pool.push_back(r.a);
             pool.push_back(r.b);
         }
         pending.clear();
     }
     void reset() {
-        for (int i = 0; i < kStages; i++) { sum_ms[i] = 0.0; count[i] = 0; }
+        for (int i = 0; i < kStages; i++) { sum_ms[i] = 0.0; count[i] = 0; cols[i] = 0; }
     }
     ~StageTimer() {
         for (const Rec& r : pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -156,12 +158,16 @@ struct StageScope {
     StageTimer* t;
     int stage;
     hipStream_t s;
+    long long ncols;
     hipEvent_t a = nullptr;
-    StageScope(StageTimer* t_, int stage_, hipStream_t s_) : t(t_), stage(stage_), s(s_) {
+    // ncols: right-hand sides the launch group processes (0 when not
+    // meaningful)
+    StageScope(StageTimer* t_, int stage_, hipStream_t s_, long long ncols_ = 0)
+        : t(t_), stage(stage_), s(s_), ncols(ncols_) {
         if (t && t->on) a = t->begin(s);
     }
     ~StageScope() noexcept(false) {
-        if (t && t->on && a) t->end(stage, a, s);
+        if (t && t->on && a) t->end(stage, a, s, ncols);
     }
 };
 

@@ -1498,14 +1498,14 @@ void SpmeGpu<T>::apply(int b, double alpha, const double* X, double beta, double
                        hipStream_t s, T* xb, bool fill_xb) {
     reserve_rhs(b);
     {
-        StageScope sc(timer, 0, s);
+        StageScope sc(timer, 0, s, b);
         if (fill_xb) real_block_copy<T>(npos, b, X, xb, s);
         spread(b, X, s, xb);
     }
-    { StageScope sc(timer, 1, s); fft_forward(b, s); }
-    { StageScope sc(timer, 2, s); influence(b, s); }
-    { StageScope sc(timer, 3, s); fft_backward(b, s); }
-    { StageScope sc(timer, 4, s); interp(b, alpha, beta, Y, s); }
+    { StageScope sc(timer, 1, s, b); fft_forward(b, s); }
+    { StageScope sc(timer, 2, s, b); influence(b, s); }
+    { StageScope sc(timer, 3, s, b); fft_backward(b, s); }
+    { StageScope sc(timer, 4, s, b); interp(b, alpha, beta, Y, s); }
 }
 
 template struct SpmeGpu<float>;
