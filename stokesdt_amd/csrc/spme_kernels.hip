@@ -328,7 +328,10 @@ constexpr int kPtMaxTiles = 200;   // contributor bucket tiles of one block (hos
 // packed input (NQ / 3 columns per block, arithmetic type T, monodisperse):
 // a candidate's NQ values are one contiguous run, staged with vector loads,
 // and accumulator q holds component q / (NQ / 3) of column q % (NQ / 3).
-template <typename T, int P, int NQ, bool BLK>
+//
+// WRAP: K < 32 + P - 1, a stencil can reach both ends of a block along x and
+// stencil offsets are taken modulo K.
+template <typename T, int P, int NQ, bool BLK, bool WRAP>
 __global__ void __launch_bounds__(kPtThreads)
 spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile_start,
                  const T* __restrict__ frac, const int* __restrict__ base,
@@ -336,7 +339,8 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                  const T* __restrict__ Xb, int npos, int b, int RB, int NC, int cap,
                  T* __restrict__ mesh) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    // record: wx[P] 0, wy[P] 0, wz[P] 0, bx by bz, pad, f[NQ] (16-byte aligned)
+    // record: wx[P] 0, wy[P] 0, wz[P] 0, base - block origin (3 ints in the
+    // slots of bx by bz), pad, f[NQ] (16-byte aligned)
     constexpr int OWY = P + 1, OWZ = 2 * (P + 1), OB = 3 * (P + 1);
     constexpr int OF = (OB + 3 + 3) & ~3;
     constexpr int REC = OF + NQ;
@@ -358,9 +362,10 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     const int r0 = byl * (NQ / NC);
     const int rb = min(NQ / NC, b - r0);
     const int nq = rb * NC;
-    const int sx = x0 + 4 * wave;                  // this wave's sub-block
     const bool wave_on = 4 * wave < xlen;
-    const int px = sx + (lane & 3), py = y0 + ((lane >> 2) & 3), pz = z0 + (lane >> 4);
+    // this lane's point, relative to the block origin
+    const int lx = 4 * wave + (lane & 3), ly = (lane >> 2) & 3, lz = lane >> 4;
+    const int px = x0 + lx, py = y0 + ly, pz = z0 + lz;
     T acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) acc[q] = T(0);
@@ -407,10 +412,10 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                 const int c = c0 + lane;
                 bool hit = false;
                 if (c < n) {
-                    const T* rec = stage + (size_t)c * REC;
-                    const int dx = wrapk((int)rec[OB] - sx, K);
-                    const int dy = wrapk((int)rec[OB + 1] - y0, K);
-                    const int dz = wrapk((int)rec[OB + 2] - z0, K);
+                    const int* rl = reinterpret_cast<const int*>(stage + (size_t)c * REC + OB);
+                    const int dx = wrapk(rl[0] - 4 * wave, K);
+                    const int dy = wrapk(rl[1], K);
+                    const int dz = wrapk(rl[2], K);
                     hit = (dx < 4 || K - dx < P) && (dy < 4 || K - dy < P) && (dz < 4 || K - dz < P);
                 }
                 const unsigned long long m = __ballot(hit);
@@ -421,14 +426,31 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
         // the list is written and read by this wave only (LDS accesses of a
         // wave complete in order)
         __builtin_amdgcn_wave_barrier();
-        for (int k = 0; k < cnt; k++) {
-            const T* rec = stage + (size_t)list[k] * REC;
-            const int dx = min(wrapk(px - (int)rec[OB], K), P);
-            const int dy = min(wrapk(py - (int)rec[OB + 1], K), P);
-            const int dz = min(wrapk(pz - (int)rec[OB + 2], K), P);
-            const T w = rec[dx] * rec[OWY + dy] * rec[OWZ + dz];
+        if constexpr (!WRAP) {
+            // no stencil reaches both ends of the block: the offset of this
+            // lane's point in a candidate's stencil is a plain difference
+            // (out of the stencil: P, a zero weight)
+            for (int k = 0; k < cnt; k++) {
+                const T* rec = stage + (size_t)list[k] * REC;
+                const int* rl = reinterpret_cast<const int*>(rec + OB);
+                const int dx = (int)min((unsigned)(lx - rl[0]), (unsigned)P);
+                const int dy = (int)min((unsigned)(ly - rl[1]), (unsigned)P);
+                const int dz = (int)min((unsigned)(lz - rl[2]), (unsigned)P);
+                const T w = rec[dx] * rec[OWY + dy] * rec[OWZ + dz];
 #pragma unroll
-            for (int q = 0; q < NQ; q++) acc[q] += w * rec[OF + q];
+                for (int q = 0; q < NQ; q++) acc[q] += w * rec[OF + q];
+            }
+        } else {
+            for (int k = 0; k < cnt; k++) {
+                const T* rec = stage + (size_t)list[k] * REC;
+                const int* rl = reinterpret_cast<const int*>(rec + OB);
+                const int dx = min(wrapk(lx - rl[0], K), P);
+                const int dy = min(wrapk(ly - rl[1], K), P);
+                const int dz = min(wrapk(lz - rl[2], K), P);
+                const T w = rec[dx] * rec[OWY + dy] * rec[OWZ + dz];
+#pragma unroll
+                for (int q = 0; q < NQ; q++) acc[q] += w * rec[OF + q];
+            }
         }
     };
 
@@ -462,13 +484,17 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                 const int sl = first + slot - done;
                 T* rec = stage + (size_t)sl * REC;
                 T w[P];
+                // base relative to the block origin, as an int in (-P, K - P]
+                int* rl = reinterpret_cast<int*>(rec + OB);
+                const int org[3] = {x0, y0, z0};
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
                     bspline_weights<T, P>(frac[3 * is + d], w);
 #pragma unroll
                     for (int k = 0; k < P; k++) rec[d * (P + 1) + k] = w[k];
                     rec[d * (P + 1) + P] = T(0);
-                    rec[OB + d] = (T)b3[d];
+                    const int r = wrapk(b3[d] - org[d], K);
+                    rl[d] = r > K - P ? r - K : r;
                 }
                 s_pid[sl] = is;
             }
@@ -1231,11 +1257,16 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
         const size_t lds = (size_t)cap * (rec * sizeof(T) + 9 * sizeof(int));
         const int nsx = (K + 31) / 32;
         dim3 grid(nsx * (K / 4) * (K / 4), ceil_div(b, NQ / NC));
-#define SDT_SPREAD_PT(PP, QQ, BB)                                                              \
-    hipLaunchKernelGGL((spread_pt_kernel<T, PP, QQ, BB>), grid, dim3(kPtThreads), lds, s, K,    \
+#define SDT_SPREAD_PW(PP, QQ, BB, WW)                                                          \
+    hipLaunchKernelGGL((spread_pt_kernel<T, PP, QQ, BB, WW>), grid, dim3(kPtThreads), lds, s, K, \
                        ts, ntile, nsx, d_tile_start.get(), d_frac.get(), d_base.get(),         \
                        poly ? d_a2.get() : (double*)nullptr, X, xb, npos, b, NQ / NC, NC, cap, \
                        d_real.get())
+#define SDT_SPREAD_PT(PP, QQ, BB)                                                              \
+    do {                                                                                      \
+        if (K < 32 + PP - 1) SDT_SPREAD_PW(PP, QQ, BB, true);                                  \
+        else SDT_SPREAD_PW(PP, QQ, BB, false);                                                 \
+    } while (0)
 #define SDT_SPREAD_P(PP)                                                                       \
     case PP:                                                                                   \
         if (NQ == 12) SDT_SPREAD_PT(PP, 12, false);                                            \
@@ -1245,6 +1276,7 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
         switch (porder) { SDT_SPREAD_P(4) SDT_SPREAD_P(5) SDT_SPREAD_P(6) SDT_SPREAD_P(7) SDT_SPREAD_P(8) }
 #undef SDT_SPREAD_P
 #undef SDT_SPREAD_PT
+#undef SDT_SPREAD_PW
         SDT_HIP_CHECK(hipGetLastError());
         return;
     }
