@@ -703,6 +703,34 @@ __device__ __forceinline__ void fft_any(cplx<T>* buf, int npen, int PS, const FF
     else fft_pencils_inplace<T, SIGN, MAXI2>(buf, npen, PS, fa, tw);
 }
 
+// Mesh side and divisions by K and K/2+1: compile-time when the two-stage
+// transform fixes K = R1 R2, magic reciprocals otherwise
+template <int R1, int R2>
+struct KDiv {
+    static constexpr int KC = R1 * R2;
+    int K, Kh;
+    unsigned K_m, Kh_m;
+    __device__ __forceinline__ explicit KDiv(const FFTArgs& fa) {
+        if constexpr (KC > 0) {
+            K = KC;
+            Kh = KC / 2 + 1;
+        } else {
+            K = fa.K;
+            Kh = fa.K / 2 + 1;
+        }
+        K_m = fa.K_m;
+        Kh_m = fa.Kh_m;
+    }
+    __device__ __forceinline__ int div_k(int i) const {
+        if constexpr (KC > 0) return (int)((unsigned)i / (unsigned)KC);
+        else return (int)fdiv((unsigned)i, K_m);
+    }
+    __device__ __forceinline__ int div_kh(int i) const {
+        if constexpr (KC > 0) return (int)((unsigned)i / (unsigned)(KC / 2 + 1));
+        else return (int)fdiv((unsigned)i, Kh_m);
+    }
+};
+
 // ------------------------------------------------ fused x + y plane -------
 // One workgroup per z-plane of one mesh: the K x K real plane is read once,
 // transformed along x (two real rows per complex FFT) and along y in LDS
@@ -717,7 +745,8 @@ __global__ void __launch_bounds__(kPlaneThreads)
 fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict__ spec,
                      const double* __restrict__ tw_g) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int K = fa.K, Kh = K / 2 + 1, PS = pencil_stride(K), H = K / 2;
+    const KDiv<R1, R2> kd(fa);
+    const int K = kd.K, Kh = kd.Kh, PS = pencil_stride(K), H = K / 2;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* buf = tw + K;
     load_twiddles(tw, tw_g, K);
@@ -725,7 +754,7 @@ fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict
     const T* src = real + plane * K * K;
     // row pairs (2j, 2j+1) -> pencil j
     for (int i = threadIdx.x; i < H * K; i += blockDim.x) {
-        const int j = (int)fdiv(i, fa.K_m), x = i - j * K;
+        const int j = kd.div_k(i), x = i - j * K;
         buf[j * PS + lpad(x)] = cplx<T>{src[(size_t)(2 * j) * K + x], src[(size_t)(2 * j + 1) * K + x]};
     }
     __syncthreads();
@@ -739,7 +768,7 @@ fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict
         for (int u = 0; u < MAXP; u++) {
             const int i = threadIdx.x + u * blockDim.x;
             if (i < nit) {
-                const int j = (int)fdiv(i, fa.Kh_m), k = i - j * Kh;
+                const int j = kd.div_kh(i), k = i - j * Kh;
                 const cplx<T> zk = buf[j * PS + lpad(k)];
                 const cplx<T> zm = buf[j * PS + lpad(k == 0 ? 0 : K - k)];
                 const T h = T(0.5);
@@ -752,7 +781,7 @@ fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict
         for (int u = 0; u < MAXP; u++) {
             const int i = threadIdx.x + u * blockDim.x;
             if (i < nit) {
-                const int j = (int)fdiv(i, fa.Kh_m), k = i - j * Kh;
+                const int j = kd.div_kh(i), k = i - j * Kh;
                 buf[k * PS + lpad(2 * j)] = va[u];
                 buf[k * PS + lpad(2 * j + 1)] = vb[u];
             }
@@ -762,7 +791,7 @@ fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict
     fft_any<T, -1, MAXI2, R1, R2>(buf, Kh, PS, fa, tw);
     cplx<T>* dst = spec + plane * K * Kh;
     for (int i = threadIdx.x; i < K * Kh; i += blockDim.x) {
-        const int y = (int)fdiv(i, fa.Kh_m), kx = i - y * Kh;
+        const int y = kd.div_kh(i), kx = i - y * Kh;
         dst[i] = buf[kx * PS + lpad(y)];
     }
 }
@@ -772,14 +801,15 @@ __global__ void __launch_bounds__(kPlaneThreads)
 fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict__ real,
                      const double* __restrict__ tw_g) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int K = fa.K, Kh = K / 2 + 1, PS = pencil_stride(K), H = K / 2;
+    const KDiv<R1, R2> kd(fa);
+    const int K = kd.K, Kh = kd.Kh, PS = pencil_stride(K), H = K / 2;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* buf = tw + K;
     load_twiddles(tw, tw_g, K);
     const size_t plane = blockIdx.x;
     const cplx<T>* src = spec + plane * K * Kh;
     for (int i = threadIdx.x; i < K * Kh; i += blockDim.x) {
-        const int y = (int)fdiv(i, fa.Kh_m), kx = i - y * Kh;
+        const int y = kd.div_kh(i), kx = i - y * Kh;
         buf[kx * PS + lpad(y)] = src[i];
     }
     __syncthreads();
@@ -792,7 +822,7 @@ fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict
         for (int u = 0; u < MAXP; u++) {
             const int i = threadIdx.x + u * blockDim.x;
             if (i < nit) {
-                const int j = (int)fdiv(i, fa.K_m), k = i - j * K;
+                const int j = kd.div_k(i), k = i - j * K;
                 const int kk = (k <= H) ? k : K - k;
                 cplx<T> A = buf[kk * PS + lpad(2 * j)];
                 cplx<T> B = buf[kk * PS + lpad(2 * j + 1)];
@@ -805,7 +835,7 @@ fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict
         for (int u = 0; u < MAXP; u++) {
             const int i = threadIdx.x + u * blockDim.x;
             if (i < nit) {
-                const int j = (int)fdiv(i, fa.K_m), k = i - j * K;
+                const int j = kd.div_k(i), k = i - j * K;
                 buf[j * PS + lpad(k)] = vz[u];
             }
         }
@@ -814,7 +844,7 @@ fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict
     fft_any<T, +1, MAXI2, R1, R2>(buf, H, PS, fa, tw);
     T* dst = real + plane * K * K;
     for (int i = threadIdx.x; i < H * K; i += blockDim.x) {
-        const int j = (int)fdiv(i, fa.K_m), x = i - j * K;
+        const int j = kd.div_k(i), x = i - j * K;
         const cplx<T> v = buf[j * PS + lpad(x)];
         dst[(size_t)(2 * j) * K + x] = v.x;
         dst[(size_t)(2 * j + 1) * K + x] = v.y;
@@ -840,7 +870,8 @@ __global__ void __launch_bounds__(kZThreads)
 fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ tw_g,
                        const double* __restrict__ bsp2, int lnp, int NC, double box, double xi) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int K = fa.K, Kh = K / 2 + 1, PS = pencil_stride(K);
+    const KDiv<R1, R2> kd(fa);
+    const int K = kd.K, Kh = kd.Kh, PS = pencil_stride(K);
     const int NP = 1 << lnp;
     const int nj = K * Kh;
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -855,7 +886,7 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
     for (int i = threadIdx.x; i < NC * NP * K; i += blockDim.x) {
         const int jl = i & (NP - 1);
         const int cz = i >> lnp;
-        const int c = (int)fdiv(cz, fa.K_m), z = cz - c * K;
+        const int c = kd.div_k(cz), z = cz - c * K;
         buf[(c * NP + jl) * PS + lpad(z)] =
             (jl < nvalid) ? base[c * mesh_sz + (size_t)z * nj + jl] : cplx<T>{T(0), T(0)};
     }
@@ -866,10 +897,10 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
     const T inv_xi2 = (T)(1.0 / (xi * xi));
     const int cs = NP * PS;    // component stride in LDS
     for (int i = threadIdx.x; i < NP * K; i += blockDim.x) {
-        const int jl = (int)fdiv(i, fa.K_m), z = i - jl * K;
+        const int jl = kd.div_k(i), z = i - jl * K;
         if (jl >= nvalid) continue;
         const int j = j0 + jl;
-        const int y = (int)fdiv(j, fa.Kh_m), kxi = j - y * Kh;
+        const int y = kd.div_kh(j), kxi = j - y * Kh;
         const int my = (y <= K / 2) ? y : y - K;
         const int mz = (z <= K / 2) ? z : z - K;
         const T kx = twopi_l * (T)kxi;
@@ -927,7 +958,7 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
     for (int i = threadIdx.x; i < NC * NP * K; i += blockDim.x) {
         const int jl = i & (NP - 1);
         const int cz = i >> lnp;
-        const int c = (int)fdiv(cz, fa.K_m), z = cz - c * K;
+        const int c = kd.div_k(cz), z = cz - c * K;
         if (jl < nvalid) base[c * mesh_sz + (size_t)z * nj + jl] = buf[(c * NP + jl) * PS + lpad(z)];
     }
 }
