@@ -752,10 +752,22 @@ fft_plane_fwd_kernel(FFTArgs fa, const T* __restrict__ real, cplx<T>* __restrict
     load_twiddles(tw, tw_g, K);
     const size_t plane = blockIdx.x;
     const T* src = real + plane * K * K;
-    // row pairs (2j, 2j+1) -> pencil j
-    for (int i = threadIdx.x; i < H * K; i += blockDim.x) {
-        const int j = kd.div_k(i), x = i - j * K;
-        buf[j * PS + lpad(x)] = cplx<T>{src[(size_t)(2 * j) * K + x], src[(size_t)(2 * j + 1) * K + x]};
+    // row pairs (2j, 2j+1) -> pencil j; the loads of a thread (8 at a time)
+    // are in flight together
+    for (int i0 = threadIdx.x; i0 < H * K; i0 += 8 * kPlaneThreads) {
+        cplx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int i = min(i0 + u * kPlaneThreads, H * K - 1);
+            const int j = kd.div_k(i), x = i - j * K;
+            v[u] = cplx<T>{src[(size_t)(2 * j) * K + x], src[(size_t)(2 * j + 1) * K + x]};
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int i = i0 + u * kPlaneThreads;
+            const int j = kd.div_k(i), x = i - j * K;
+            if (i < H * K) buf[j * PS + lpad(x)] = v[u];
+        }
     }
     __syncthreads();
     fft_any<T, -1, MAXI2, R1, R2>(buf, H, PS, fa, tw);
@@ -808,9 +820,17 @@ fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict
     load_twiddles(tw, tw_g, K);
     const size_t plane = blockIdx.x;
     const cplx<T>* src = spec + plane * K * Kh;
-    for (int i = threadIdx.x; i < K * Kh; i += blockDim.x) {
-        const int y = kd.div_kh(i), kx = i - y * Kh;
-        buf[kx * PS + lpad(y)] = src[i];
+    // the loads of a thread (8 at a time) are in flight together
+    for (int i0 = threadIdx.x; i0 < K * Kh; i0 += 8 * kPlaneThreads) {
+        cplx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = src[min(i0 + u * kPlaneThreads, K * Kh - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int i = i0 + u * kPlaneThreads;
+            const int y = kd.div_kh(i), kx = i - y * Kh;
+            if (i < K * Kh) buf[kx * PS + lpad(y)] = v[u];
+        }
     }
     __syncthreads();
     fft_any<T, +1, MAXI2, R1, R2>(buf, Kh, PS, fa, tw);
@@ -877,18 +897,36 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
     cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* buf = tw + K;
     load_twiddles(tw, tw_g, K);
+    // |b|^2 factors in arithmetic T, after the spectrum buffer
+    T* s_b2 = reinterpret_cast<T*>(buf + (size_t)NC * NP * PS);
+    for (int i = threadIdx.x; i < K; i += blockDim.x) s_b2[i] = (T)bsp2[i];
     const int nch = (nj + NP - 1) >> lnp;
     const int j0 = (int)(blockIdx.x % nch) * NP;
     const int rhs = (int)(blockIdx.x / nch);
     const size_t mesh_sz = (size_t)nj * K;
     cplx<T>* base = spec + (size_t)rhs * NC * mesh_sz + j0;
     const int nvalid = min(NP, nj - j0);
-    for (int i = threadIdx.x; i < NC * NP * K; i += blockDim.x) {
-        const int jl = i & (NP - 1);
-        const int cz = i >> lnp;
-        const int c = kd.div_k(cz), z = cz - c * K;
-        buf[(c * NP + jl) * PS + lpad(z)] =
-            (jl < nvalid) ? base[c * mesh_sz + (size_t)z * nj + jl] : cplx<T>{T(0), T(0)};
+    // the loads of a thread (ZB at a time) are in flight together
+    constexpr int ZB = 8;
+    for (int i0 = threadIdx.x; i0 < NC * NP * K; i0 += ZB * kZThreads) {
+        cplx<T> v[ZB];
+#pragma unroll
+        for (int u = 0; u < ZB; u++) {
+            const int i = i0 + u * kZThreads;
+            const int jl = i & (NP - 1);
+            const int cz = i >> lnp;
+            const int c = kd.div_k(cz), z = cz - c * K;
+            v[u] = (i < NC * NP * K && jl < nvalid) ? base[c * mesh_sz + (size_t)z * nj + jl]
+                                                    : cplx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < ZB; u++) {
+            const int i = i0 + u * kZThreads;
+            const int jl = i & (NP - 1);
+            const int cz = i >> lnp;
+            const int c = kd.div_k(cz), z = cz - c * K;
+            if (i < NC * NP * K) buf[(c * NP + jl) * PS + lpad(z)] = v[u];
+        }
     }
     __syncthreads();
     fft_any<T, -1, kZMaxI2, R1, R2>(buf, NC * NP, PS, fa, tw);
@@ -912,7 +950,7 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
             const T kx2 = kk * inv_xi2;
             m2 = T(6.0 * M_PI) * exp(-kx2 * T(0.25)) / kk *
                  (T(1) + kx2 * (T(0.25) + kx2 * T(0.125)));
-            m2 *= (T)(bsp2[kxi] * bsp2[y] * bsp2[z]);
+            m2 *= s_b2[kxi] * s_b2[y] * s_b2[z];
             const T rk = T(1) / sqrt(kk);
             ex = kx * rk; ey = ky * rk; ez = kz * rk;
         }
@@ -1438,7 +1476,8 @@ void SpmeGpu<T>::influence(int b, hipStream_t s) {
     const size_t budget = std::min<size_t>(max_lds_bytes(), 112 * 1024);
     int lnp = 4;
     auto lds_of = [&](int l) {
-        return (K + (size_t)NC * ((size_t)1 << l) * pencil_stride(K)) * sizeof(cplx<T>);
+        return (K + (size_t)NC * ((size_t)1 << l) * pencil_stride(K)) * sizeof(cplx<T>) +
+               (size_t)K * sizeof(T);
     };
     while (lnp > 0 && (lds_of(lnp) > budget || NC * (K << lnp) > 2 * kZMaxI2 * kZThreads)) lnp--;
     SDT_REQUIRE(lds_of(lnp) <= max_lds_bytes() && NC * K <= 2 * kZMaxI2 * kZThreads, SDT_EINVAL,
