@@ -25,7 +25,7 @@ struct SpmeGpu {
     FFTPlan plan;
     DevBuf<double> d_tw;   // K roots of unity (cos, sin) of -2*pi*j/K
     DevBuf<double> d_bsp2; // |b(m)|^2 per wavenumber index, K entries
-    DevBuf<T> d_frac;      // 3N fractional offsets (sorted order)
+    DevBuf<double> d_frac; // 3N fractional offsets (sorted order)
     DevBuf<int> d_base;    // 3N B-spline base mesh indices (sorted order)
     DevBuf<double> d_a2;   // N radii squared (sorted order), poly only
     DevBuf<int> d_tile_start;  // ntile^3 + 1
@@ -60,7 +60,7 @@ struct SpmeGpu {
     StageTimer* timer = nullptr;
 
     // scratch from prepare()
-    DevBuf<T> d_frac_u;
+    DevBuf<double> d_frac_u;
     DevBuf<int> d_base_u;
 };
 

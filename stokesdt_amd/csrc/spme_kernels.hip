@@ -37,12 +37,22 @@ __device__ __forceinline__ void bspline_weights(T frac, T* w) {
     }
 }
 
+// the weights from the FP64 fractional offset, evaluated in FP64 and rounded
+// to the arithmetic type once (FP32 weights carry no error from the offset)
+template <typename T, int P>
+__device__ __forceinline__ void bspline_weights_r(double frac, T* w) {
+    double wd[P];
+    bspline_weights<double, P>(frac, wd);
+#pragma unroll
+    for (int k = 0; k < P; k++) w[k] = (T)wd[k];
+}
+
 // ------------------------------------------------------------ prepare ------
 // Wrap into [0, L) (pair-list convention, src/pairlist/pair_base.cc:94-99),
 // scaled coordinate c = x / L * K, base index as src/mob/spme.cc:419-443.
 template <typename T, int P>
 __global__ void spme_prepare_kernel(int npos, const double* __restrict__ pos, double box, int K,
-                                    int ts, int ntile, T* __restrict__ frac_u,
+                                    int ts, int ntile, double* __restrict__ frac_u,
                                     int* __restrict__ base_u, int* __restrict__ key) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npos) return;
@@ -55,7 +65,7 @@ __global__ void spme_prepare_kernel(int npos, const double* __restrict__ pos, do
         double g = floor(c);
         int b0 = (P % 2 == 0) ? (int)(g - P / 2 + 1) : (int)(g - (P - 1) / 2);
         int b = ((b0 % K) + K) % K;
-        frac_u[3 * i + d] = (T)(c - g);
+        frac_u[3 * i + d] = c - g;
         base_u[3 * i + d] = b;
         int tt = b / ts;
         t[d] = tt < ntile ? tt : ntile - 1;
@@ -65,8 +75,8 @@ __global__ void spme_prepare_kernel(int npos, const double* __restrict__ pos, do
 
 template <typename T>
 __global__ void spme_gather_kernel(int npos, const int* __restrict__ perm,
-                                   const T* __restrict__ frac_u, const int* __restrict__ base_u,
-                                   const double* __restrict__ rdi, T* __restrict__ frac,
+                                   const double* __restrict__ frac_u, const int* __restrict__ base_u,
+                                   const double* __restrict__ rdi, double* __restrict__ frac,
                                    int* __restrict__ base, double* __restrict__ a2) {
     const int is = blockIdx.x * blockDim.x + threadIdx.x;
     if (is >= npos) return;
@@ -116,7 +126,7 @@ __device__ __forceinline__ int wrapk(int d, int K) {
 template <typename T, int P, int RBNC>
 __global__ void __launch_bounds__(256)
 spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile_start,
-                     const T* __restrict__ frac, const int* __restrict__ base,
+                     const double* __restrict__ frac, const int* __restrict__ base,
                      const double* __restrict__ a2, const double* __restrict__ X, int b, int RB,
                      int NC, int cap, T* __restrict__ mesh) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -275,17 +285,17 @@ spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ 
                         rec[OF + q] = fv;
                     }
                     T w[P];
-                    bspline_weights<T, P>(frac[3 * is], w);
+                    bspline_weights_r<T, P>(frac[3 * is], w);
 #pragma unroll
                     for (int k = 0; k < 7; k++) rec[k] = T(0);
 #pragma unroll
                     for (int k = 0; k < P; k++) rec[7 + k] = w[k];
 #pragma unroll
                     for (int k = 7 + P; k < P + 14; k++) rec[k] = T(0);
-                    bspline_weights<T, P>(frac[3 * is + 1], w);
+                    bspline_weights_r<T, P>(frac[3 * is + 1], w);
 #pragma unroll
                     for (int k = 0; k < P; k++) rec[OWY + k] = w[k];
-                    bspline_weights<T, P>(frac[3 * is + 2], w);
+                    bspline_weights_r<T, P>(frac[3 * is + 2], w);
 #pragma unroll
                     for (int k = 0; k < P; k++) rec[OWZ + k] = w[k];
 #pragma unroll
@@ -334,7 +344,7 @@ constexpr int kPtMaxTiles = 200;   // contributor bucket tiles of one block (hos
 template <typename T, int P, int NQ, bool BLK, bool WRAP>
 __global__ void __launch_bounds__(kPtThreads)
 spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile_start,
-                 const T* __restrict__ frac, const int* __restrict__ base,
+                 const double* __restrict__ frac, const int* __restrict__ base,
                  const double* __restrict__ a2, const double* __restrict__ X,
                  const T* __restrict__ Xb, int npos, int b, int RB, int NC, int cap,
                  T* __restrict__ mesh) {
@@ -489,7 +499,7 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                 const int org[3] = {x0, y0, z0};
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    bspline_weights<T, P>(frac[3 * is + d], w);
+                    bspline_weights_r<T, P>(frac[3 * is + d], w);
 #pragma unroll
                     for (int k = 0; k < P; k++) rec[d * (P + 1) + k] = w[k];
                     rec[d * (P + 1) + P] = T(0);
@@ -1020,7 +1030,7 @@ struct InterpVec {
 template <typename T, int P, int RREG, int MV>
 __global__ void __launch_bounds__(256)
 interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
-                   const int* __restrict__ tile_start, const T* __restrict__ frac,
+                   const int* __restrict__ tile_start, const double* __restrict__ frac,
                    const int* __restrict__ base, const double* __restrict__ a2,
                    const T* __restrict__ mesh, int b, int RB, int NC, double alpha, double beta,
                    double* __restrict__ Y) {
@@ -1092,7 +1102,7 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
             T w[P];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                bspline_weights<T, P>(frac[3 * is + d], w);
+                bspline_weights_r<T, P>(frac[3 * is + d], w);
 #pragma unroll
                 for (int k = 0; k < P; k++) s_w[(t * 3 + d) * P + k] = w[k];
             }
