@@ -322,6 +322,11 @@ spread_gather_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ 
     }
 }
 
+// logical column block of a workgroup (the y part of its XCD-remapped id)
+__device__ __forceinline__ unsigned byl_of(unsigned linear, unsigned n, unsigned gx) {
+    return xcd_logical_id(linear, n) / gx;
+}
+
 // Point-owner form (K % 4 == 0): a workgroup of 8 waves owns a 32 x 4 x 4
 // block of mesh points (wave w: x in [4w, 4w+4)), lane l one point
 // (x = l % 4, y = (l / 4) % 4, z = l / 16), and NQ meshes (NQ / NC
@@ -341,7 +346,11 @@ constexpr int kPtMaxTiles = 200;   // contributor bucket tiles of one block (hos
 //
 // WRAP: K < 32 + P - 1, a stencil can reach both ends of a block along x and
 // stencil offsets are taken modulo K.
-template <typename T, int P, int NQ, bool BLK, bool WRAP>
+// SGF (with BLK): the forces are not staged; every wave reads a candidate's
+// NQ values at a wave-uniform address, through the scalar data cache into
+// scalar registers, so the FMAs take them as scalar operands and LDS serves
+// only the per-lane weights.
+template <typename T, int P, int NQ, bool BLK, bool WRAP, bool SGF = false>
 __global__ void __launch_bounds__(kPtThreads)
 spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile_start,
                  const double* __restrict__ frac, const int* __restrict__ base,
@@ -353,8 +362,11 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     // slots of bx by bz), pad, f[NQ] (16-byte aligned)
     constexpr int OWY = P + 1, OWZ = 2 * (P + 1), OB = 3 * (P + 1);
     constexpr int OF = (OB + 3 + 3) & ~3;
-    constexpr int REC = OF + NQ;
+    constexpr int REC = SGF ? OF : OF + NQ;
     static_assert(NQ % 4 == 0, "NQ must be a multiple of 4");
+    static_assert(!SGF || BLK, "scalar force reads need the blocked copy");
+    const T* __restrict__ xbb = Xb + (size_t)byl_of(blockIdx.x + blockIdx.y * gridDim.x,
+                                                    gridDim.x * gridDim.y, gridDim.x) * npos * NQ;
     T* stage = reinterpret_cast<T*>(smem_raw);
     int* s_list = reinterpret_cast<int*>(stage + (size_t)cap * REC);   // [8][cap]
     int* s_pid = s_list + 8 * cap;                                       // [cap]
@@ -447,8 +459,15 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                 const int dy = (int)min((unsigned)(ly - rl[1]), (unsigned)P);
                 const int dz = (int)min((unsigned)(lz - rl[2]), (unsigned)P);
                 const T w = rec[dx] * rec[OWY + dy] * rec[OWZ + dz];
+                if constexpr (SGF) {
+                    const int pid = __builtin_amdgcn_readfirstlane(s_pid[list[k]]);
+                    const T* fp = xbb + (size_t)pid * NQ;
 #pragma unroll
-                for (int q = 0; q < NQ; q++) acc[q] += w * rec[OF + q];
+                    for (int q = 0; q < NQ; q++) acc[q] += w * fp[q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) acc[q] += w * rec[OF + q];
+                }
             }
         } else {
             for (int k = 0; k < cnt; k++) {
@@ -458,8 +477,15 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                 const int dy = min(wrapk(ly - rl[1], K), P);
                 const int dz = min(wrapk(lz - rl[2], K), P);
                 const T w = rec[dx] * rec[OWY + dy] * rec[OWZ + dz];
+                if constexpr (SGF) {
+                    const int pid = __builtin_amdgcn_readfirstlane(s_pid[list[k]]);
+                    const T* fp = xbb + (size_t)pid * NQ;
 #pragma unroll
-                for (int q = 0; q < NQ; q++) acc[q] += w * rec[OF + q];
+                    for (int q = 0; q < NQ; q++) acc[q] += w * fp[q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) acc[q] += w * rec[OF + q];
+                }
             }
         }
     };
@@ -509,7 +535,9 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                 s_pid[sl] = is;
             }
             __syncthreads();
-            if constexpr (BLK) {
+            if constexpr (SGF) {
+                // forces are read in the accumulation
+            } else if constexpr (BLK) {
                 // NQ contiguous values per slot, 16 bytes per load, the loads
                 // of a thread in flight together
                 constexpr int VW = 16 / sizeof(T);
@@ -1331,16 +1359,28 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
     if (point_owner()) {
         constexpr int NQL = sizeof(T) == 4 ? 48 : 24;
         const int NQ = pt_meshes<T>(b, NC);
-        const int rec = ((3 * (porder + 1) + 3 + 3) & ~3) + NQ;
+        // FP32 reads the blocked forces through the scalar path (measured
+        // faster at K = 128; FP64 keeps them staged in LDS)
+        const bool sgf = xb != nullptr && sizeof(T) == 4;
+        const int rec = ((3 * (porder + 1) + 3 + 3) & ~3) + (sgf ? 0 : NQ);
         const int cap = (int)(65536 / (rec * sizeof(T) + 9 * sizeof(int)));
         const size_t lds = (size_t)cap * (rec * sizeof(T) + 9 * sizeof(int));
         const int nsx = (K + 31) / 32;
         dim3 grid(nsx * (K / 4) * (K / 4), ceil_div(b, NQ / NC));
-#define SDT_SPREAD_PW(PP, QQ, BB, WW)                                                          \
-    hipLaunchKernelGGL((spread_pt_kernel<T, PP, QQ, BB, WW>), grid, dim3(kPtThreads), lds, s, K, \
-                       ts, ntile, nsx, d_tile_start.get(), d_frac.get(), d_base.get(),         \
+#define SDT_SPREAD_PWS(PP, QQ, BB, WW, SS)                                                     \
+    hipLaunchKernelGGL((spread_pt_kernel<T, PP, QQ, BB, WW, SS>), grid, dim3(kPtThreads), lds, s, \
+                       K, ts, ntile, nsx, d_tile_start.get(), d_frac.get(), d_base.get(),      \
                        poly ? d_a2.get() : (double*)nullptr, X, xb, npos, b, NQ / NC, NC, cap, \
                        d_real.get())
+#define SDT_SPREAD_PW(PP, QQ, BB, WW)                                                          \
+    do {                                                                                      \
+        if constexpr (BB && sizeof(T) == 4) {                                                 \
+            if (sgf) SDT_SPREAD_PWS(PP, QQ, BB, WW, true);                                     \
+            else SDT_SPREAD_PWS(PP, QQ, BB, WW, false);                                        \
+        } else {                                                                              \
+            SDT_SPREAD_PWS(PP, QQ, BB, WW, false);                                             \
+        }                                                                                     \
+    } while (0)
 #define SDT_SPREAD_PT(PP, QQ, BB)                                                              \
     do {                                                                                      \
         if (K < 32 + PP - 1) SDT_SPREAD_PW(PP, QQ, BB, true);                                  \
@@ -1356,6 +1396,7 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
 #undef SDT_SPREAD_P
 #undef SDT_SPREAD_PT
 #undef SDT_SPREAD_PW
+#undef SDT_SPREAD_PWS
         SDT_HIP_CHECK(hipGetLastError());
         return;
     }
