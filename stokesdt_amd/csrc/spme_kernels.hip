@@ -1156,13 +1156,24 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
                 const VT* rp = region + ((size_t)(s_b[t * 3 + 2] + iz) * RY + s_b[t * 3 + 1]) * RX +
                                s_b[t * 3 + 0];
                 VT acc = VT(0);
-                // one stencil row at a time keeps the live vector reads to P
-#pragma unroll 1
-                for (int iy = 0; iy < P; iy++) {
-                    VT row = VT(0);
+                if constexpr (MV == 1) {
+                    // rows in pairs: their reads are issued ahead of the sums
+#pragma unroll 2
+                    for (int iy = 0; iy < P; iy++) {
+                        VT row = VT(0);
 #pragma unroll
-                    for (int ix = 0; ix < P; ix++) row += wx[ix] * rp[iy * RX + ix];
-                    acc += wy[iy] * row;
+                        for (int ix = 0; ix < P; ix++) row += wx[ix] * rp[iy * RX + ix];
+                        acc += wy[iy] * row;
+                    }
+                } else {
+                    // one stencil row at a time keeps the live vector reads to P
+#pragma unroll 1
+                    for (int iy = 0; iy < P; iy++) {
+                        VT row = VT(0);
+#pragma unroll
+                        for (int ix = 0; ix < P; ix++) row += wx[ix] * rp[iy * RX + ix];
+                        acc += wy[iy] * row;
+                    }
                 }
                 s_part[t * P + iz] = acc * s_w[(t * 3 + 2) * P + iz];
             }
