@@ -171,7 +171,9 @@ template <typename T, int RC, typename XT, bool BLK>
 static void launch_real(int npos, const int* d_rowptr, const int* d_colidx, const double* d_pos,
                         const double* d_rdi, double box, double xi, int b, double alpha,
                         const XT* X, double beta, double* Y, hipStream_t s) {
-    constexpr int LPR = 8;
+    // lanes per row: 4 for the blocked gathers (measured at b = 100), 8 for
+    // the in-place ones (b < 8)
+    constexpr int LPR = BLK ? 4 : 8;
     dim3 grid(ceil_div((size_t)npos * LPR, 256), ceil_div(b, RC));
     hipLaunchKernelGGL((real_matvec_kernel<T, RC, LPR, XT, BLK>), grid, dim3(256), 0, s, npos,
                        d_rowptr, d_colidx, d_pos, d_rdi, box, xi, b, alpha, X, beta, Y);
