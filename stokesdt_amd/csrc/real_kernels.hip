@@ -302,18 +302,29 @@ void RealOverlap::build(int npos, const int* d_rowptr, const int* d_colidx, cons
 }
 
 // Y[(3 i + c) b + r] += alpha sum over the overlapping pairs of row i, one
-// thread per (listed row, column); runs after real_matvec_kernel.
-__global__ void real_overlap_kernel(int nrows, const int* __restrict__ rows,
+// thread per (listed row, block of kOvRC columns): each pair's FP64 block is
+// evaluated once per thread and applied to its columns; runs after
+// real_matvec_kernel.
+constexpr int kOvRC = 4;
+
+__global__ void __launch_bounds__(256)
+real_overlap_kernel(int nrows, const int* __restrict__ rows,
                                     const int* __restrict__ orp, const int* __restrict__ ocol,
                                     const double* __restrict__ pos, const double* __restrict__ rdi,
                                     double box, double xi, int b, double alpha,
                                     const double* __restrict__ X, double* __restrict__ Y) {
+    const int nblk = (b + kOvRC - 1) / kOvRC;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)nrows * b) return;
-    const int i = rows[t / b];
-    const int r = (int)(t % b);
+    if (t >= (long long)nrows * nblk) return;
+    const int i = rows[t / nblk];
+    const int r0 = (int)(t % nblk) * kOvRC;
+    const int rc = min(kOvRC, b - r0);
     const double aa = rdi[i];
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[3][kOvRC];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < kOvRC; r++) acc[c][r] = 0.0;
     for (int k = orp[i]; k < orp[i + 1]; k++) {
         const int j = ocol[k];
         const double dx = remainder(pos[3 * j] - pos[3 * i], box);
@@ -329,20 +340,30 @@ __global__ void real_overlap_kernel(int nrows, const int* __restrict__ rows,
         ya += -(0.75 + s2 * 0.25 / rr) / rv + 2.0 / (aa + ab) - 9.0 * rv / 16.0 / s2;
         const double ex = dx / rv, ey = dy / rv, ez = dz / rv;
         const double d = xa - ya;
-        const double* xj = X + (size_t)(3 * j) * b + r;
-        const double v0 = xj[0], v1 = xj[b], v2 = xj[2 * b];
-        acc[0] += (d * ex * ex + ya) * v0 + d * ex * ey * v1 + d * ex * ez * v2;
-        acc[1] += d * ex * ey * v0 + (d * ey * ey + ya) * v1 + d * ey * ez * v2;
-        acc[2] += d * ex * ez * v0 + d * ey * ez * v1 + (d * ez * ez + ya) * v2;
+        const double bxx = d * ex * ex + ya, bxy = d * ex * ey, bxz = d * ex * ez;
+        const double byy = d * ey * ey + ya, byz = d * ey * ez, bzz = d * ez * ez + ya;
+        const double* xj = X + (size_t)(3 * j) * b + r0;
+#pragma unroll
+        for (int r = 0; r < kOvRC; r++) {
+            if (r < rc) {
+                const double v0 = xj[r], v1 = xj[b + r], v2 = xj[2 * b + r];
+                acc[0][r] += bxx * v0 + bxy * v1 + bxz * v2;
+                acc[1][r] += bxy * v0 + byy * v1 + byz * v2;
+                acc[2][r] += bxz * v0 + byz * v1 + bzz * v2;
+            }
+        }
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) Y[(size_t)(3 * i + c) * b + r] += alpha * acc[c];
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < kOvRC; r++)
+            if (r < rc) Y[(size_t)(3 * i + c) * b + r0 + r] += alpha * acc[c][r];
 }
 
 void real_overlap(const RealOverlap& ov, const double* d_pos, const double* d_rdi, double box,
                   double xi, int b, double alpha, const double* X, double* Y, hipStream_t s) {
     if (ov.nrows == 0) return;
-    const size_t nt = (size_t)ov.nrows * b;
+    const size_t nt = (size_t)ov.nrows * ((b + kOvRC - 1) / kOvRC);
     hipLaunchKernelGGL(real_overlap_kernel, dim3(ceil_div(nt, 256)), dim3(256), 0, s, ov.nrows,
                        ov.rows.get(), ov.rowptr.get(), ov.col.get(), d_pos, d_rdi, box, xi, b,
                        alpha, X, Y);
