@@ -872,20 +872,21 @@ fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict
     }
     __syncthreads();
     fft_any<T, +1, MAXI2, R1, R2>(buf, Kh, PS, fa, tw);
-    // pack the half spectra of rows 2j, 2j+1 into Z_j = A + i B (full x)
+    // pack the half spectra of rows 2j, 2j+1 into Z_j = A + i B (full x):
+    // the pair A, B at kx = kk gives Z_j[kk] and Z_j[K - kk]
     {
-        cplx<T> vz[MAXP];
-        const int nit = H * K;
+        cplx<T> vz[MAXP], vw[MAXP];
+        const int nit = H * Kh;
 #pragma unroll
         for (int u = 0; u < MAXP; u++) {
             const int i = threadIdx.x + u * blockDim.x;
             if (i < nit) {
-                const int j = kd.div_k(i), k = i - j * K;
-                const int kk = (k <= H) ? k : K - k;
+                const int j = kd.div_kh(i), kk = i - j * Kh;
                 cplx<T> A = buf[kk * PS + lpad(2 * j)];
                 cplx<T> B = buf[kk * PS + lpad(2 * j + 1)];
                 if (kk == 0 || 2 * kk == K) { A.y = T(0); B.y = T(0); }
-                vz[u] = (k <= H) ? cplx<T>{A.x - B.y, A.y + B.x} : cplx<T>{A.x + B.y, B.x - A.y};
+                vz[u] = cplx<T>{A.x - B.y, A.y + B.x};
+                vw[u] = cplx<T>{A.x + B.y, B.x - A.y};
             }
         }
         __syncthreads();
@@ -893,8 +894,9 @@ fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict
         for (int u = 0; u < MAXP; u++) {
             const int i = threadIdx.x + u * blockDim.x;
             if (i < nit) {
-                const int j = kd.div_k(i), k = i - j * K;
-                buf[j * PS + lpad(k)] = vz[u];
+                const int j = kd.div_kh(i), kk = i - j * Kh;
+                buf[j * PS + lpad(kk)] = vz[u];
+                if (kk > 0 && 2 * kk < K) buf[j * PS + lpad(K - kk)] = vw[u];
             }
         }
         __syncthreads();
