@@ -1353,6 +1353,7 @@ bool SpmeGpu<T>::point_owner() const {
 // point-owner meshes per workgroup: 12 when few are needed
 template <typename T>
 static int pt_meshes(int b, int NC) {
+    if (b * NC <= 4) return 4;
     return (b * NC <= 12) ? 12 : (sizeof(T) == 4 ? 48 : 24);
 }
 
@@ -1360,7 +1361,7 @@ template <typename T>
 int SpmeGpu<T>::blocked_cols(int b) const {
     if (poly || !point_owner()) return 0;
     const int NQ = pt_meshes<T>(b, 3);
-    return NQ == 12 ? 0 : NQ / 3;
+    return NQ <= 12 ? 0 : NQ / 3;
 }
 
 template <typename T>
@@ -1401,7 +1402,8 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
     } while (0)
 #define SDT_SPREAD_P(PP)                                                                       \
     case PP:                                                                                   \
-        if (NQ == 12) SDT_SPREAD_PT(PP, 12, false);                                            \
+        if (NQ == 4) SDT_SPREAD_PT(PP, 4, false);                                              \
+        else if (NQ == 12) SDT_SPREAD_PT(PP, 12, false);                                       \
         else if (xb) SDT_SPREAD_PT(PP, NQL, true);                                             \
         else SDT_SPREAD_PT(PP, NQL, false);                                                    \
         break;
