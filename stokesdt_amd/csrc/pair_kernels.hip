@@ -265,57 +265,71 @@ __global__ void steric_kernel(int npos, const double* __restrict__ pos, const do
 // 2 r / (a_i + a_j) < cutoff with r from the wrapped positions exactly as in
 // find_pairs_kernel<'S'> (pair_s.cc:148-178; the reference's whole-cell
 // shortcuts only accept pairs that pass this test), and the force of every
-// taken j != i is the one steric_kernel adds; partners are visited cell by
-// cell in ascending particle order. No pair list is stored and the host
-// never waits for its size.
+// taken j != i is the one steric_kernel adds. Eight lanes share a particle,
+// each visiting every eighth of the 27 cells (members in ascending order);
+// the lanes' sums are combined in a fixed tree. No pair list is stored and
+// the host never waits for its size.
+constexpr int kStericLanes = 8;   // lanes per particle, each taking every 8th cell
+
 __global__ void steric_cell_kernel(int npos, const double* __restrict__ pos,
                                    const double* __restrict__ pos0,
                                    const double* __restrict__ rdi, const int* __restrict__ cidx,
                                    const int* __restrict__ cell_start,
                                    const int* __restrict__ cell_perm, int nc1, double box,
                                    double cutoff, double r0, double k0, double* __restrict__ f) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npos) return;
-    const int c1 = cidx[i];
-    const int ix1 = c1 / (nc1 * nc1), iy1 = (c1 / nc1) % nc1, iz1 = c1 % nc1;
-    const double x1 = pos0[3 * i], y1 = pos0[3 * i + 1], z1 = pos0[3 * i + 2];
-    const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
-    const double r1 = rdi[i];
-    const double half = box / 2.0;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(tid / kStericLanes), sub = (int)(tid % kStericLanes);
+    const bool on = i < npos;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int dx = -1; dx <= 1; dx++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dz = -1; dz <= 1; dz++) {
-                const int c2 = (wrapc(ix1 + dx, nc1) * nc1 + wrapc(iy1 + dy, nc1)) * nc1 +
-                               wrapc(iz1 + dz, nc1);
-                const int jb = cell_start[c2], je = cell_start[c2 + 1];
-                for (int t = jb; t < je; t++) {
-                    const int j = cell_perm[t];
-                    if (j == i) continue;
-                    const double r2 = rdi[j];
-                    double rx = fabs(pos0[3 * j] - x1);
-                    double ry = fabs(pos0[3 * j + 1] - y1);
-                    double rz = fabs(pos0[3 * j + 2] - z1);
-                    rx = (rx > half ? rx - box : rx);
-                    ry = (ry > half ? ry - box : ry);
-                    rz = (rz > half ? rz - box : rz);
-                    double rr = rx * rx + ry * ry + rz * rz;
-                    if (rr == 0.0) rr = 1.0e-10;
-                    const double sv = 2.0 * sqrt(rr) / (r1 + r2);
-                    if (!(sv < cutoff)) continue;
-                    const double qx = remainder(px - pos[3 * j], box);
-                    const double qy = remainder(py - pos[3 * j + 1], box);
-                    const double qz = remainder(pz - pos[3 * j + 2], box);
-                    const double r = sqrt(qx * qx + qy * qy + qz * qz);
-                    const double force = k0 * ((r1 + r2) * r0 / 2.0 - r);
-                    f0 += force * (qx / r);
-                    f1 += force * (qy / r);
-                    f2 += force * (qz / r);
-                }
+    if (on) {
+        const int c1 = cidx[i];
+        const int ix1 = c1 / (nc1 * nc1), iy1 = (c1 / nc1) % nc1, iz1 = c1 % nc1;
+        const double x1 = pos0[3 * i], y1 = pos0[3 * i + 1], z1 = pos0[3 * i + 2];
+        const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        const double r1 = rdi[i];
+        const double half = box / 2.0;
+        for (int d = sub; d < 27; d += kStericLanes) {
+            const int dx = d / 9 - 1, dy = (d / 3) % 3 - 1, dz = d % 3 - 1;
+            const int c2 = (wrapc(ix1 + dx, nc1) * nc1 + wrapc(iy1 + dy, nc1)) * nc1 +
+                           wrapc(iz1 + dz, nc1);
+            const int jb = cell_start[c2], je = cell_start[c2 + 1];
+            for (int t = jb; t < je; t++) {
+                const int j = cell_perm[t];
+                if (j == i) continue;
+                const double r2 = rdi[j];
+                double rx = fabs(pos0[3 * j] - x1);
+                double ry = fabs(pos0[3 * j + 1] - y1);
+                double rz = fabs(pos0[3 * j + 2] - z1);
+                rx = (rx > half ? rx - box : rx);
+                ry = (ry > half ? ry - box : ry);
+                rz = (rz > half ? rz - box : rz);
+                double rr = rx * rx + ry * ry + rz * rz;
+                if (rr == 0.0) rr = 1.0e-10;
+                const double sv = 2.0 * sqrt(rr) / (r1 + r2);
+                if (!(sv < cutoff)) continue;
+                const double qx = remainder(px - pos[3 * j], box);
+                const double qy = remainder(py - pos[3 * j + 1], box);
+                const double qz = remainder(pz - pos[3 * j + 2], box);
+                const double r = sqrt(qx * qx + qy * qy + qz * qz);
+                const double force = k0 * ((r1 + r2) * r0 / 2.0 - r);
+                f0 += force * (qx / r);
+                f1 += force * (qy / r);
+                f2 += force * (qz / r);
             }
-    f[3 * i] += f0;
-    f[3 * i + 1] += f1;
-    f[3 * i + 2] += f2;
+        }
+    }
+    // a particle's lanes are consecutive: sum them in a fixed tree
+#pragma unroll
+    for (int off = kStericLanes / 2; off > 0; off >>= 1) {
+        f0 += __shfl_xor(f0, off, kStericLanes);
+        f1 += __shfl_xor(f1, off, kStericLanes);
+        f2 += __shfl_xor(f2, off, kStericLanes);
+    }
+    if (on && sub == 0) {
+        f[3 * i] += f0;
+        f[3 * i + 1] += f1;
+        f[3 * i + 2] += f2;
+    }
 }
 
 void PairListGpu::steric_direct(const double* d_pos, double r0, double k0, double* d_f,
@@ -337,7 +351,8 @@ void PairListGpu::steric_direct(const double* d_pos, double r0, double k0, doubl
     hipLaunchKernelGGL(cell_assign_kernel, dim3(g), dim3(256), 0, s, npos, d_pos, box, ncd,
                        d_pos0.get(), d_dcidx.get());
     bucket_sort(d_dcidx.get(), npos, nc3, d_dperm.get(), d_dstart.get(), nullptr, sc, s);
-    hipLaunchKernelGGL(steric_cell_kernel, dim3(g), dim3(256), 0, s, npos, d_pos, d_pos0.get(),
+    hipLaunchKernelGGL(steric_cell_kernel, dim3(ceil_div((size_t)npos * kStericLanes, 256)),
+                       dim3(256), 0, s, npos, d_pos, d_pos0.get(),
                        d_rdi.get(), d_dcidx.get(), d_dstart.get(), d_dperm.get(), ncd, box, cutoff,
                        r0, k0, d_f);
     SDT_HIP_CHECK(hipGetLastError());
