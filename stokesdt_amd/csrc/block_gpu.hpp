@@ -33,6 +33,20 @@ void unpack(int npos, int b, const int* perm, int ldv, const double* X, double* 
 // to a compact nrow x nc block (to_compact), or back
 void column_subset(long long nrow, int ld, int nc, const int* d_cols, const double* src,
                    double* dst, bool to_compact, hipStream_t s);
+// Device-side steps of the lockstep block Lanczos iteration, one column
+// group (bs <= 16) per workgroup. The per-group record of an iteration holds
+// kLzRec doubles: [0, 256) H_i (raw 16 x 16 Gram tile), [256, 512) B_i
+// (upper, 16 x 16), [512] 1 if the CholQR succeeded. Groups with mask 0 get
+// zero coefficient blocks.
+constexpr int kLzRec = 520;
+// mode 0: S = H from the Gram tiles d_G (also stored into d_rec); mode 1:
+// S = B^T from the record d_prev of the previous iteration
+void lz_coef(int ng, int bs, int btot, const int* d_mask, const double* d_G, const double* d_prev,
+             double* d_rec, double* d_S, int mode, hipStream_t s);
+// one CholQR pass on the Gram tiles d_G: S = R^{-1}; pass 0 keeps R in
+// d_R1 (ng x 256), pass 1 stores B = R R1 and the success flag in d_rec
+void lz_chol(int ng, int bs, int btot, const int* d_mask, const double* d_G, double* d_R1,
+             double* d_rec, double* d_S, int pass, hipStream_t s);
 void axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s);
 void fill(long long n, double v, double* p, hipStream_t s);
 

@@ -354,6 +354,109 @@ void unpack(int npos, int b, const int* perm, int ldv, const double* X, double* 
     SDT_HIP_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------- Lanczos coefficient steps -----
+// One workgroup per column group; the Gram tiles are 16 x 16 row-major, the
+// coefficient blocks S[g] bs x bs row-major (zero for a masked group and
+// beyond a narrower last group's width).
+
+// mode 0: S[g] = H_g = G[g] (and the raw tile into the record);
+// mode 1: S[g] = B_g^T from the previous iteration's record
+__global__ void __launch_bounds__(256)
+lz_coef_kernel(int bs, int btot, const int* __restrict__ mask, const double* __restrict__ G,
+               const double* __restrict__ prev, double* __restrict__ rec, double* __restrict__ S,
+               int mode) {
+    const int g = blockIdx.x, t = threadIdx.x;
+    const int i = t >> 4, j = t & 15;
+    const int b = min(bs, btot - g * bs);
+    double v;
+    if (mode == 0) {
+        v = G[(size_t)g * 256 + t];
+        rec[(size_t)g * kLzRec + t] = v;
+    } else {
+        v = prev[(size_t)g * kLzRec + 256 + j * 16 + i];
+    }
+    if (i < bs && j < bs)
+        S[((size_t)g * bs + i) * bs + j] = (mask[g] && i < b && j < b) ? v : 0.0;
+}
+
+// One CholQR pass over the Gram tiles G[g] = W_g^T W_g: R = chol(sym G)
+// (upper; host_linalg.cpp chol_upper), S[g] = R^{-1} (inv_upper). Pass 0
+// keeps R in R1[g]; pass 1 writes B_g = R R1[g] (mul_upper) into the record.
+// A masked group, or one whose pivot is not positive (numerically rank
+// deficient: the pass zeroes its columns), gets S = 0, B = 0 and ok = 0.
+// Lane c (< 16) holds column c; the width is padded to 16 with the identity.
+__global__ void __launch_bounds__(64)
+lz_chol_kernel(int bs, int btot, const int* __restrict__ mask, const double* __restrict__ G,
+               double* __restrict__ R1, double* __restrict__ rec, double* __restrict__ S,
+               int pass) {
+    const int g = blockIdx.x, c = threadIdx.x & 15;
+    const bool lead = threadIdx.x < 16;
+    const int b = min(bs, btot - g * bs);
+    double* rg = rec + (size_t)g * kLzRec;
+    bool ok = mask[g] != 0;
+    if (pass == 1) ok = ok && rg[512] != 0.0;
+    const double* Gg = G + (size_t)g * 256;
+    double a[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        a[r] = (r < b && c < b) ? 0.5 * (Gg[r * 16 + c] + Gg[c * 16 + r]) : (r == c ? 1.0 : 0.0);
+    double rc[16];   // column c of R
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const double p = __shfl(a[j], j);
+        if (!(p > 0.0)) ok = false;
+        const double rjj = p > 0.0 ? sqrt(p) : 1.0;
+        const double rjc = c == j ? rjj : (c > j ? a[j] / rjj : 0.0);
+        rc[j] = rjc;
+#pragma unroll
+        for (int i = j + 1; i < 16; i++) a[i] -= __shfl(rjc, i) * rjc;
+    }
+    // column c of R^{-1} by back substitution
+    double x[16];
+#pragma unroll
+    for (int r = 15; r >= 0; r--) {
+        double t = r == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = r + 1; k < 16; k++) t -= __shfl(rc[r], k) * x[k];
+        x[r] = r <= c ? t / __shfl(rc[r], r) : 0.0;
+    }
+    if (lead) {
+        if (c < bs)
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if (i < bs) S[((size_t)g * bs + i) * bs + c] = (ok && i < b && c < b) ? x[i] : 0.0;
+        if (pass == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) R1[(size_t)g * 256 + r * 16 + c] = rc[r];
+        }
+    }
+    if (pass == 1) {
+        double r1[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) r1[k] = R1[(size_t)g * 256 + k * 16 + c];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = i; k < 16; k++) t += __shfl(rc[i], k) * r1[k];
+            if (lead) rg[256 + i * 16 + c] = (ok && c >= i) ? t : 0.0;
+        }
+    }
+    if (threadIdx.x == 0) rg[512] = ok ? 1.0 : 0.0;
+}
+
+void lz_coef(int ng, int bs, int btot, const int* d_mask, const double* d_G, const double* d_prev,
+             double* d_rec, double* d_S, int mode, hipStream_t s) {
+    lz_coef_kernel<<<ng, 256, 0, s>>>(bs, btot, d_mask, d_G, d_prev, d_rec, d_S, mode);
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
+void lz_chol(int ng, int bs, int btot, const int* d_mask, const double* d_G, double* d_R1,
+             double* d_rec, double* d_S, int pass, hipStream_t s) {
+    lz_chol_kernel<<<ng, 64, 0, s>>>(bs, btot, d_mask, d_G, d_R1, d_rec, d_S, pass);
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
 __global__ void axpby_kernel(long long n, double a, const double* __restrict__ x, double b,
                              double* __restrict__ y) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -3,7 +3,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <future>
 #include <thread>
 #include "mob_gpu.hpp"
 
@@ -342,7 +341,6 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
     d_G_.reserve((size_t)ng * 256);
     double* V = d_V_.get();
     const size_t sset = (size_t)ng * bs * bs;   // one coefficient block per group
-    std::vector<double> hS(sset), hG((size_t)ng * 256);
     std::vector<LzGroup> grp(ng);
     for (int gi = 0; gi < ng; gi++) {
         LzGroup& g = grp[gi];
@@ -365,162 +363,180 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
             throw Error{SDT_ECONVERGE, "block Lanczos: the right-hand sides are rank deficient"};
         grp[gi].normz = sqrt(tr[gi]);
     }
-    std::future<void> task;
-    int task_iter = -1;
-    auto collect = [&]() {
-        if (!task.valid()) return;
-        task.get();
-        for (LzGroup& g : grp)
-            if (!g.done && g.d_iter == task_iter && g.met) {
-                g.done = true;
-                g.it = task_iter;
-                g.Dfin = g.D;
-            }
-    };
     auto upload_S = [&](const std::vector<double>& h) {
         SDT_HIP_CHECK(hipMemcpyAsync(d_S_.get(), h.data(), sizeof(double) * h.size(),
                                      hipMemcpyHostToDevice, s));
     };
-    int iter;
-    try {
-        for (iter = 0; iter < max_iters_; iter++) {
-            double* Vi = V + (size_t)iter * blk;
-            double* W = V + (size_t)(iter + 1) * blk;
-            double beta = 0.0;
-            if (iter > 0) {
-                // W = V_{i-1} B_{i-1}^T
-                std::fill(hS.begin(), hS.end(), 0.0);
-                for (int gi = 0; gi < ng; gi++) {
-                    const LzGroup& g = grp[gi];
-                    if (g.done) continue;
-                    for (int i = 0; i < g.b; i++)
-                        for (int j = 0; j < g.b; j++)
-                            hS[((size_t)gi * bs + i) * bs + j] = g.B[j * g.b + i];
+    // The coefficient blocks of an iteration (H_i, the CholQR factors, B_i)
+    // are formed on the device; their records are copied to page-locked
+    // memory and read one iteration later, while the device runs the next
+    // iteration: the evaluation of iteration i - 1 decides which groups
+    // iterate in i + 1, as with a host round trip per iteration.
+    const size_t rsz = (size_t)ng * kLzRec;
+    d_rec_.reserve(2 * rsz);
+    d_R1_.reserve((size_t)ng * 256);
+    d_mask_.reserve(ng);
+    h_rec_.reserve((size_t)max_iters_ * rsz);
+    h_mask_.reserve((size_t)max_iters_ * ng);
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        Events() {
+            for (hipEvent_t& x : e) SDT_HIP_CHECK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+        }
+        ~Events() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    // reads the records of iteration pi into the tridiagonal blocks and
+    // evaluates the groups; true when every group is done
+    auto process = [&](int pi) {
+        SDT_HIP_CHECK(hipEventSynchronize(ev.e[pi & 1]));
+        const double* hr = h_rec_.get() + (size_t)pi * rsz;
+        for (int gi = 0; gi < ng; gi++) {
+            LzGroup& g = grp[gi];
+            if (g.done) continue;
+            const double* r = hr + (size_t)gi * kLzRec;
+            const int b = g.b, o = pi * b, lt = 2 * b + 1;
+            ok[gi] = r[512] != 0.0;
+            for (int i = 0; i < b; i++)
+                for (int j = 0; j < b; j++) {
+                    g.H[i * b + j] = r[i * 16 + j];
+                    g.B[i * b + j] = r[256 + i * 16 + j];
                 }
-                upload_S(hS);
-                StageScope sc(timer, 6, s);
-                tsmm_groups1(n, btot, bs, V + (size_t)(iter - 1) * blk, d_S_.get(), 1.0, 0.0, W, s);
-                beta = -1.0;
-            }
-            // W = M V_i - V_{i-1} B^T, one application for the columns of
-            // the groups still iterating (the columns of finished groups stay
-            // zero: their coefficient blocks are zero from here on)
-            std::vector<int> act;
-            for (const LzGroup& g : grp)
-                if (!g.done)
-                    for (int c = 0; c < g.b; c++) act.push_back(g.c0 + c);
-            if ((int)act.size() == btot || 4 * (int)act.size() > 3 * btot) {
-                mob->apply_packed(0, btot, 1.0, Vi, beta, W);
-            } else {
-                const int na = (int)act.size();
-                d_cols_.reserve(na);
-                d_Xa_.reserve((size_t)n * na);
-                d_Wa_.reserve((size_t)n * na);
-                SDT_HIP_CHECK(hipMemcpyAsync(d_cols_.get(), act.data(), sizeof(int) * na,
-                                             hipMemcpyHostToDevice, s));
-                column_subset(n, btot, na, d_cols_.get(), Vi, d_Xa_.get(), true, s);
-                if (beta != 0.0) column_subset(n, btot, na, d_cols_.get(), W, d_Wa_.get(), true, s);
-                mob->apply_packed(0, na, 1.0, d_Xa_.get(), beta, d_Wa_.get());
-                column_subset(n, btot, na, d_cols_.get(), d_Wa_.get(), W, false, s);
-            }
-            // H = V_i^T W ; W -= V_i H
-            { StageScope sc(timer, 6, s); gram_groups(n, btot, bs, Vi, W, d_G_.get(), d_part_, s); }
-            SDT_HIP_CHECK(hipMemcpyAsync(hG.data(), d_G_.get(), sizeof(double) * hG.size(),
-                                         hipMemcpyDeviceToHost, s));
-            SDT_HIP_CHECK(hipStreamSynchronize(s));
-            std::fill(hS.begin(), hS.end(), 0.0);
-            for (int gi = 0; gi < ng; gi++) {
-                LzGroup& g = grp[gi];
-                if (g.done) continue;
-                for (int i = 0; i < g.b; i++)
-                    for (int j = 0; j < g.b; j++) {
-                        g.H[i * g.b + j] = hG[(size_t)gi * 256 + i * 16 + j];
-                        hS[((size_t)gi * bs + i) * bs + j] = g.H[i * g.b + j];
-                    }
-            }
-            upload_S(hS);
-            { StageScope sc(timer, 6, s); tsmm_groups1(n, btot, bs, Vi, d_S_.get(), -1.0, 1.0, W, s); }
-            if (reorth) {
-                // full reorthogonalisation against V_0 .. V_i (classical
-                // Gram-Schmidt, one pass)
-                std::vector<double> C((size_t)(iter + 1) * ng * 256);
-                for (int k = 0; k <= iter; k++) {
-                    gram_groups(n, btot, bs, V + (size_t)k * blk, W, d_G_.get(), d_part_, s);
-                    SDT_HIP_CHECK(hipMemcpyAsync(&C[(size_t)k * ng * 256], d_G_.get(),
-                                                 sizeof(double) * ng * 256, hipMemcpyDeviceToHost,
-                                                 s));
-                }
-                SDT_HIP_CHECK(hipStreamSynchronize(s));
-                const int per = std::max(1, 8192 / (int)sset);
-                for (int k0 = 0; k0 <= iter; k0 += per) {
-                    const int nk = std::min(per, iter + 1 - k0);
-                    std::vector<double> hC((size_t)nk * sset, 0.0);
-                    for (int gi = 0; gi < ng; gi++) {
-                        if (grp[gi].done) continue;
-                        for (int k = 0; k < nk; k++)
-                            for (int i = 0; i < bs; i++)
-                                for (int j = 0; j < bs; j++)
-                                    hC[(((size_t)gi * nk + k) * bs + i) * bs + j] =
-                                        C[((size_t)(k0 + k) * ng + gi) * 256 + i * 16 + j];
-                    }
-                    upload_S(hC);
-                    tsmm_groups(n, btot, bs, nk, V + (size_t)k0 * blk, (long long)blk, d_S_.get(),
-                                -1.0, 1.0, W, s);
-                }
-            }
-            // [V_{i+1}, B_i] = QR(W)
-            cholqr2_groups(n, btot, bs, W, grp, &LzGroup::B, ok, nullptr, d_G_, d_S_,
-                           d_part_, s, timer);
             // T: H_i on the diagonal, B_i below it (and B_i^T above)
-            for (int gi = 0; gi < ng; gi++) {
-                LzGroup& g = grp[gi];
-                if (g.done) continue;
-                const int b = g.b, o = iter * b, lt = 2 * b + 1;
-                auto T = [&](int r, int c) -> double& { return g.Tb[(size_t)r * lt + (c - r + b)]; };
-                for (int c = 0; c < b; c++)
-                    for (int a = c; a < b; a++) {
-                        // H(a, c) for a >= c from the row-major Gram (V_i^T W)[a][c]
-                        const double hv = g.H[a * b + c];
-                        T(o + a, o + c) = hv;
-                        T(o + c, o + a) = hv;
+            auto T = [&](int rr, int cc) -> double& { return g.Tb[(size_t)rr * lt + (cc - rr + b)]; };
+            for (int c = 0; c < b; c++)
+                for (int a = c; a < b; a++) {
+                    // H(a, c) for a >= c from the row-major Gram (V_i^T W)[a][c]
+                    const double hv = g.H[a * b + c];
+                    T(o + a, o + c) = hv;
+                    T(o + c, o + a) = hv;
+                }
+            if (ok[gi])
+                for (int a = 0; a < b; a++)
+                    for (int c = a; c < b; c++) {
+                        const double bv = g.B[a * b + c];   // B(a, c), upper
+                        T(o + b + a, o + c) = bv;
+                        T(o + c, o + b + a) = bv;
                     }
-                if (ok[gi])
-                    for (int a = 0; a < b; a++)
-                        for (int c = a; c < b; c++) {
-                            const double bv = g.B[a * b + c];   // B(a, c), upper
-                            T(o + b + a, o + c) = bv;
-                            T(o + c, o + b + a) = bv;
-                        }
-            }
-            collect();   // the evaluation of iteration iter - 1
-            // an invariant subspace was found: the current iterate is exact
-            for (int gi = 0; gi < ng; gi++) {
-                LzGroup& g = grp[gi];
-                if (g.done || ok[gi]) continue;
-                lz_eval(g, iter, tol_);
+        }
+        // an invariant subspace was found: the iterate of pi is exact
+        for (int gi = 0; gi < ng; gi++) {
+            LzGroup& g = grp[gi];
+            if (g.done || ok[gi]) continue;
+            lz_eval(g, pi, tol_);
+            g.done = true;
+            g.it = pi;
+            g.Dfin = g.D;
+        }
+        std::vector<int> todo;
+        for (int gi = 0; gi < ng; gi++)
+            if (!grp[gi].done && pi >= grp[gi].start - 1) todo.push_back(gi);
+        const double tol = tol_;
+        parallel_for((int)todo.size(), [&](int t) { lz_eval(grp[todo[t]], pi, tol); });
+        bool all = true;
+        for (LzGroup& g : grp) {
+            if (!g.done && g.d_iter == pi && g.met) {
                 g.done = true;
-                g.it = iter;
+                g.it = pi;
                 g.Dfin = g.D;
             }
-            bool all = true;
-            for (const LzGroup& g : grp) all = all && g.done;
-            if (all) break;
-            std::vector<int> todo;
-            for (int gi = 0; gi < ng; gi++)
-                if (!grp[gi].done && iter >= grp[gi].start - 1) todo.push_back(gi);
-            if (!todo.empty()) {
-                const double tol = tol_;
-                task = std::async(std::launch::async, [&grp, todo, iter, tol] {
-                    parallel_for((int)todo.size(), [&](int t) { lz_eval(grp[todo[t]], iter, tol); });
-                });
-                task_iter = iter;
+            all = all && g.done;
+        }
+        return all;
+    };
+    int iter;
+    bool all = false;
+    for (iter = 0; iter < max_iters_ && !all; iter++) {
+        double* Vi = V + (size_t)iter * blk;
+        double* W = V + (size_t)(iter + 1) * blk;
+        double* rec = d_rec_.get() + (size_t)(iter & 1) * rsz;
+        const double* prev = d_rec_.get() + (size_t)((iter + 1) & 1) * rsz;
+        int* hm = h_mask_.get() + (size_t)iter * ng;
+        for (int gi = 0; gi < ng; gi++) hm[gi] = grp[gi].done ? 0 : 1;
+        SDT_HIP_CHECK(hipMemcpyAsync(d_mask_.get(), hm, sizeof(int) * ng, hipMemcpyHostToDevice, s));
+        double beta = 0.0;
+        if (iter > 0) {
+            // W = V_{i-1} B_{i-1}^T
+            StageScope sc(timer, 6, s);
+            lz_coef(ng, bs, btot, d_mask_.get(), nullptr, prev, nullptr, d_S_.get(), 1, s);
+            tsmm_groups1(n, btot, bs, V + (size_t)(iter - 1) * blk, d_S_.get(), 1.0, 0.0, W, s);
+            beta = -1.0;
+        }
+        // W = M V_i - V_{i-1} B^T, one application for the columns of
+        // the groups still iterating (the columns of finished groups stay
+        // zero: their coefficient blocks are zero from here on)
+        std::vector<int> act;
+        for (const LzGroup& g : grp)
+            if (!g.done)
+                for (int c = 0; c < g.b; c++) act.push_back(g.c0 + c);
+        if ((int)act.size() == btot || 4 * (int)act.size() > 3 * btot) {
+            mob->apply_packed(0, btot, 1.0, Vi, beta, W);
+        } else {
+            const int na = (int)act.size();
+            d_cols_.reserve(na);
+            d_Xa_.reserve((size_t)n * na);
+            d_Wa_.reserve((size_t)n * na);
+            SDT_HIP_CHECK(hipMemcpyAsync(d_cols_.get(), act.data(), sizeof(int) * na,
+                                         hipMemcpyHostToDevice, s));
+            column_subset(n, btot, na, d_cols_.get(), Vi, d_Xa_.get(), true, s);
+            if (beta != 0.0) column_subset(n, btot, na, d_cols_.get(), W, d_Wa_.get(), true, s);
+            mob->apply_packed(0, na, 1.0, d_Xa_.get(), beta, d_Wa_.get());
+            column_subset(n, btot, na, d_cols_.get(), d_Wa_.get(), W, false, s);
+        }
+        // H = V_i^T W ; W -= V_i H
+        {
+            StageScope sc(timer, 6, s);
+            gram_groups(n, btot, bs, Vi, W, d_G_.get(), d_part_, s);
+            lz_coef(ng, bs, btot, d_mask_.get(), d_G_.get(), nullptr, rec, d_S_.get(), 0, s);
+            tsmm_groups1(n, btot, bs, Vi, d_S_.get(), -1.0, 1.0, W, s);
+        }
+        if (reorth) {
+            // full reorthogonalisation against V_0 .. V_i (classical
+            // Gram-Schmidt, one pass)
+            std::vector<double> C((size_t)(iter + 1) * ng * 256);
+            for (int k = 0; k <= iter; k++) {
+                gram_groups(n, btot, bs, V + (size_t)k * blk, W, d_G_.get(), d_part_, s);
+                SDT_HIP_CHECK(hipMemcpyAsync(&C[(size_t)k * ng * 256], d_G_.get(),
+                                             sizeof(double) * ng * 256, hipMemcpyDeviceToHost,
+                                             s));
+            }
+            SDT_HIP_CHECK(hipStreamSynchronize(s));
+            const int per = std::max(1, 8192 / (int)sset);
+            for (int k0 = 0; k0 <= iter; k0 += per) {
+                const int nk = std::min(per, iter + 1 - k0);
+                std::vector<double> hC((size_t)nk * sset, 0.0);
+                for (int gi = 0; gi < ng; gi++) {
+                    if (grp[gi].done) continue;
+                    for (int k = 0; k < nk; k++)
+                        for (int i = 0; i < bs; i++)
+                            for (int j = 0; j < bs; j++)
+                                hC[(((size_t)gi * nk + k) * bs + i) * bs + j] =
+                                    C[((size_t)(k0 + k) * ng + gi) * 256 + i * 16 + j];
+                }
+                upload_S(hC);
+                tsmm_groups(n, btot, bs, nk, V + (size_t)k0 * blk, (long long)blk, d_S_.get(),
+                            -1.0, 1.0, W, s);
             }
         }
-        collect();
-    } catch (...) {
-        if (task.valid()) task.wait();
-        throw;
+        // [V_{i+1}, B_i] = QR(W), CholQR2
+        {
+            StageScope sc(timer, 6, s);
+            for (int pass = 0; pass < 2; pass++) {
+                gram_groups(n, btot, bs, W, W, d_G_.get(), d_part_, s);
+                lz_chol(ng, bs, btot, d_mask_.get(), d_G_.get(), d_R1_.get(), rec, d_S_.get(), pass,
+                        s);
+                tsmm_groups1(n, btot, bs, W, d_S_.get(), 1.0, 0.0, W, s);   // in place
+            }
+        }
+        SDT_HIP_CHECK(hipMemcpyAsync(h_rec_.get() + (size_t)iter * rsz, rec, sizeof(double) * rsz,
+                                     hipMemcpyDeviceToHost, s));
+        SDT_HIP_CHECK(hipEventRecord(ev.e[iter & 1], s));
+        if (iter > 0) all = process(iter - 1);
     }
+    // the last iteration's records when the loop ran out of iterations
+    if (!all) process(iter - 1);
     // groups that did not converge keep the last iterate (brwn_lanczos.cc:208-221)
     for (LzGroup& g : grp) {
         if (g.done) continue;

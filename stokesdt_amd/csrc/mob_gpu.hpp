@@ -97,6 +97,13 @@ class BrwnLanczosGpu {
     DevBuf<double> d_V_, d_G_, d_S_, d_part_, d_io_z_, d_io_y_, d_Zp_, d_Yp_;
     DevBuf<double> d_Xa_, d_Wa_;   // active columns of a partly finished batch
     DevBuf<int> d_cols_;
+    // device-side iteration steps: per-group records of the last two
+    // iterations, the first CholQR factor, the group mask; host copies of
+    // every iteration's records and masks
+    DevBuf<double> d_rec_, d_R1_;
+    DevBuf<int> d_mask_;
+    PinnedBuf<double> h_rec_;
+    PinnedBuf<int> h_mask_;
 };
 
 // Steric force (src/force/force_steric.cc) on the S pair list.

@@ -68,6 +68,36 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// page-locked host buffer (hipHostMalloc), grow-only; released only when no
+// copy into or out of it can be pending
+template <typename T>
+struct PinnedBuf {
+    T* ptr = nullptr;
+    size_t n = 0;
+    PinnedBuf() {}
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    void reserve(size_t count) {
+        if (count <= n && ptr) return;
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipHostMalloc((void**)&ptr, count * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            throw Error{SDT_ENOMEM, "hipHostMalloc of " + std::to_string(count * sizeof(T)) +
+                                        " bytes failed: " + hipGetErrorString(e)};
+        }
+        n = count;
+    }
+    T* get() const { return ptr; }
+};
+
 template <typename T>
 struct cplx {
     T x, y;
