@@ -392,11 +392,14 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
 #pragma unroll
     for (int q = 0; q < NQ; q++) acc[q] = T(0);
 
-    // contributor base tiles (distinct, with wrap): bases in [lo, lo + len)
-    if (threadIdx.x == 0) {
+    // contributor base tiles (distinct, with wrap): bases in [lo, lo + len);
+    // wave 0 reads the bucket bounds of 64 tiles at a time and scans the
+    // counts into the candidate offsets
+    if (wave == 0) {
         int first[3], cnt[3];
         const int lo[3] = {x0 - (P - 1), y0 - (P - 1), z0 - (P - 1)};
         const int len[3] = {xlen + P - 1, 4 + P - 1, 4 + P - 1};
+#pragma unroll
         for (int d = 0; d < 3; d++) {
             if (len[d] >= K) { first[d] = 0; cnt[d] = ntile; continue; }
             const int a = (lo[d] >= 0) ? lo[d] / ts : -((-lo[d] + ts - 1) / ts);
@@ -405,20 +408,35 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
             cnt[d] = min(bt - a + 1, ntile);
             first[d] = ((a % ntile) + ntile) % ntile;
         }
-        int off = 0, k = 0;
-        for (int jz = 0; jz < cnt[2]; jz++)
-            for (int jy = 0; jy < cnt[1]; jy++)
-                for (int jx = 0; jx < cnt[0]; jx++) {
-                    const int tt = (((first[2] + jz) % ntile) * ntile + (first[1] + jy) % ntile) *
-                                       ntile + (first[0] + jx) % ntile;
-                    const int pb = tile_start[tt], pe = tile_start[tt + 1];
-                    s_beg[k] = pb;
-                    s_off[k] = off;
-                    off += pe - pb;
-                    k++;
-                }
-        s_off[k] = off;
-        s_wave[0] = k;     // tile count, read below before s_wave is reused
+        const int ntot = cnt[0] * cnt[1] * cnt[2];
+        int run = 0;
+        for (int k0 = 0; k0 < ntot; k0 += 64) {
+            const int k = k0 + lane;
+            int pb = 0, n = 0;
+            if (k < ntot) {
+                const int jx = k % cnt[0], jyz = k / cnt[0];
+                const int jy = jyz % cnt[1], jz = jyz / cnt[1];
+                const int tt = (((first[2] + jz) % ntile) * ntile + (first[1] + jy) % ntile) *
+                                   ntile + (first[0] + jx) % ntile;
+                pb = tile_start[tt];
+                n = tile_start[tt + 1] - pb;
+            }
+            int inc = n;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
+            }
+            if (k < ntot) {
+                s_beg[k] = pb;
+                s_off[k] = run + inc - n;
+            }
+            run += __shfl(inc, 63);
+        }
+        if (lane == 0) {
+            s_off[ntot] = run;
+            s_wave[0] = ntot;   // tile count, read below before s_wave is reused
+        }
     }
     __syncthreads();
     const int nbt = s_wave[0];
@@ -496,8 +514,13 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
         int is = -1, b3[3] = {0, 0, 0};
         bool take = false;
         if (ci < ncand) {
-            int k = 0;
-            while (k + 1 < nbt && s_off[k + 1] <= ci) k++;
+            // the last tile whose offset is <= ci (empty tiles skipped)
+            int k = 0, hi = nbt - 1;
+            while (k < hi) {
+                const int mid = (k + hi + 1) >> 1;
+                if (s_off[mid] <= ci) k = mid;
+                else hi = mid - 1;
+            }
             is = s_beg[k] + (ci - s_off[k]);
             b3[0] = base[3 * is]; b3[1] = base[3 * is + 1]; b3[2] = base[3 * is + 2];
             const int dx = wrapk(b3[0] - x0, K);
