@@ -173,6 +173,25 @@ def test_lanczos_reorth_agrees(sdt, orc):
     assert orc.frob_rel_err(out[0], out[1]) < 1e-7
 
 
+def test_lanczos_exhausted_krylov_space(sdt, orc):
+    # 4 particles, 12 unknowns, blocks of 4: the Krylov space is the whole
+    # space after three iterations, later blocks are rounding noise (or
+    # exactly rank deficient); the iterate stays the eigen-sqrt product
+    n, box = 4, 16.0
+    g, o = _mob(sdt, orc, n, box, 32, 6, sdt.FP64, 11)
+    M2 = _eig_sqrt(g.mul_vector(np.eye(3 * n)))
+    z = np.random.default_rng(5).standard_normal((3 * n, 4))
+    for maxnrhs in (4, 2):
+        bl = sdt.BrwnLanczos(3 * n, 40, maxnrhs, 1e-9)
+        assert bl.init()
+        y, ok = bl.compute(g, z)
+        _, oit = orc.lanczos(o, z, 40, maxnrhs, 1e-9)
+        assert ok
+        assert np.all(np.isfinite(y))
+        assert orc.frob_rel_err(M2 @ z, y) < 1e-8
+        assert np.abs(np.asarray(bl.iters()) - oit).max() <= 3
+
+
 def test_lanczos_non_convergence_reported(sdt, orc):
     n, box = 200, 16.0
     g, _ = _mob(sdt, orc, n, box, 32, 6, sdt.FP64, 7)
