@@ -84,6 +84,17 @@ int sdt_rpy_spme_ex(const double* pos, const double* rdi, int npos, double L, co
                     int nrhs, int ldf, const char* mode, double xi, double rmax, int dim,
                     int porder, int precision, double* v, int ldv);
 
+/* Dense Ewald-summed RPY matrix with the semantics of the MATLAB entry point
+ * mat = rpy_ewald_matrix_mex(pos, rdi, L, tol, mode, xi)
+ * (mex/rpy_ewald_matrix_mex.cc:10-64 over MobDebug::GetMatrix,
+ * src/mob/mob_debug.cc, and EwaldKernel, src/mob/ewald.cc:86-330): pos AoS
+ * 3 x npos, mode "full", "real" or "recip", tol the truncation tolerance of
+ * both sums; xi <= 0 selects the reference's default 10^(1/6) sqrt(pi) / L.
+ * mat receives the symmetric 3npos x 3npos matrix (leading dimension
+ * ldm >= 3npos); FP64 on the device, npos <= 30000. */
+int sdt_rpy_ewald_matrix(const double* pos, const double* rdi, int npos, double L, double tol,
+                         const char* mode, double xi, double* mat, int ldm);
+
 /* --------------------------------------------------------------- PairList
  * Replaces PairListR / PairListS (src/pairlist/pair_base.h:30-69, pair_r.h,
  * pair_s.h): mode 'R' (min-image r <= cutoff) or 'S' (2r/(a_i+a_j) <

@@ -78,6 +78,7 @@ SIGNATURES = {
     "sdt_bd_last_iters": (_I, [_P, _P, _I, _PI]),
     "sdt_bd_destroy": (_I, [_P]),
     "sdt_force_steric": (_I, [_P, _P, _I, _D, _D, _D, _P]),
+    "sdt_rpy_ewald_matrix": (_I, [_P, _P, _I, _D, _D, ctypes.c_char_p, _D, _P, _I]),
     "sdt_simbox_create": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                ctypes.POINTER(_P)]),
     "sdt_simbox_init": (_I, [_P]),
@@ -241,6 +242,21 @@ def rpy_spme(pos, rdi, L, f, mode, xi, rmax, dim, porder, precision=FP64):
     _check(lib().sdt_rpy_spme_ex(_ptr(pos), _ptr(rdi), rdi.size, L, _ptr(f), nrhs, n,
                                  mode.encode(), xi, rmax, dim, porder, precision, _ptr(v), n))
     return v
+
+
+def rpy_ewald_matrix(pos, rdi, L, tol, mode="full", xi=None):
+    """mat = rpy_ewald_matrix_mex(pos, rdi, L, tol, mode, xi)
+    (mex/rpy_ewald_matrix_mex.cc:10-64). pos: (3, npos) or flat AoS; xi None
+    selects the reference's default splitting. Returns the 3 npos x 3 npos
+    matrix."""
+    pos = _f64(np.asarray(pos).T.reshape(-1) if np.ndim(pos) == 2 and np.shape(pos)[0] == 3
+               else np.asarray(pos).reshape(-1))
+    rdi = _f64(rdi).reshape(-1)
+    n = 3 * rdi.size
+    mat = np.zeros((n, n), order="F")
+    _check(lib().sdt_rpy_ewald_matrix(_ptr(pos), _ptr(rdi), rdi.size, L, tol, mode.encode(),
+                                      -1.0 if xi is None else xi, _ptr(mat), n))
+    return mat
 
 
 class PairList:

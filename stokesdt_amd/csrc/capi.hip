@@ -159,6 +159,34 @@ int sdt_rpy_spme(const double* pos, const double* rdi, int npos, double L, const
                            v, ldv);
 }
 
+int sdt_rpy_ewald_matrix(const double* pos, const double* rdi, int npos, double L, double tol,
+                         const char* mode, double xi, double* mat, int ldm) {
+    // argument checks of mex/rpy_ewald_matrix_mex.cc:12-50 and MobDebug::Init
+    // (src/mob/mob_debug.cc)
+    if (!pos || !rdi || !mat || npos <= 0) { set_error("invalid array arguments"); return SDT_EINVAL; }
+    if (L <= 0. || tol <= 0.) { set_error("Scalar arguments must be positive."); return SDT_EINVAL; }
+    if (!mode) { set_error("invalid argument: mode."); return SDT_EINVAL; }
+    int md;
+    if (strcmp(mode, "full") == 0) md = 0;
+    else if (strcmp(mode, "real") == 0) md = 1;
+    else if (strcmp(mode, "recip") == 0) md = 2;
+    else { set_error("mode must be full, real, or recip."); return SDT_EINVAL; }
+    if (ldm < 3 * npos) { set_error("ldm must be at least 3*npos"); return SDT_EINVAL; }
+    // the reference's default splitting when xi is not given
+    if (xi <= 0.) xi = pow(10.0, 1.0 / 6.0) * sqrt(M_PI) / L;
+    return guard([&] {
+        hipStream_t s;
+        SDT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            ewald_matrix(npos, pos, rdi, L, xi, tol, md, ldm, mat, s);
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        (void)hipStreamDestroy(s);
+    });
+}
+
 // --------------------------------------------------------------- PairList --
 int sdt_pairlist_create(int npos, const double* rdi, double box_size, double cutoff, char mode,
                         sdt_pairlist_t* out) {

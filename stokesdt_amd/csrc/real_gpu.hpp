@@ -38,4 +38,10 @@ void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const doubl
                  const double* d_rdi, double box, double xi, int b, double alpha, const double* X,
                  double beta, double* Y, void* scratch, bool copied, hipStream_t s);
 
+// Dense Ewald-summed RPY matrix (MobDebug::GetMatrix): mode 0 full, 1 real
+// part, 2 reciprocal part; host pos (AoS) and rdi, host output 3N x 3N with
+// leading dimension ldm (symmetric). Synchronises.
+void ewald_matrix(int npos, const double* pos, const double* rdi, double box, double xi,
+                  double tol, int mode, int ldm, double* mat, hipStream_t s);
+
 }  // namespace sdt

@@ -8,6 +8,7 @@
 // thread, so a matvec streams only the column indices and the input vector.
 #include <algorithm>
 #include <cmath>
+#include <vector>
 #include "real_gpu.hpp"
 #include "sort_gpu.hpp"
 
@@ -383,4 +384,197 @@ template void real_matvec<double>(int, const int*, const int*, const double*, co
                                   double, int, double, const double*, double, double*, void*, bool,
                                   hipStream_t);
 
+// ------------------------------------------------------ dense Ewald matrix --
+// The Ewald-summed RPY matrix of MobDebug::GetMatrix (src/mob/mob_debug.cc),
+// i.e. EwaldKernel (src/mob/ewald.cc:86-330) over the tables of
+// InitRealTable / InitRecipTable (:643-797) with the truncation of
+// XitoRmax / XitoKmax (:19-83). One thread per 3x3 block (i, j), j fastest;
+// the image and k-vector tables are read at wave-uniform addresses.
+
+namespace {
+
+double ewald_real_tail(double r, double xi) {
+    const double xir = xi * r, xir2 = xir * xir, r2 = r * r;
+    return erfc(xir) * (0.75 + 1.5 / r2) / r +
+           exp(-xir2) * xi / sqrt(M_PI) *
+               (4.5 + 3.0 * xir2 + (3.0 + xir2 * (14.0 + xir2 * (4.0 + xir2))) / r2);
+}
+
+double ewald_recip_tail(double k, double xi) {
+    const double kx = k / xi, kx2 = kx * kx, k2 = k * k;
+    return 6.0 * M_PI * exp(-kx2 / 4.0) * (1.0 + k2 / 3.0) / k2 *
+           (1.0 + kx2 * (1.0 / 4.0 + kx2 / 8.0));
+}
+
+// shrink the cutoff by 1 % while the bound (one term at the cutoff plus six
+// one period further) stays below tol; xi is detuned by the safety factor
+double ewald_rmax(double xi, double tol, double box) {
+    const double x0 = xi / 1.15;
+    double rmax = 10.0 * sqrt(-log(tol)) / xi;
+    while (ewald_real_tail(rmax * 0.99, x0) + 6.0 * ewald_real_tail(rmax * 0.99 + box, x0) < tol)
+        rmax *= 0.99;
+    return rmax;
+}
+
+double ewald_kmax(double xi, double tol, double box) {
+    const double x0 = xi * 1.15, dk = 2.0 * M_PI / box;
+    double kmax = 20.0 * sqrt(-log(tol)) * xi;
+    while (ewald_recip_tail(kmax * 0.99, x0) + 6.0 * ewald_recip_tail(kmax * 0.99 + dk, x0) < tol)
+        kmax *= 0.99;
+    return kmax;
+}
+
+}  // namespace
+
+constexpr int kEwaldK = 8;   // k1, k2, k3, ex, ey, ez, |k|^2, exp factor
+
+__global__ void __launch_bounds__(256)
+ewald_block_kernel(int npos, const double* __restrict__ pos, const double* __restrict__ rdi,
+                   double box, double xi, double rmax2, int nr, const double* __restrict__ img,
+                   int nk, const double* __restrict__ ktab, int parts, long long ld,
+                   double* __restrict__ mat) {
+    // blocks with j >= i, each stored at (i, j) and (j, i), as EwaldKernel
+    // does: the matrix is symmetric bit for bit
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= npos || j < i) return;
+    const double aa = rdi[i], ab = rdi[j], aa2 = aa * aa, ab2 = ab * ab;
+    const double x0 = remainder(pos[3 * j] - pos[3 * i], box);
+    const double y0 = remainder(pos[3 * j + 1] - pos[3 * i + 1], box);
+    const double z0 = remainder(pos[3 * j + 2] - pos[3 * i + 2], box);
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    auto add = [&](double xa, double ya, double ex, double ey, double ez) {
+        const double d = xa - ya;
+        s[0] += d * ex * ex + ya;
+        s[1] += d * ex * ey;
+        s[2] += d * ex * ez;
+        s[3] += d * ey * ey + ya;
+        s[4] += d * ey * ez;
+        s[5] += d * ez * ez + ya;
+    };
+    if (parts & 1) {
+        if (i == j) {
+            const double self_a =
+                1.0 / aa - xi / sqrt(M_PI) * (6.0 - 40.0 / 3.0 * xi * xi * aa2);
+            s[0] = s[3] = s[5] = self_a;
+        }
+        for (int m = 0; m < nr; m++) {
+            const double xx = x0 + img[3 * m], yy = y0 + img[3 * m + 1], zz = z0 + img[3 * m + 2];
+            const double rr = xx * xx + yy * yy + zz * zz;
+            if (rr == 0.0 || rr > rmax2) continue;
+            const double r = sqrt(rr);
+            const double ex = xx / r, ey = yy / r, ez = zz / r;
+            if (r < aa + ab) {
+                // overlapping pair: the RPY form for overlapping spheres
+                // replaces the far-field one (src/mob/ewald.cc:177-195)
+                const double s2 = aa2 + ab2;
+                add(-(1.5 - s2 * 0.5 / rr) / r + 2.0 / (aa + ab) - 3.0 * r / 8.0 / s2,
+                    -(0.75 + s2 * 0.25 / rr) / r + 2.0 / (aa + ab) - 9.0 * r / 16.0 / s2, ex, ey,
+                    ez);
+            }
+            double xa, ya;
+            ewald_scalars<double>(xi, r, aa, ab, &xa, &ya);
+            add(xa, ya, ex, ey, ez);
+        }
+    }
+    if (parts & 2) {
+        const double s2 = aa2 + ab2;
+        for (int m = 0; m < nk; m++) {
+            const double* k = ktab + (size_t)m * kEwaldK;
+            const double ya = 6.0 * (1.0 - k[6] * s2 / 6.0) * k[7];
+            const double cf = cos(k[0] * x0 + k[1] * y0 + k[2] * z0) * ya;
+            s[0] += cf * (1.0 - k[3] * k[3]);
+            s[1] -= cf * k[3] * k[4];
+            s[2] -= cf * k[3] * k[5];
+            s[3] += cf * (1.0 - k[4] * k[4]);
+            s[4] -= cf * k[4] * k[5];
+            s[5] += cf * (1.0 - k[5] * k[5]);
+        }
+    }
+    // the blocks are symmetric, so row-major rows are the column-major columns
+    auto put = [&](int r, int c) {
+        double* o = mat + (size_t)(3 * r) * ld + 3 * c;
+        o[0] = s[0]; o[1] = s[1]; o[2] = s[2];
+        o += ld;
+        o[0] = s[1]; o[1] = s[3]; o[2] = s[4];
+        o += ld;
+        o[0] = s[2]; o[1] = s[4]; o[2] = s[5];
+    };
+    put(i, j);
+    if (j != i) put(j, i);
+}
+
+void ewald_matrix(int npos, const double* pos, const double* rdi, double box, double xi,
+                  double tol, int mode, int ldm, double* mat, hipStream_t s) {
+    SDT_REQUIRE(npos > 0 && npos <= 30000, SDT_EINVAL, "dense Ewald: npos must be in [1, 30000]");
+    SDT_REQUIRE(box > 0.0 && tol > 0.0 && xi > 0.0, SDT_EINVAL,
+                "dense Ewald: L, tol and xi must be positive");
+    SDT_REQUIRE(ldm >= 3 * npos, SDT_EINVAL, "dense Ewald: ldm must be at least 3*npos");
+    SDT_REQUIRE(mode >= 0 && mode <= 2, SDT_EINVAL, "dense Ewald: mode must be full, real or recip");
+    const int parts = mode == 0 ? 3 : (mode == 1 ? 1 : 2);
+    // periodic images whose shifted cell can hold a pair within rmax
+    std::vector<double> img;
+    double rmax = 0.0;
+    if (parts & 1) {
+        rmax = ewald_rmax(xi, tol, box);
+        const int mi = (int)(rmax / box) + 1;
+        const double reach = rmax + sqrt(3.0) * box;
+        auto near = [&](int m) { return box * m - (m > 0 ? 0.5 * box : (m < 0 ? -0.5 * box : 0.0)); };
+        for (int a = -mi; a <= mi; a++)
+            for (int b = -mi; b <= mi; b++)
+                for (int c = -mi; c <= mi; c++) {
+                    const double dx = near(a), dy = near(b), dz = near(c);
+                    if (sqrt(dx * dx + dy * dy + dz * dz) <= reach) {
+                        img.push_back(box * a);
+                        img.push_back(box * b);
+                        img.push_back(box * c);
+                    }
+                }
+    }
+    // k vectors 2 pi m / L with 0 < |k| <= kmax
+    std::vector<double> kt;
+    if (parts & 2) {
+        const double kmax = ewald_kmax(xi, tol, box);
+        const int mk = (int)(kmax * box / (2.0 * M_PI));
+        const double pivol = M_PI / (box * box * box);
+        for (int a = -mk; a <= mk; a++)
+            for (int b = -mk; b <= mk; b++)
+                for (int c = -mk; c <= mk; c++) {
+                    if (a == 0 && b == 0 && c == 0) continue;
+                    const double k1 = 2.0 * M_PI * a / box, k2 = 2.0 * M_PI * b / box,
+                                 k3 = 2.0 * M_PI * c / box;
+                    const double kk = k1 * k1 + k2 * k2 + k3 * k3, k = sqrt(kk);
+                    if (k > kmax) continue;
+                    const double q = kk / (4.0 * xi * xi);
+                    const double row[kEwaldK] = {k1, k2, k3, k1 / k, k2 / k, k3 / k, kk,
+                                                 pivol * (1.0 + q * (1.0 + 2.0 * q)) / kk * exp(-q)};
+                    kt.insert(kt.end(), row, row + kEwaldK);
+                }
+    }
+    const int nr = (int)(img.size() / 3), nk = (int)(kt.size() / kEwaldK);
+    const size_t dim = (size_t)3 * npos;
+    DevBuf<double> d_pos, d_rdi, d_img, d_k, d_mat;
+    d_pos.reserve(dim);
+    d_rdi.reserve(npos);
+    d_img.reserve(std::max<size_t>(1, img.size()));
+    d_k.reserve(std::max<size_t>(1, kt.size()));
+    d_mat.reserve(dim * dim);
+    SDT_HIP_CHECK(hipMemcpyAsync(d_pos.get(), pos, sizeof(double) * dim, hipMemcpyHostToDevice, s));
+    SDT_HIP_CHECK(hipMemcpyAsync(d_rdi.get(), rdi, sizeof(double) * npos, hipMemcpyHostToDevice, s));
+    if (nr)
+        SDT_HIP_CHECK(hipMemcpyAsync(d_img.get(), img.data(), sizeof(double) * img.size(),
+                                     hipMemcpyHostToDevice, s));
+    if (nk)
+        SDT_HIP_CHECK(hipMemcpyAsync(d_k.get(), kt.data(), sizeof(double) * kt.size(),
+                                     hipMemcpyHostToDevice, s));
+    const dim3 grid((unsigned)((npos + 255) / 256), (unsigned)npos);
+    ewald_block_kernel<<<grid, 256, 0, s>>>(npos, d_pos.get(), d_rdi.get(), box, xi, rmax * rmax,
+                                            nr, d_img.get(), nk, d_k.get(), parts,
+                                            (long long)dim, d_mat.get());
+    SDT_HIP_CHECK(hipGetLastError());
+    SDT_HIP_CHECK(hipMemcpy2DAsync(mat, sizeof(double) * ldm, d_mat.get(), sizeof(double) * dim,
+                                   sizeof(double) * dim, dim, hipMemcpyDeviceToHost, s));
+    SDT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 }  // namespace sdt
+
