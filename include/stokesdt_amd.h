@@ -95,6 +95,15 @@ int sdt_rpy_spme_ex(const double* pos, const double* rdi, int npos, double L, co
 int sdt_rpy_ewald_matrix(const double* pos, const double* rdi, int npos, double L, double tol,
                          const char* mode, double xi, double* mat, int ldm);
 
+/* Diffusion coefficients of tools/calc_diff (calc_diff.cc:100-140): pos
+ * holds nframes > 2 frames of npos AoS positions, frame after frame, dt the
+ * time between frames. For lag i = 1 .. nframes/2 - 1,
+ * D[i-1] = sum over origins j < nframes/2 and particles of
+ * |r(j+i) - r(j)|^2 / (6 npos (nframes/2) i dt), the value calc_diff prints
+ * (every particle counts with radius 1, as there). *nlags receives
+ * nframes/2 - 1; D must hold that many values. FP64 on the device. */
+int sdt_diffusion(const double* pos, int nframes, int npos, double dt, double* D, int* nlags);
+
 /* --------------------------------------------------------------- PairList
  * Replaces PairListR / PairListS (src/pairlist/pair_base.h:30-69, pair_r.h,
  * pair_s.h): mode 'R' (min-image r <= cutoff) or 'S' (2r/(a_i+a_j) <

@@ -79,6 +79,7 @@ SIGNATURES = {
     "sdt_bd_destroy": (_I, [_P]),
     "sdt_force_steric": (_I, [_P, _P, _I, _D, _D, _D, _P]),
     "sdt_rpy_ewald_matrix": (_I, [_P, _P, _I, _D, _D, ctypes.c_char_p, _D, _P, _I]),
+    "sdt_diffusion": (_I, [_P, _I, _I, _D, _P, ctypes.POINTER(ctypes.c_int)]),
     "sdt_simbox_create": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                ctypes.POINTER(_P)]),
     "sdt_simbox_init": (_I, [_P]),
@@ -257,6 +258,18 @@ def rpy_ewald_matrix(pos, rdi, L, tol, mode="full", xi=None):
     _check(lib().sdt_rpy_ewald_matrix(_ptr(pos), _ptr(rdi), rdi.size, L, tol, mode.encode(),
                                       -1.0 if xi is None else xi, _ptr(mat), n))
     return mat
+
+
+def diffusion(pos, dt):
+    """Diffusion coefficients of tools/calc_diff (calc_diff.cc:100-140).
+    pos: (nframes, npos, 3) positions, dt: time between frames. Returns D for
+    lags 1 .. nframes/2 - 1."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    nf, n = pos.shape[0], pos.shape[1]
+    D = np.zeros(max(nf // 2 - 1, 1))
+    nl = ctypes.c_int(0)
+    _check(lib().sdt_diffusion(_ptr(pos), nf, n, dt, _ptr(D), ctypes.byref(nl)))
+    return D[:nl.value].copy()
 
 
 class PairList:

@@ -1,5 +1,6 @@
 """Builds stokesdt_amd/libstokesdt_amd.so for gfx950 with hipcc (in-tree) and the
-command-line BD driver stokesdt_amd/stokesdt_bd linked against it."""
+command-line BD driver stokesdt_amd/stokesdt_bd and the calc_diff tool linked
+against it."""
 import concurrent.futures
 import os
 import subprocess
@@ -11,6 +12,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libstokesdt_amd.so")
 PKG = HERE
 EXE = os.path.join(HERE, "stokesdt_bd")
+CALC_DIFF = os.path.join(HERE, "calc_diff")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SDT_OFFLOAD_ARCH", "gfx950")
 
@@ -23,6 +25,7 @@ HIP_SOURCES = [
     ("block_kernels.hip", []),
     ("mob_gpu.hip", []),
     ("simbox.hip", []),
+    ("diffusion.hip", []),
     ("capi.hip", []),
 ]
 HOST_SOURCES = [("host_linalg.cpp", []), ("host_util.cpp", []), ("molecule_io.cpp", [])]
@@ -68,6 +71,11 @@ def build(verbose=False, jobs=8):
     _run(["g++", "-O2", "-std=c++17", os.path.join(CSRC, "bd_main.cpp"), "-o", exe_tmp,
           "-L" + PKG, "-lstokesdt_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(exe_tmp, EXE)
+    # calc_diff (tools/calc_diff) over the C ABI
+    exe_tmp = CALC_DIFF + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", os.path.join(CSRC, "calc_diff_main.cpp"), "-o", exe_tmp,
+          "-L" + PKG, "-lstokesdt_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe_tmp, CALC_DIFF)
     return LIB
 
 

@@ -4,6 +4,7 @@
 #include <string>
 #include "../../include/stokesdt_amd.h"
 #include "../../include/stokesdt_amd_debug.h"
+#include "diffusion.hpp"
 #include "mob_gpu.hpp"
 #include "simbox.hpp"
 
@@ -179,6 +180,25 @@ int sdt_rpy_ewald_matrix(const double* pos, const double* rdi, int npos, double 
         SDT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         try {
             ewald_matrix(npos, pos, rdi, L, xi, tol, md, ldm, mat, s);
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        (void)hipStreamDestroy(s);
+    });
+}
+
+int sdt_diffusion(const double* pos, int nframes, int npos, double dt, double* D, int* nlags) {
+    if (!pos || !D || !nlags || npos <= 0) { set_error("invalid array arguments"); return SDT_EINVAL; }
+    // calc_diff.cc:47 requires more than two frames
+    if (nframes <= 2) { set_error("need more than two frames"); return SDT_EINVAL; }
+    if (dt == 0.) { set_error("the frame interval must be non-zero"); return SDT_EINVAL; }
+    *nlags = diffusion_lags(nframes);
+    return guard([&] {
+        hipStream_t s;
+        SDT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            diffusion(pos, nframes, npos, dt, D, s);
         } catch (...) {
             (void)hipStreamDestroy(s);
             throw;
