@@ -104,6 +104,25 @@ int sdt_rpy_ewald_matrix(const double* pos, const double* rdi, int npos, double 
  * nframes/2 - 1; D must hold that many values. FP64 on the device. */
 int sdt_diffusion(const double* pos, int nframes, int npos, double dt, double* D, int* nlags);
 
+/* ------------------------------------------------------------- BrwnRandom
+ * Replaces BrwnRandom(dim, num_vecs, max_nrhs) (src/brwn/brwn_random.h:16-86),
+ * Init (src/brwn/brwn_random.cc:42-116) and Compute(mob, nrhs, ldz, z, ldy,
+ * y) (:118-263): with a mobility handle, num_vecs Gaussian vectors (the
+ * Philox stream of seed, one stream index per sampling; the reference
+ * draws from MKL with seed 5678) span V = orth(M^3 Omega); y = V B^{1/2}
+ * V^T z + alpha (I - V V^T) z with B = V^T M V and alpha making
+ * |y|^2 = z^T M z, per column. mob = NULL reuses the last sampling.
+ * num_vecs <= min(dim, 128). Column-major z, y (ldz, ldy >= dim). */
+typedef struct sdt_brwn_random_s* sdt_brwn_random_t;
+int sdt_brwn_random_create(int dim, int num_vecs, int max_nrhs, uint64_t seed,
+                           sdt_brwn_random_t* out);
+int sdt_brwn_random_init(sdt_brwn_random_t br);
+int sdt_brwn_random_compute(sdt_brwn_random_t br, sdt_mob_t mob, int num_rhs, int ldz,
+                            const double* z, int ldy, double* y);
+int sdt_brwn_random_compute_dev(sdt_brwn_random_t br, sdt_mob_t mob, int num_rhs, int ldz,
+                                const double* d_z, int ldy, double* d_y);
+int sdt_brwn_random_destroy(sdt_brwn_random_t br);
+
 /* --------------------------------------------------------------- PairList
  * Replaces PairListR / PairListS (src/pairlist/pair_base.h:30-69, pair_r.h,
  * pair_s.h): mode 'R' (min-image r <= cutoff) or 'S' (2r/(a_i+a_j) <

@@ -80,6 +80,11 @@ SIGNATURES = {
     "sdt_force_steric": (_I, [_P, _P, _I, _D, _D, _D, _P]),
     "sdt_rpy_ewald_matrix": (_I, [_P, _P, _I, _D, _D, ctypes.c_char_p, _D, _P, _I]),
     "sdt_diffusion": (_I, [_P, _I, _I, _D, _P, ctypes.POINTER(ctypes.c_int)]),
+    "sdt_brwn_random_create": (_I, [_I, _I, _I, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "sdt_brwn_random_init": (_I, [_P]),
+    "sdt_brwn_random_compute": (_I, [_P, _P, _I, _I, _P, _I, _P]),
+    "sdt_brwn_random_compute_dev": (_I, [_P, _P, _I, _I, _P, _I, _P]),
+    "sdt_brwn_random_destroy": (_I, [_P]),
     "sdt_simbox_create": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                ctypes.POINTER(_P)]),
     "sdt_simbox_init": (_I, [_P]),
@@ -333,6 +338,35 @@ def pairlist(pos, rdi, L, cutoff, mode):
             continue
         _check(rc)
         return ia, ja[:nnz.value].copy()
+
+
+class BrwnRandom:
+    """BrwnRandom(dim, num_vecs, max_nrhs) (src/brwn/brwn_random.h:16-86);
+    the sample vectors come from the Gaussian stream of seed."""
+
+    def __init__(self, dim, num_vecs, max_nrhs, seed=5678):
+        h = _P()
+        _check(lib().sdt_brwn_random_create(dim, num_vecs, max_nrhs, seed, ctypes.byref(h)))
+        self.h = h
+        self.dim = dim
+
+    def init(self):
+        return lib().sdt_brwn_random_init(self.h) == SDT_OK
+
+    def compute(self, mob, z):
+        """y = sampled approximation of M^{1/2} z; z: (dim, nrhs). mob None
+        reuses the last sampling."""
+        z = _colmajor(z)
+        n, nrhs = z.shape
+        y = np.zeros((n, nrhs), order="F")
+        _check(lib().sdt_brwn_random_compute(self.h, mob.h if mob is not None else None, nrhs, n,
+                                             _ptr(z), n, _ptr(y)))
+        return y
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_brwn_random_destroy(self.h)
+            self.h = None
 
 
 class BrwnLanczos:

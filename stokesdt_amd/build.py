@@ -26,6 +26,7 @@ HIP_SOURCES = [
     ("mob_gpu.hip", []),
     ("simbox.hip", []),
     ("diffusion.hip", []),
+    ("brwn_random.hip", []),
     ("capi.hip", []),
 ]
 HOST_SOURCES = [("host_linalg.cpp", []), ("host_util.cpp", []), ("molecule_io.cpp", [])]
@@ -64,7 +65,8 @@ def build(verbose=False, jobs=8):
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     # command-line BD driver over the C ABI
     exe_tmp = EXE + ".tmp"

@@ -4,6 +4,7 @@
 #include <string>
 #include "../../include/stokesdt_amd.h"
 #include "../../include/stokesdt_amd_debug.h"
+#include "brwn_random.hpp"
 #include "diffusion.hpp"
 #include "mob_gpu.hpp"
 #include "simbox.hpp"
@@ -20,6 +21,7 @@ using namespace sdt;
 
 struct sdt_mob_s { MobSpmeGpu* m; };
 struct sdt_brwn_s { BrwnLanczosGpu* b; };
+struct sdt_brwn_random_s { BrwnRandomGpu* b; };
 struct sdt_pairlist_s {
     PairListGpu pl;
     std::vector<double> rdi;
@@ -204,6 +206,35 @@ int sdt_diffusion(const double* pos, int nframes, int npos, double dt, double* D
             throw;
         }
         (void)hipStreamDestroy(s);
+    });
+}
+
+// ------------------------------------------------------------- BrwnRandom --
+int sdt_brwn_random_create(int dim, int num_vecs, int max_nrhs, uint64_t seed,
+                           sdt_brwn_random_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] { *out = new sdt_brwn_random_s{new BrwnRandomGpu(dim, num_vecs, max_nrhs, seed)}; });
+}
+int sdt_brwn_random_init(sdt_brwn_random_t br) {
+    SDT_CHECK_HANDLE(br);
+    return guard([&] { br->b->init(); });
+}
+int sdt_brwn_random_compute(sdt_brwn_random_t br, sdt_mob_t mob, int num_rhs, int ldz,
+                            const double* z, int ldy, double* y) {
+    SDT_CHECK_HANDLE(br);
+    return guard([&] { br->b->compute(mob ? mob->m : nullptr, num_rhs, ldz, z, ldy, y, false); });
+}
+int sdt_brwn_random_compute_dev(sdt_brwn_random_t br, sdt_mob_t mob, int num_rhs, int ldz,
+                                const double* d_z, int ldy, double* d_y) {
+    SDT_CHECK_HANDLE(br);
+    return guard([&] { br->b->compute(mob ? mob->m : nullptr, num_rhs, ldz, d_z, ldy, d_y, true); });
+}
+int sdt_brwn_random_destroy(sdt_brwn_random_t br) {
+    if (!br) return SDT_OK;
+    return guard([&] {
+        delete br->b;
+        delete br;
     });
 }
 
