@@ -5,6 +5,7 @@
 #include <cstring>
 #include <thread>
 #include "mob_gpu.hpp"
+#include "brwn_random.hpp"
 
 namespace sdt {
 
@@ -713,9 +714,19 @@ BDGpu::BDGpu(int npos, const double* pos, const double* rdi, double box, double 
     steric_.reset(new StericGpu(npos, rdi, box, r0, k0));
 }
 
+BDGpu::~BDGpu() {}
+
+void BDGpu::use_random(int num_vecs, int max_nrhs) {
+    SDT_REQUIRE(!d_pos_.get(), SDT_ESTATE, "use_random after init");
+    // its own Gaussian stream, as the reference's BrwnRandom has its own
+    // RndStream (src/brwn/brwn_random.cc:109)
+    rnd_.reset(new BrwnRandomGpu(3 * npos_, num_vecs, max_nrhs, seed_ ^ 0x5678ull));
+}
+
 void BDGpu::init() {
     mob_->init();
-    brwn_->init();
+    if (rnd_) rnd_->init();
+    else brwn_->init();
     if (steric_on_) steric_->init();
     mob_->set_timer(&timer_);
     brwn_->timer = &timer_;
@@ -739,7 +750,8 @@ void BDGpu::advance(int nsteps) {
         if (irhs == 0 || blk != d_block_) {
             gaussian_fill_device(seed_, (unsigned)blk, (long long)mob_interval_ * ldm_, d_Z_.get(), s);
             mob_->update(d_pos_.get(), nullptr, true);
-            brwn_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
+            if (rnd_) rnd_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
+            else brwn_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
             d_block_ = blk;
         }
         double* disp = d_D_.get() + (size_t)irhs * ldm_;

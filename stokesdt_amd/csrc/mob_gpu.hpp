@@ -127,11 +127,16 @@ class StericGpu {
 
 // BDSimBox::Advance (src/simbox/simbox_bd.cc:296-366) with device-resident
 // positions, Gaussian block, displacements and forces.
+class BrwnRandomGpu;
 class BDGpu {
   public:
     BDGpu(int npos, const double* pos, const double* rdi, double box, double xi, double rmax,
           int dim, int porder, int precision, int max_iters, int max_nrhs, double tol, double r0,
           double k0, double dt, int mob_interval, unsigned long long seed);
+    ~BDGpu();
+    // Brownian displacements by randomised sampling (BrwnRandom) instead of
+    // block Lanczos; before init
+    void use_random(int num_vecs, int max_nrhs);
     void init();
     void advance(int nsteps);
     void get_positions(double* pos);
@@ -152,6 +157,7 @@ class BDGpu {
     std::vector<double> h_pos0_, rdi_;
     std::unique_ptr<MobSpmeGpu> mob_;
     std::unique_ptr<BrwnLanczosGpu> brwn_;
+    std::unique_ptr<BrwnRandomGpu> rnd_;
     std::unique_ptr<StericGpu> steric_;
     DevBuf<double> d_pos_, d_Z_, d_D_, d_f_;
     long long istep_ = 0;

@@ -24,13 +24,20 @@ SimBoxGpu::SimBoxGpu(const char* config_file, const char* model_file, const char
     prm_.spme_dim = io_.get_int_key("spme-nfft-grids", 1, 1024, 128);
     prm_.spme_porder = io_.get_int_key("spme-porder", 4, 6, 4);
     prm_.mob_interval = io_.get_int_key("mob-interval", 1, INT_MAX, 100);
-    // InitBrwn (:127-182): block Lanczos
+    // InitBrwn (:127-182): block Lanczos or randomised sampling
     const char* brwn = io_.get_string_key("brownian-method");
-    SDT_REQUIRE(brwn && strcmp(brwn, "lanczos") == 0, SDT_EINVAL,
-                "brownian-method must be lanczos (Cholesky needs the dense mobility)");
-    prm_.lanczos_maxiters = io_.get_int_key("lanczos-maxiters", 1, INT_MAX, 50);
-    prm_.lanczos_maxnrhs = io_.get_int_key("lanczos-maxnrhs", 1, INT_MAX, 20);
-    prm_.lanczos_tol = io_.get_float_key("lanczos-tol", 0.0, DBL_MAX, 1.0e-3);
+    SDT_REQUIRE(brwn && (strcmp(brwn, "lanczos") == 0 || strcmp(brwn, "random") == 0),
+                SDT_EINVAL,
+                "brownian-method must be lanczos or random (Cholesky needs the dense mobility)");
+    prm_.random = strcmp(brwn, "random") == 0;
+    if (prm_.random) {
+        prm_.random_numvecs = io_.get_int_key("random-numvecs", 1, INT_MAX, 30);
+        prm_.random_maxnrhs = io_.get_int_key("random-maxnrhs", 1, INT_MAX, 50);
+    } else {
+        prm_.lanczos_maxiters = io_.get_int_key("lanczos-maxiters", 1, INT_MAX, 50);
+        prm_.lanczos_maxnrhs = io_.get_int_key("lanczos-maxnrhs", 1, INT_MAX, 20);
+        prm_.lanczos_tol = io_.get_float_key("lanczos-tol", 0.0, DBL_MAX, 1.0e-3);
+    }
     // InitForces (:185-234): steric; bonded forces are not on this path
     SDT_REQUIRE(io_.line_count("bond") == 0, SDT_EINVAL, "bond lines are not supported");
     const int nst = io_.line_count("steric");
@@ -73,6 +80,7 @@ void SimBoxGpu::init() {
                         prm_.lanczos_maxnrhs, prm_.lanczos_tol,
                         prm_.steric ? prm_.steric_r0 : 0.0, prm_.steric ? prm_.steric_k0 : 0.0,
                         prm_.delta_t, prm_.mob_interval, prm_.seed));
+    if (prm_.random) bd_->use_random(prm_.random_numvecs, prm_.random_maxnrhs);
     bd_->init();
     bd_->set_step(step_);
     if (prm_.traj_interval > 0) {

@@ -17,6 +17,8 @@ struct SimBoxParams {
     // Brownian displacements (InitBrwn, :127-182)
     int lanczos_maxiters = 50, lanczos_maxnrhs = 20;
     double lanczos_tol = 1.0e-3;
+    bool random = false;           // brownian-method random (BrwnRandom)
+    int random_numvecs = 30, random_maxnrhs = 50;
     // forces (InitForces, :185-234)
     bool steric = false;
     double steric_r0 = 0.0, steric_k0 = 0.0;

@@ -132,3 +132,37 @@ def test_command_line_driver(sdt, tmp_path):
     assert int(lines[1].split()[0]) == 5
     p = np.array([[float(v) for v in l.split()[1:4]] for l in lines[2:]]).ravel()
     assert np.array_equal(p, sb.positions())
+
+
+def test_simbox_random_method_matches_composed_oracle(sdt, orc, tmp_path):
+    # brownian-method random (simbox_bd.cc:153-157): D = BrwnRandom(Z) on the
+    # mobility at the interval start; without forces every step moves the
+    # particles by sqrt(2 dt) D[:, step]
+    from oracle import brwn_random as BR
+    n, box, nv = 120, 14.0, 20
+    pos = sdt.rsa_positions(n, box, 1.0, 6)
+    xyz = tmp_path / "start.xyz"
+    with open(xyz, "w") as fh:
+        fh.write("%d\n0 0.0 %.17g %.17g %.17g\n" % (n, box, box, box))
+        for i in range(n):
+            fh.write("X %.17g %.17g %.17g\n" % tuple(pos[3 * i:3 * i + 3]))
+    cfg = tmp_path / "run.cfg"
+    cfg.write_text(CFG.replace("brownian-method lanczos", "brownian-method random") +
+                   "random-numvecs %d\n" % nv)
+    mod = tmp_path / "run.mod"
+    mod.write_text("par X 1.0 0.0\n")
+    sb = sdt.SimBox(cfg, mod, xyz)
+    sb.init()
+    sb.advance(4)
+    g = sdt.MobSpme(n, np.ones(n), box, 0.9, 5.0, 32, 6, sdt.FP64)
+    assert g.init()
+    g.update(pos)
+    M = g.mul_vector(np.eye(3 * n))
+    dim = 3 * n
+    Z = orc.gaussian(77, 0, 4 * dim).reshape(4, dim).T
+    omega = orc.gaussian(77 ^ 0x5678, 0, dim * nv).reshape(nv, dim).T
+    D = BR.apply(M, *BR.sample(M, omega), Z)
+    expect = pos + np.sqrt(2e-3) * D.sum(axis=1)
+    disp = sb.positions() - pos
+    assert np.abs(disp - (expect - pos)).max() < 1e-9 * np.abs(expect - pos).max()
+
