@@ -55,6 +55,7 @@ bool SpmeEngine::create(int npos_, const double* rdi, double box_, double xi_,
     std::vector<double> sv(porder - 1);
     for (int i = 0; i < porder - 1; i++) sv[i] = bspline(porder, i);
     std::vector<cplx> b(K);
+    std::vector<double> den2(K);
     for (int x = 0; x < K; x++) {
         double kx = map[x];
         cplx den(0.0, 0.0);
@@ -62,11 +63,19 @@ bool SpmeEngine::create(int npos_, const double* rdi, double box_, double xi_,
             double phi = i * kx * box / K;
             den += sv[i] * cplx(cos(phi), sin(phi));
         }
-        double phi0 = box * kx * (porder - 1) / K;
-        cplx bt(cos(phi0), sin(phi0));
-        double m2 = std::norm(den);
-        b[x] = cplx((bt.real() * den.real() + bt.imag() * den.imag()) / m2,
-                    (bt.imag() * den.real() - bt.real() * den.imag()) / m2);
+        den2[x] = std::norm(den);
+    }
+    // Deviation from src/mob/spme.cc:110-128 (a defect there): for odd
+    // porder the sum vanishes at the Nyquist index and rounding leaves
+    // |den|^2 ~ 1e-33, amplifying that plane by ~1e33. |den|^2 is replaced
+    // there by the mean of its neighbours (Essmann et al. 1995); even porder
+    // is unchanged (|den|^2 > 1e-3 everywhere).
+    for (int x = 0; x < K; x++)
+        if (den2[x] < 1e-7) den2[x] = 0.5 * (den2[(x + K - 1) % K] + den2[(x + 1) % K]);
+    for (int x = 0; x < K; x++) {
+        // only |b|^2 enters lm2: b = e^{i phi0} / den with |den|^2 as above
+        double phi0 = box * map[x] * (porder - 1) / K;
+        b[x] = cplx(cos(phi0), sin(phi0)) / sqrt(den2[x]);
     }
     const int Kh = K / 2 + 1;
     lm2.assign((size_t)K * K * Kh, 0.0);

@@ -1247,7 +1247,7 @@ static void b_moduli(int K, int P, std::vector<double>& out) {
         }
         sv[i] = a[i];   // M_P(1 + i)
     }
-    out.resize(K);
+    std::vector<double> den2(K);
     for (int m = 0; m < K; m++) {
         int mm = (m <= K / 2) ? m : m - K;
         double re = 0.0, im = 0.0;
@@ -1256,8 +1256,15 @@ static void b_moduli(int K, int P, std::vector<double>& out) {
             re += sv[i] * cos(ph);
             im += sv[i] * sin(ph);
         }
-        out[m] = 1.0 / (re * re + im * im);
+        den2[m] = re * re + im * im;
     }
+    // odd P: the sum vanishes at the Nyquist index (M_P is symmetric), and
+    // rounding leaves |den|^2 ~ 1e-33; take the mean of the neighbours there
+    // instead (Essmann et al. 1995). Even P is unaffected (|den|^2 > 1e-3).
+    for (int m = 0; m < K; m++)
+        if (den2[m] < 1e-7) den2[m] = 0.5 * (den2[(m + K - 1) % K] + den2[(m + 1) % K]);
+    out.resize(K);
+    for (int m = 0; m < K; m++) out[m] = 1.0 / den2[m];
 }
 
 bool make_fft_plan(int K, FFTPlan* plan) {
@@ -1650,6 +1657,12 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
                (size_t)c * 3 * porder * sizeof(T) + (size_t)c * 3 * sizeof(int) + 8 +
                (size_t)c * RB * 3 * sizeof(double);
     };
+    // narrower strips for wide stencils: the region must fit the prefetch
+    // registers (26 points per thread)
+    auto rreg_of = [&](int sx) {
+        return (int)ceil_div((size_t)(sx * ts + R) * (ts + R) * (ts + R), 256);
+    };
+    while (rreg_of(sxt) > 26 && sxt > 1) sxt /= 2;
     while (lds_of(sxt, ch) > 80 * 1024 && (sxt > 1 || ch > 32)) {
         if (sxt > 1) sxt /= 2;
         else ch /= 2;
@@ -1658,7 +1671,7 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
     SDT_REQUIRE(lds <= 80 * 1024, SDT_EINVAL, "interp: LDS region too large");
     const int nsx = (ntile + sxt - 1) / sxt;
     // region points per thread (registers of the prefetch)
-    const int rreg = (int)ceil_div((size_t)(sxt * ts + R) * (ts + R) * (ts + R), 256);
+    const int rreg = rreg_of(sxt);
     SDT_REQUIRE(rreg <= 26, SDT_EINVAL, "interp: region too large for the prefetch registers");
     dim3 grid(nsx * ntile * ntile, ceil_div(b, RB));
 #define SDT_INTERP_R(PP, RR)                                                                  \

@@ -36,6 +36,26 @@ def uniform_system(n, box, seed):
     return rng.uniform(0.0, box, 3 * n), np.ones(n)
 
 
+def poly_rsa_system(n, box, seed):
+    """Radii from the 15 sizes of poly_system, largest first, placed by
+    random sequential addition without overlaps (periodic): a system like the
+    reference's tests/data/N412.xyz, whose SPME test needs every overlapping
+    pair inside the real-space cutoff. Returns (flat AoS positions, radii)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.geomspace(21.4, 115.2, 15)
+    rdi = np.sort(rng.choice(sizes, size=n))[::-1].copy()
+    pos = np.zeros((n, 3))
+    for i in range(n):
+        while True:
+            p = rng.uniform(0.0, box, 3)
+            d = p - pos[:i]
+            d -= box * np.round(d / box)
+            if i == 0 or np.all(np.sqrt((d * d).sum(1)) > rdi[i] + rdi[:i]):
+                break
+        pos[i] = p
+    return pos.reshape(-1), rdi
+
+
 def poly_system(n, box, seed):
     """Polydisperse radii drawn from 15 sizes spanning 21..115 (the spread of
     tests/data/poly.mod), uniform positions."""

@@ -6,7 +6,7 @@ Ewald sum."""
 import numpy as np
 import pytest
 
-from conftest import uniform_system
+from conftest import poly_rsa_system, uniform_system
 
 pytestmark = pytest.mark.gpu
 
@@ -62,3 +62,33 @@ def test_ewald_matrix_matches_gpu_spme(sdt):
     f = np.random.default_rng(7).uniform(0, 1, (600, 3))
     v = g.mul_vector(f)
     assert np.linalg.norm(v - M @ f) / np.linalg.norm(M @ f) < 1e-7
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-5)])
+def test_spme_matches_ewald_n412_reference_params(sdt, prec, tol):
+    # tests/spme/test.sh (N412 poly, radii 21..115 in a box of 1000): dim 256,
+    # porder 8, xi 0.04, rmax 200.37188, alpha 2.1, beta 1.1, < 1e-6 (FP64)
+    pos, rdi = poly_rsa_system(412, 1000.0, 19)
+    # an explicit splitting: the default 10^(1/6) sqrt(pi) / L = 0.0026 needs
+    # ~3e5 k vectors per pair in this dilute box (the sum does not depend on it)
+    M = sdt.rpy_ewald_matrix(pos, rdi, 1000.0, 1e-12, "full", 0.01)
+    g = sdt.MobSpme(412, rdi, 1000.0, 0.04, 200.37188, 256, 8, prec)
+    assert g.init()
+    g.update(pos)
+    f = np.random.default_rng(7).uniform(0, 1, (1236, 3))
+    v = g.mul_vector(f, alpha=2.1, beta=1.1, v=f)
+    ve = 2.1 * (M @ f) + 1.1 * f
+    assert np.linalg.norm(v - ve) / np.linalg.norm(ve) < tol
+
+
+@pytest.mark.parametrize("p,tol", [(5, 2e-5), (7, 1e-6)])
+def test_odd_order_spme_matches_ewald(sdt, p, tol):
+    # odd porder with the Nyquist modulus taken as the neighbour mean
+    pos, rdi = uniform_system(200, 20.0, 12)
+    M = sdt.rpy_ewald_matrix(pos, rdi, 20.0, 1e-12)
+    g = sdt.MobSpme(200, rdi, 20.0, 0.8, 8.3, 128, p, sdt.FP64)
+    assert g.init()
+    g.update(pos)
+    f = np.random.default_rng(7).uniform(0, 1, (600, 3))
+    v = g.mul_vector(f)
+    assert np.linalg.norm(v - M @ f) / np.linalg.norm(M @ f) < tol

@@ -73,8 +73,9 @@ def test_fp64_matches_golden(sdt, orc):
             assert orc.frob_rel_err(G[tag + "_" + mode], out) < 1e-10, (tag, mode)
 
 
-@pytest.mark.parametrize("poly", [False, True])
-def test_fp32_within_1e5_of_fp64(sdt, orc, poly):
+@pytest.mark.parametrize("poly,p", [(False, 6), (True, 6), (False, 4), (False, 7), (True, 8)])
+def test_fp32_within_1e5_of_fp64(sdt, orc, poly, p):
+    # every spline order of the FP32 kernels (p = 7, 8 narrow the interp strip)
     n, box = 2000, 30.0
     pos, rdi = uniform_system(n, box, 7)
     if poly:
@@ -82,7 +83,7 @@ def test_fp32_within_1e5_of_fp64(sdt, orc, poly):
     f = np.random.default_rng(9).standard_normal((3 * n, 4))
     outs = []
     for prec in (sdt.FP64, sdt.FP32):
-        g = sdt.MobSpme(n, rdi, box, 0.9, 5.0, 64, 6, prec)
+        g = sdt.MobSpme(n, rdi, box, 0.9, 5.0, 64, p, prec)
         assert g.init()
         g.update(pos)
         outs.append(g.mul_vector(f))

@@ -50,6 +50,15 @@ def test_spme_matches_ewald_poly(orc):
     assert _spme_vs_ewald(orc, pos, rdi, 20.0, 0.8, 8.3, 128, 8) < 1e-7
 
 
+def test_odd_order_nyquist_fix_converges_to_ewald(orc):
+    # odd porder: the reference's B-spline modulus divides by ~1e-33 at the
+    # Nyquist index (src/mob/spme.cc:110-128); with the neighbour mean there
+    # SPME converges to the Ewald sum like the even orders around it
+    pos, rdi = uniform_system(200, 20.0, 12)
+    errs = {p: _spme_vs_ewald(orc, pos, rdi, 20.0, 0.8, 8.3, 128, p) for p in (6, 7, 8)}
+    assert errs[8] < errs[7] < errs[6] < 1e-5, errs
+
+
 def test_spme_converges_with_mesh(orc):
     pos, rdi = uniform_system(150, 18.0, 14)
     errs = [_spme_vs_ewald(orc, pos, rdi, 18.0, 1.0, 6.0, K, 6, nrhs=1) for K in (32, 64, 128)]
