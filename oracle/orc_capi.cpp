@@ -70,7 +70,8 @@ struct BDSim {
         if (!mob.init()) return false;
         if (!brwn.init()) return false;
         if (steric_pairs.init() <= 0) return false;
-        // the reference over-allocates by 3N (SURVEY defect list); M_I columns
+        // M_I columns (the reference allocates mob_interval * dim * ldm,
+        // src/simbox/simbox_bd.cc:171-175, 3N times what it uses)
         Z.assign((size_t)mob_interval * ldm, 0.0);
         D.assign((size_t)mob_interval * ldm, 0.0);
         f.assign(ldm, 0.0);
