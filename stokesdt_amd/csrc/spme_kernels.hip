@@ -351,7 +351,7 @@ constexpr int kPtMaxTiles = 200;   // contributor bucket tiles of one block (hos
 // scalar registers, so the FMAs take them as scalar operands and LDS serves
 // only the per-lane weights.
 template <typename T, int P, int NQ, bool BLK, bool WRAP, bool SGF = false>
-__global__ void __launch_bounds__(kPtThreads)
+__global__ void __launch_bounds__(kPtThreads) __attribute__((amdgpu_waves_per_eu(NQ <= 12 ? 6 : 1)))
 spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile_start,
                  const double* __restrict__ frac, const int* __restrict__ base,
                  const double* __restrict__ a2, const double* __restrict__ X,
@@ -1407,7 +1407,9 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
         // faster at K = 128; FP64 keeps them staged in LDS)
         const bool sgf = xb != nullptr && sizeof(T) == 4;
         const int rec = ((3 * (porder + 1) + 3 + 3) & ~3) + (sgf ? 0 : NQ);
-        const int cap = (int)(65536 / (rec * sizeof(T) + 9 * sizeof(int)));
+        // few meshes: a 48 KB stage leaves room for three workgroups per CU
+        const int stage_bytes = NQ <= 12 ? 48 * 1024 : 65536;
+        const int cap = (int)(stage_bytes / (rec * sizeof(T) + 9 * sizeof(int)));
         const size_t lds = (size_t)cap * (rec * sizeof(T) + 9 * sizeof(int));
         const int nsx = (K + 31) / 32;
         dim3 grid(nsx * (K / 4) * (K / 4), ceil_div(b, NQ / NC));
