@@ -29,6 +29,12 @@ int sdt_debug_band_sqrt(int m, int kd, const double* T, int kb, double* S);
 /* Spread one force vector (3*npos, reference layout) with an updated
  * mobility and return the three real meshes [c][z][y][x] (3*dim^3). */
 int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh);
+/* Interpolate three given real meshes [c][z][y][x] (3*dim^3) at the
+ * particles of an updated monodisperse mobility: v (3*npos, reference
+ * layout) = alpha * sum of the B-spline weights times the mesh values, the
+ * interpolation step of the reference's recip driver (src/mob/spme.cc:
+ * 163-198). */
+int sdt_debug_spme_interp(sdt_mob_t mob, const double* mesh, double alpha, double* v);
 /* Mean device time (ms, HIP events on the handle's stream) of each stage of
  * one packed matvec with b right-hand sides over `reps` repetitions:
  * t[0] spread, t[1] forward x/y FFT, t[2] z FFT + influence + inverse z,

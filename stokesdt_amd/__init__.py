@@ -102,6 +102,7 @@ SIGNATURES = {
     "sdt_debug_gram_groups": (_I, [_I64, _I, _I, _P, _P, _P]),
     "sdt_debug_band_sqrt": (_I, [_I, _I, _P, _I, _P]),
     "sdt_debug_spme_spread": (_I, [_P, _P, _P]),
+    "sdt_debug_spme_interp": (_I, [_P, _P, _D, _P]),
     "sdt_bench_stage_times": (_I, [_P, _I, _I, _P]),
     "sdt_bench_copy": (_I, [_I64, _I, ctypes.POINTER(_D)]),
     "sdt_bench_bd_steps": (_I, [_P, _I, ctypes.POINTER(_D)]),
@@ -224,6 +225,14 @@ class MobSpme:
         mesh = np.zeros(3 * K * K * K)
         _check(lib().sdt_debug_spme_spread(self.h, _ptr(f), _ptr(mesh)))
         return mesh.reshape(3, K, K, K)
+
+    def debug_interp(self, meshes, alpha=1.0):
+        """Interpolation of three meshes [3][K][K][K] at the particles
+        (monodisperse); returns 3N values in the reference layout."""
+        m = _f64(np.asarray(meshes).reshape(-1))
+        v = np.zeros(3 * self.npos)
+        _check(lib().sdt_debug_spme_interp(self.h, _ptr(m), alpha, _ptr(v)))
+        return v
 
     def stage_times(self, b, reps=5):
         t = np.zeros(7)

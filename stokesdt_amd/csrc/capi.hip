@@ -629,6 +629,44 @@ int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh) {
     });
 }
 
+int sdt_debug_spme_interp(sdt_mob_t mob, const double* mesh, double alpha, double* v) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        MobSpmeGpu* m = mob->m;
+        SDT_REQUIRE(m->updated(), SDT_ESTATE, "mobility not updated");
+        const int n3 = m->dim();
+        const size_t K3 = (size_t)m->K() * m->K() * m->K();
+        DevBuf<double> dY, dv;
+        dY.reserve(n3);
+        dv.reserve(n3);
+        hipStream_t s = m->stream();
+        // interp applies alpha / L^3 (the mesh normalisation of the matvec)
+        const double L = m->box();
+        const double a = alpha * L * L * L;
+        if (m->precision() == kFP32) {
+            SpmeGpu<float>* e = m->spme32();
+            SDT_REQUIRE(!e->poly, SDT_EINVAL, "interp probe: monodisperse only");
+            e->reserve_rhs(1);
+            std::vector<float> tmp(3 * K3);
+            for (size_t i = 0; i < 3 * K3; i++) tmp[i] = (float)mesh[i];
+            SDT_HIP_CHECK(hipMemcpyAsync(e->d_real.get(), tmp.data(), sizeof(float) * 3 * K3,
+                                         hipMemcpyHostToDevice, s));
+            e->interp(1, a, 0.0, dY.get(), s);
+            SDT_HIP_CHECK(hipStreamSynchronize(s));
+        } else {
+            SpmeGpu<double>* e = m->spme64();
+            SDT_REQUIRE(!e->poly, SDT_EINVAL, "interp probe: monodisperse only");
+            e->reserve_rhs(1);
+            SDT_HIP_CHECK(hipMemcpyAsync(e->d_real.get(), mesh, sizeof(double) * 3 * K3,
+                                         hipMemcpyHostToDevice, s));
+            e->interp(1, a, 0.0, dY.get(), s);
+        }
+        unpack(m->npos(), 1, m->perm(), n3, dY.get(), dv.get(), s);
+        SDT_HIP_CHECK(hipMemcpyAsync(v, dv.get(), sizeof(double) * n3, hipMemcpyDeviceToHost, s));
+        SDT_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
 }  // extern "C"
 
 template <typename T>

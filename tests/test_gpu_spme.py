@@ -104,6 +104,23 @@ def test_spread_mesh_matches_oracle(sdt, orc, prec, tol):
     assert np.abs(g.debug_spread(f) - ref).max() <= tol * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("prec,tol,K,p", [(0, 1e-12, 32, 6), (1, 1e-6, 32, 6),
+                                            (0, 1e-12, 64, 7), (1, 1e-6, 48, 8)])
+def test_interp_matches_oracle(sdt, orc, prec, tol, K, p):
+    # the interpolation step alone (src/mob/spme.cc:163-198) on random meshes
+    n, box = 300, 20.0
+    pos, rdi = uniform_system(n, box, 3)
+    g = sdt.MobSpme(n, rdi, box, 0.9, 5.0, K, p, prec)
+    assert g.init()
+    g.update(pos)
+    e = orc.SpmeEngine(n, None, box, 0.9, K, p)
+    e.update(pos)
+    meshes = np.random.default_rng(5).standard_normal((3, K, K, K))
+    ref = e.interp(meshes, 1.7)
+    out = g.debug_interp(meshes, 1.7)
+    assert np.abs(out - ref).max() <= tol * np.abs(ref).max()
+
+
 def test_rpy_spme_one_shot(sdt, orc):
     n, box = 200, 18.0
     pos, rdi = uniform_system(n, box, 5)
