@@ -29,6 +29,33 @@ def sdt():
     return S
 
 
+REF_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_data")
+
+
+def reference_system(xyz, mod):
+    """One of the reference's own test inputs (tests/data/*.xyz with
+    mono.mod / poly.mod, committed under tests/golden/reference_data):
+    returns (flat AoS positions, radii, box). The model's `par name radius`
+    lines give the radii; types without a line have radius 1 (the XYZ reader
+    and model rules of src/util/molecule_io.cc:39-231)."""
+    par = {}
+    with open(os.path.join(REF_DATA, mod)) as fh:
+        for line in fh:
+            t = line.split()
+            if len(t) >= 3 and t[0] == "par":
+                par[t[1]] = float(t[2])
+    with open(os.path.join(REF_DATA, xyz)) as fh:
+        lines = fh.read().splitlines()
+    n = int(lines[0])
+    box = float(lines[1].split()[2])
+    pos, rdi = [], []
+    for line in lines[2:2 + n]:
+        t = line.split()
+        rdi.append(par.get(t[0], 1.0))
+        pos.extend(float(v) for v in t[1:4])
+    return np.array(pos), np.array(rdi), box
+
+
 def uniform_system(n, box, seed):
     """Uniform positions in [0, box)^3 (overlaps allowed, like the
     reference's tests/data/N1000_Phi0.1.xyz), unit radii."""

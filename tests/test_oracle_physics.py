@@ -6,13 +6,21 @@ Reference tests restated here: tests/spme/testspme.cc (SPME vs dense Ewald
 < 1e-6 at the parameters of tests/spme/test.sh:2-9), tests/steric/
 teststeric.cc (cell list vs O(N^2) DirectSteric < 1e-6, :34-79,155),
 tests/lanczos/testlanczos.cc (single and block Lanczos vs dense eigen-sqrt
-< 1e-6 at tol 1e-7, :81-144). The reference's input files are not copied:
-equivalent systems are generated from seeds (tests/conftest.py).
+< 1e-6 at tol 1e-7, :81-144). Where a relation is stated on the
+reference's own inputs (tests/data/N1000_Phi0.1.xyz, N412.xyz with mono.mod
+/ poly.mod), the committed copies under tests/golden/reference_data are
+read (conftest.reference_system); other systems are generated from seeds.
+
+The overlap correction (r < a_i + a_j) and the unequal-radius a^2 terms of
+the real-space RPY scalars are pinned against the free-space
+Rotne-Prager-Yamakawa tensor (Rotne & Prager 1969; Yamakawa 1970;
+overlapping equal spheres as in Wajnryb et al. 2013), which the periodic
+sums approach as 1/L in growing boxes.
 """
 import numpy as np
 import pytest
 
-from conftest import poly_system, uniform_system
+from conftest import poly_system, reference_system, uniform_system
 
 L1000 = 34.729314   # box of the reference's N1000 phi=0.1 system
 
@@ -31,9 +39,19 @@ def _spme_vs_ewald(orc, pos, rdi, box, xi, rmax, K, p, nrhs=3, seed=7):
 
 @pytest.mark.slow
 def test_spme_matches_ewald_reference_params_mono(orc):
-    # tests/spme/test.sh:2-3: N1000 phi 0.1, dim 256, porder 6, xi 0.8, rmax 8.371887
-    pos, rdi = uniform_system(1000, L1000, 11)
-    err = _spme_vs_ewald(orc, pos, rdi, L1000, 0.8, 8.371887, 256, 6, nrhs=1)
+    # tests/spme/test.sh:2-3 on the reference's own input: N1000_Phi0.1.xyz
+    # with mono.mod, dim 256, porder 6, xi 0.8, rmax 8.371887
+    pos, rdi, box = reference_system("N1000_Phi0.1.xyz", "mono.mod")
+    err = _spme_vs_ewald(orc, pos, rdi, box, 0.8, 8.371887, 256, 6, nrhs=1)
+    assert err < 1e-6, err
+
+
+@pytest.mark.slow
+def test_spme_matches_ewald_reference_params_poly(orc):
+    # tests/spme/test.sh:8-9 on the reference's own input: N412.xyz with
+    # poly.mod, dim 256, porder 8, xi 0.04, rmax 200.37188
+    pos, rdi, box = reference_system("N412.xyz", "poly.mod")
+    err = _spme_vs_ewald(orc, pos, rdi, box, 0.04, 200.37188, 256, 8, nrhs=2)
     assert err < 1e-6, err
 
 
@@ -93,6 +111,82 @@ def test_ewald_free_space_limit(orc):
         assert abs(dev[0][c] / dev[1][c] - 2.0) < 0.1
         assert abs(dev[1][c] / dev[2][c] - 2.0) < 0.05
     assert abs(dev[2][0]) < 0.05 * xa and abs(dev[2][1]) < 0.05 * ya
+
+
+def _pair_block(orc, r, ai, aj, box, spme=None):
+    """Off-diagonal 3x3 block of the periodic mobility of two spheres at
+    separation r along x in a cube of side `box`: from the dense Ewald sum,
+    or from the SPME operator when spme = (xi, rmax, K, p)."""
+    pos = np.array([0.3 * box, 0.5 * box, 0.5 * box, 0.3 * box + r, 0.5 * box, 0.5 * box])
+    rdi = np.array([ai, aj])
+    if spme is None:
+        M = orc.ewald_matrix(pos, rdi, box, tol=1e-14)
+        return M[0:3, 3:6]
+    xi, rmax, K, p = spme
+    m = orc.MobSpme(2, rdi, box, xi, rmax, K, p)
+    assert m.init()
+    m.update(pos, rdi)
+    f = np.zeros((6, 3))
+    f[3:6, :] = np.eye(3)
+    return m.mul_vector(f)[0:3, :]
+
+
+def _free_space_rpy(r, ai, aj):
+    """(xa, ya) of the free-space RPY tensor, in units where the self
+    mobility of a sphere of radius a is 1/a (6 pi eta = 1). Far apart:
+    xa = 3/(2r) - (ai^2 + aj^2)/(2 r^3), ya = 3/(4r) + (ai^2 + aj^2)/(4 r^3).
+    Overlapping equal spheres (r < 2a): (1/a)(1 - 9r/(32a)) I +
+    (1/a)(3r/(32a)) e e, i.e. xa = (1/a)(1 - 3r/(16a)),
+    ya = (1/a)(1 - 9r/(32a))."""
+    if r >= ai + aj:
+        s = ai * ai + aj * aj
+        return 1.5 / r - s / (2 * r ** 3), 0.75 / r + s / (4 * r ** 3)
+    assert ai == aj
+    a = ai
+    return (1.0 - 3.0 * r / (16.0 * a)) / a, (1.0 - 9.0 * r / (32.0 * a)) / a
+
+
+def _check_one_over_l(boxes, devs, xa, ya, tol):
+    # the leading periodic correction is c1/L (Hasimoto's -2.837/L on the
+    # diagonal): doubling L about halves the deviation. Fitting
+    # d(L) = d0 + c1/L + c3/L^3 through the three boxes removes the periodic
+    # terms up to O(1/L^5); d0 must vanish.
+    for c in range(2):
+        assert abs(devs[0][c] / devs[1][c] - 2.0) < 0.25, devs
+        assert abs(devs[1][c] / devs[2][c] - 2.0) < 0.1, devs
+    A = np.array([[1.0, 1.0 / L, 1.0 / L ** 3] for L in boxes])
+    d0 = [np.linalg.solve(A, np.array([d[c] for d in devs]))[0] for c in range(2)]
+    assert abs(d0[0]) < tol * abs(xa) and abs(d0[1]) < tol * abs(ya), (d0, xa, ya)
+
+
+@pytest.mark.parametrize("r,ai,aj", [(1.5, 1.0, 1.0), (0.5, 1.0, 1.0), (3.0, 2.0, 2.0),
+                                     (12.0, 1.0, 2.0), (9.0, 0.5, 3.0)])
+def test_ewald_pair_tends_to_free_space_rpy(orc, r, ai, aj):
+    # overlap branch (src/mob/ewald.cc:177-195) for r < ai + aj and the
+    # unequal-radius a^2 terms of scalars_ewald_F (src/mob/ewald.h:55-85)
+    xa, ya = _free_space_rpy(r, ai, aj)
+    devs = []
+    boxes = (200.0, 400.0, 800.0)
+    for box in boxes:
+        blk = _pair_block(orc, r, ai, aj, box)
+        assert abs(blk[0, 1]) < 1e-12 and abs(blk[1, 2]) < 1e-12 and abs(blk[0, 2]) < 1e-12
+        assert abs(blk[1, 1] - blk[2, 2]) < 1e-12
+        devs.append((blk[0, 0] - xa, blk[1, 1] - ya))
+    _check_one_over_l(boxes, devs, xa, ya, 1e-6)
+
+
+@pytest.mark.parametrize("r,ai,aj", [(1.5, 1.0, 1.0), (3.0, 2.0, 2.0), (6.0, 1.0, 2.0)])
+def test_spme_pair_tends_to_free_space_rpy(orc, r, ai, aj):
+    # the SPME operator's own real-space copy of the overlap branch and the
+    # a^2 terms (src/mob/mob_spme.cc:151-197), in growing boxes; xi and the
+    # mesh are chosen so that both Ewald parts are converged (< 1e-9)
+    xa, ya = _free_space_rpy(r, ai, aj)
+    devs = []
+    boxes = (40.0, 80.0, 160.0)
+    for box, K in zip(boxes, (64, 128, 256)):
+        blk = _pair_block(orc, r, ai, aj, box, spme=(0.35, 14.0, K, 8))
+        devs.append((blk[0, 0] - xa, blk[1, 1] - ya))
+    _check_one_over_l(boxes, devs, xa, ya, 1e-4)
 
 
 def test_spme_operator_symmetric(orc):
@@ -202,10 +296,11 @@ def _direct_steric(pos, rdi, box, cutoff, k0):
 
 
 def test_steric_matches_direct(orc):
-    # tests/steric/test.sh:2: polydisperse, cutoff 2.3, k0 125
-    pos, rdi = poly_system(412, 1000.0, 19)
-    f = orc.steric(pos, rdi, 1000.0, 2.3, 125.0)
-    fd = _direct_steric(pos, rdi, 1000.0, 2.3, 125.0)
+    # tests/steric/test.sh:2 on the reference's own input: N412.xyz with
+    # poly.mod, cutoff 2.3, k0 125
+    pos, rdi, box = reference_system("N412.xyz", "poly.mod")
+    f = orc.steric(pos, rdi, box, 2.3, 125.0)
+    fd = _direct_steric(pos, rdi, box, 2.3, 125.0)
     assert np.linalg.norm(f) > 0
     assert np.linalg.norm(f - fd) / np.linalg.norm(fd) < 1e-6
     # Newton's third law
@@ -214,8 +309,9 @@ def test_steric_matches_direct(orc):
 
 @pytest.fixture(scope="module")
 def ewald1000(orc):
-    pos, rdi = uniform_system(1000, L1000, 20)
-    return orc.ewald_matrix(pos, rdi, L1000, tol=1e-12)
+    # testlanczos.cc builds MobEwald on N1000_Phi0.1.xyz with mono.mod
+    pos, rdi, box = reference_system("N1000_Phi0.1.xyz", "mono.mod")
+    return orc.ewald_matrix(pos, rdi, box, tol=1e-12)
 
 
 def _eig_sqrt(M):
