@@ -64,6 +64,45 @@ int sdt_bd_profile_columns(sdt_bd_t bd, int64_t* columns, int nstages);
 int sdt_util_rsa_positions(int n, double box, double a, uint64_t seed, int64_t max_attempts,
                            double* pos, int64_t* placed);
 
+/* Plain FP64 GPU path: the reference algorithm on the device as it is
+ * structured on the CPU, the baseline of the FP32-mixed path. MobSpme
+ * (src/mob/mob_spme.cc:53-262) with a per-RHS reciprocal loop
+ * (src/mob/spme.cc:714-839), stored N x p^3 weight and index tables
+ * (:366-509, :625-633), spread by FP64 global atomics, three R2C and three
+ * C2R library FFTs per RHS, a separate influence kernel (:201-310) and
+ * interpolation kernel (:163-198), and stored FP64 3x3 real-space blocks
+ * (src/mob/mob_spme.cc:135-202) applied by SpMV3x3 (src/util/sparse.cc:
+ * 79-153); BlockLanczos (src/brwn/brwn_lanczos.cc:85-201) group after group
+ * with Householder QR and Y = V S R every iteration; the BD step of
+ * src/simbox/simbox_bd.cc:296-366. FP64 only; host pointers. create also
+ * runs the reference's Init checks. mode: 0 full, 1 real, 2 recip. */
+typedef struct sdt_plain_mob_s* sdt_plain_mob_t;
+typedef struct sdt_plain_bd_s* sdt_plain_bd_t;
+int sdt_plain_mob_create(int npos, const double* rdi, double box_size, double xi, double rmax,
+                         int dim, int porder, sdt_plain_mob_t* out);
+int sdt_plain_mob_update(sdt_plain_mob_t mob, const double* pos);
+int sdt_plain_mob_mulvector(sdt_plain_mob_t mob, int mode, int num_rhs, double alpha, int ldf,
+                            const double* f, double beta, int ldv, double* v);
+/* y ~= M^{1/2} z by BlockLanczos in groups of max_nrhs on the plain
+ * mobility; iters[g] = iteration of convergence of group g (-1 if none). */
+int sdt_plain_lanczos(sdt_plain_mob_t mob, int max_iters, int max_nrhs, double tol, int num_rhs,
+                      int ldz, const double* z, int ldy, double* y, int* iters, int cap,
+                      int* ngroups);
+int sdt_plain_mob_destroy(sdt_plain_mob_t mob);
+int sdt_plain_bd_create(int npos, const double* pos, const double* rdi, double box_size, double xi,
+                        double rmax, int dim, int porder, int max_iters, int max_nrhs, double tol,
+                        double steric_r0, double steric_k0, double dt, int mob_interval,
+                        uint64_t seed, sdt_plain_bd_t* out);
+int sdt_plain_bd_init(sdt_plain_bd_t bd);
+int sdt_plain_bd_advance(sdt_plain_bd_t bd, int nsteps);
+int sdt_plain_bd_get_positions(sdt_plain_bd_t bd, double* pos);
+int sdt_plain_bd_last_iters(sdt_plain_bd_t bd, int* iters, int cap, int* ngroups);
+/* stages as sdt_bd_profile */
+int sdt_plain_bd_set_profiling(sdt_plain_bd_t bd, int on);
+int sdt_plain_bd_profile(sdt_plain_bd_t bd, double* sum_ms, int64_t* counts, int64_t* columns,
+                         int nstages);
+int sdt_plain_bd_destroy(sdt_plain_bd_t bd);
+
 #ifdef __cplusplus
 }
 #endif

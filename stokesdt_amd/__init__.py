@@ -110,6 +110,20 @@ SIGNATURES = {
     "sdt_bd_profile": (_I, [_P, _P, _P, _I]),
     "sdt_bd_profile_columns": (_I, [_P, _P, _I]),
     "sdt_util_rsa_positions": (_I, [_I, _D, _D, ctypes.c_uint64, _I64, _P, _PI64]),
+    "sdt_plain_mob_create": (_I, [_I, _P, _D, _D, _D, _I, _I, ctypes.POINTER(_P)]),
+    "sdt_plain_mob_update": (_I, [_P, _P]),
+    "sdt_plain_mob_mulvector": (_I, [_P, _I, _I, _D, _I, _P, _D, _I, _P]),
+    "sdt_plain_lanczos": (_I, [_P, _I, _I, _D, _I, _I, _P, _I, _P, _P, _I, _PI]),
+    "sdt_plain_mob_destroy": (_I, [_P]),
+    "sdt_plain_bd_create": (_I, [_I, _P, _P, _D, _D, _D, _I, _I, _I, _I, _D, _D, _D, _D, _I,
+                                 ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "sdt_plain_bd_init": (_I, [_P]),
+    "sdt_plain_bd_advance": (_I, [_P, _I]),
+    "sdt_plain_bd_get_positions": (_I, [_P, _P]),
+    "sdt_plain_bd_last_iters": (_I, [_P, _P, _I, _PI]),
+    "sdt_plain_bd_set_profiling": (_I, [_P, _I]),
+    "sdt_plain_bd_profile": (_I, [_P, _P, _P, _P, _I]),
+    "sdt_plain_bd_destroy": (_I, [_P]),
 }
 
 STAGES = ["spread", "fft_xy_fwd", "fft_z_influence", "fft_xy_inv", "interp", "real_space",
@@ -621,3 +635,95 @@ def rsa_positions(n, box, a=1.0, seed=0, max_attempts=None):
     _check(lib().sdt_util_rsa_positions(n, box, a, seed, max_attempts, _ptr(pos),
                                         ctypes.byref(placed)))
     return pos
+
+
+class PlainMobSpme:
+    """The plain FP64 GPU form of MobSpme (per-RHS loop, stored P/ind tables,
+    atomic spread, library FFTs, stored 3x3 blocks + SpMV): the baseline of
+    the optimised path. create runs the reference's Init checks."""
+
+    def __init__(self, npos, rdi, box_size, xi, rmax, dim, porder):
+        self.npos = int(npos)
+        self.rdi = _f64(rdi)
+        h = _P()
+        _check(lib().sdt_plain_mob_create(self.npos, _ptr(self.rdi), box_size, xi, rmax, dim,
+                                          porder, ctypes.byref(h)))
+        self.h = h
+
+    def update(self, pos):
+        pos = _f64(pos)
+        _check(lib().sdt_plain_mob_update(self.h, _ptr(pos)))
+
+    def mul_vector(self, f, alpha=1.0, beta=0.0, v=None, mode=0):
+        f = _colmajor(f)
+        n, nrhs = f.shape
+        out = np.zeros((n, nrhs), order="F") if v is None else _colmajor(v).copy(order="F")
+        _check(lib().sdt_plain_mob_mulvector(self.h, mode, nrhs, alpha, n, _ptr(f), beta, n,
+                                             _ptr(out)))
+        return out
+
+    def lanczos(self, z, max_iters, max_nrhs, tol):
+        """(y ~= M^{1/2} z, iteration of convergence per group) by the
+        reference's BlockLanczos, group after group."""
+        z = _colmajor(z)
+        n, nrhs = z.shape
+        y = np.zeros((n, nrhs), order="F")
+        it = np.zeros(4096, dtype=np.int32)
+        ng = ctypes.c_int(0)
+        _check(lib().sdt_plain_lanczos(self.h, max_iters, max_nrhs, tol, nrhs, n, _ptr(z), n,
+                                       _ptr(y), _ptr(it), 4096, ctypes.byref(ng)))
+        return y, it[:ng.value].copy()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_plain_mob_destroy(self.h)
+            self.h = None
+
+
+class PlainBDSim:
+    """BDSimBox::Advance (src/simbox/simbox_bd.cc:296-366) on the plain FP64
+    GPU path."""
+
+    def __init__(self, pos, rdi, box_size, xi, rmax, dim, porder, max_iters, max_nrhs, tol,
+                 steric_r0, steric_k0, dt, mob_interval, seed):
+        pos = _f64(pos)
+        self.rdi = _f64(rdi)
+        self.npos = self.rdi.size
+        h = _P()
+        _check(lib().sdt_plain_bd_create(self.npos, _ptr(pos), _ptr(self.rdi), box_size, xi, rmax,
+                                         dim, porder, max_iters, max_nrhs, tol, steric_r0,
+                                         steric_k0, dt, mob_interval, seed, ctypes.byref(h)))
+        self.h = h
+
+    def init(self):
+        _check(lib().sdt_plain_bd_init(self.h))
+        return True
+
+    def advance(self, nsteps):
+        _check(lib().sdt_plain_bd_advance(self.h, nsteps))
+
+    def positions(self):
+        out = np.zeros(3 * self.npos)
+        _check(lib().sdt_plain_bd_get_positions(self.h, _ptr(out)))
+        return out
+
+    def last_iters(self):
+        out = np.zeros(4096, dtype=np.int32)
+        ng = ctypes.c_int(0)
+        _check(lib().sdt_plain_bd_last_iters(self.h, _ptr(out), 4096, ctypes.byref(ng)))
+        return out[:ng.value].copy()
+
+    def set_profiling(self, on):
+        _check(lib().sdt_plain_bd_set_profiling(self.h, 1 if on else 0))
+
+    def profile(self):
+        ms = np.zeros(len(STAGES))
+        cnt = np.zeros(len(STAGES), dtype=np.int64)
+        cols = np.zeros(len(STAGES), dtype=np.int64)
+        _check(lib().sdt_plain_bd_profile(self.h, _ptr(ms), _ptr(cnt), _ptr(cols), len(STAGES)))
+        return {name: (float(ms[i]), int(cnt[i]), int(cols[i])) for i, name in enumerate(STAGES)}
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_plain_bd_destroy(self.h)
+            self.h = None

@@ -27,6 +27,7 @@ HIP_SOURCES = [
     ("simbox.hip", []),
     ("diffusion.hip", []),
     ("brwn_random.hip", []),
+    ("plain_fp64.hip", []),
     ("capi.hip", []),
 ]
 HOST_SOURCES = [("host_linalg.cpp", []), ("host_util.cpp", []), ("molecule_io.cpp", [])]
@@ -66,7 +67,7 @@ def build(verbose=False, jobs=8):
                 print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+         ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     # command-line BD driver over the C ABI
     exe_tmp = EXE + ".tmp"

@@ -7,6 +7,7 @@
 #include "brwn_random.hpp"
 #include "diffusion.hpp"
 #include "mob_gpu.hpp"
+#include "plain_gpu.hpp"
 #include "simbox.hpp"
 
 namespace sdt {
@@ -32,6 +33,11 @@ struct sdt_pairlist_s {
 struct sdt_force_s { StericGpu* f; hipStream_t s; };
 struct sdt_bd_s { BDGpu* bd; };
 struct sdt_simbox_s { SimBoxGpu* sb; };
+struct sdt_plain_mob_s {
+    PlainMobGpu* m;
+    DevBuf<double> d_pos, d_f, d_v;
+};
+struct sdt_plain_bd_s { PlainBDGpu* bd; };
 
 template <typename F>
 static int guard(F&& fn) {
@@ -824,6 +830,139 @@ int sdt_util_rsa_positions(int n, double box, double a, uint64_t seed, int64_t m
     if (placed) *placed = k;
     if (k < n) { set_error("random sequential addition ran out of attempts"); return SDT_ECAP; }
     return SDT_OK;
+}
+
+// ------------------------------------------------------ plain FP64 path --
+int sdt_plain_mob_create(int npos, const double* rdi, double box_size, double xi, double rmax,
+                         int dim, int porder, sdt_plain_mob_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] {
+        PlainMobGpu* m = new PlainMobGpu(npos, rdi, box_size, xi, rmax, dim, porder);
+        try {
+            m->init();
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = new sdt_plain_mob_s{m, {}, {}, {}};
+    });
+}
+int sdt_plain_mob_update(sdt_plain_mob_t mob, const double* pos) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        SDT_REQUIRE(pos != nullptr, SDT_EINVAL, "pos is null");
+        const size_t n = (size_t)mob->m->dim();
+        mob->d_pos.reserve(n);
+        SDT_HIP_CHECK(hipMemcpy(mob->d_pos.get(), pos, sizeof(double) * n, hipMemcpyHostToDevice));
+        mob->m->update(mob->d_pos.get());
+    });
+}
+int sdt_plain_mob_mulvector(sdt_plain_mob_t mob, int mode, int num_rhs, double alpha, int ldf,
+                            const double* f, double beta, int ldv, double* v) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        SDT_REQUIRE(f && v && num_rhs >= 1, SDT_EINVAL, "null vector or no right-hand side");
+        const int n = mob->m->dim();
+        SDT_REQUIRE(ldf >= n && ldv >= n, SDT_EINVAL, "leading dimensions must be at least 3*npos");
+        mob->d_f.reserve((size_t)n * num_rhs);
+        mob->d_v.reserve((size_t)n * num_rhs);
+        SDT_HIP_CHECK(hipMemcpy2D(mob->d_f.get(), sizeof(double) * n, f, sizeof(double) * ldf,
+                                  sizeof(double) * n, num_rhs, hipMemcpyHostToDevice));
+        SDT_HIP_CHECK(hipMemcpy2D(mob->d_v.get(), sizeof(double) * n, v, sizeof(double) * ldv,
+                                  sizeof(double) * n, num_rhs, hipMemcpyHostToDevice));
+        mob->m->mul_vector(mode, num_rhs, alpha, n, mob->d_f.get(), beta, n, mob->d_v.get());
+        SDT_HIP_CHECK(hipStreamSynchronize(mob->m->stream()));
+        SDT_HIP_CHECK(hipMemcpy2D(v, sizeof(double) * ldv, mob->d_v.get(), sizeof(double) * n,
+                                  sizeof(double) * n, num_rhs, hipMemcpyDeviceToHost));
+    });
+}
+int sdt_plain_lanczos(sdt_plain_mob_t mob, int max_iters, int max_nrhs, double tol, int num_rhs,
+                      int ldz, const double* z, int ldy, double* y, int* iters, int cap,
+                      int* ngroups) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        SDT_REQUIRE(z && y, SDT_EINVAL, "null vector");
+        const int n = mob->m->dim();
+        SDT_REQUIRE(num_rhs >= 1 && ldz >= n && ldy >= n, SDT_EINVAL, "bad shapes");
+        PlainLanczosGpu lz(n, max_iters, max_nrhs, tol);
+        lz.init();
+        DevBuf<double> dz, dy;
+        dz.reserve((size_t)n * num_rhs);
+        dy.reserve((size_t)n * num_rhs);
+        SDT_HIP_CHECK(hipMemcpy2D(dz.get(), sizeof(double) * n, z, sizeof(double) * ldz,
+                                  sizeof(double) * n, num_rhs, hipMemcpyHostToDevice));
+        lz.compute(mob->m, num_rhs, n, dz.get(), n, dy.get());
+        SDT_HIP_CHECK(hipMemcpy2D(y, sizeof(double) * ldy, dy.get(), sizeof(double) * n,
+                                  sizeof(double) * n, num_rhs, hipMemcpyDeviceToHost));
+        const std::vector<int>& it = lz.iters();
+        if (ngroups) *ngroups = (int)it.size();
+        for (int i = 0; i < (int)it.size() && i < cap && iters; i++) iters[i] = it[i];
+    });
+}
+int sdt_plain_mob_destroy(sdt_plain_mob_t mob) {
+    if (!mob) return SDT_OK;
+    return guard([&] {
+        delete mob->m;
+        delete mob;
+    });
+}
+
+int sdt_plain_bd_create(int npos, const double* pos, const double* rdi, double box_size, double xi,
+                        double rmax, int dim, int porder, int max_iters, int max_nrhs, double tol,
+                        double steric_r0, double steric_k0, double dt, int mob_interval,
+                        uint64_t seed, sdt_plain_bd_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] {
+        PlainBDGpu* bd = new PlainBDGpu(npos, pos, rdi, box_size, xi, rmax, dim, porder, max_iters,
+                                        max_nrhs, tol, steric_r0, steric_k0, dt, mob_interval, seed);
+        *out = new sdt_plain_bd_s{bd};
+    });
+}
+int sdt_plain_bd_init(sdt_plain_bd_t bd) {
+    SDT_CHECK_HANDLE(bd);
+    return guard([&] { bd->bd->init(); });
+}
+int sdt_plain_bd_advance(sdt_plain_bd_t bd, int nsteps) {
+    SDT_CHECK_HANDLE(bd);
+    return guard([&] { bd->bd->advance(nsteps); });
+}
+int sdt_plain_bd_get_positions(sdt_plain_bd_t bd, double* pos) {
+    SDT_CHECK_HANDLE(bd);
+    return guard([&] { bd->bd->get_positions(pos); });
+}
+int sdt_plain_bd_last_iters(sdt_plain_bd_t bd, int* iters, int cap, int* ngroups) {
+    SDT_CHECK_HANDLE(bd);
+    const std::vector<int>& it = bd->bd->last_iters();
+    if (ngroups) *ngroups = (int)it.size();
+    for (int i = 0; i < (int)it.size() && i < cap && iters; i++) iters[i] = it[i];
+    return SDT_OK;
+}
+int sdt_plain_bd_set_profiling(sdt_plain_bd_t bd, int on) {
+    SDT_CHECK_HANDLE(bd);
+    StageTimer& t = bd->bd->timer();
+    t.on = on != 0;
+    if (t.on) t.reset();
+    return SDT_OK;
+}
+int sdt_plain_bd_profile(sdt_plain_bd_t bd, double* sum_ms, int64_t* counts, int64_t* columns,
+                         int nstages) {
+    SDT_CHECK_HANDLE(bd);
+    StageTimer& t = bd->bd->timer();
+    for (int i = 0; i < nstages && i < StageTimer::kStages; i++) {
+        if (sum_ms) sum_ms[i] = t.sum_ms[i];
+        if (counts) counts[i] = t.count[i];
+        if (columns) columns[i] = t.cols[i];
+    }
+    return SDT_OK;
+}
+int sdt_plain_bd_destroy(sdt_plain_bd_t bd) {
+    if (!bd) return SDT_OK;
+    return guard([&] {
+        delete bd->bd;
+        delete bd;
+    });
 }
 
 }  // extern "C"

@@ -1,0 +1,36 @@
+// Device helpers of the real-space RPY-Ewald operator shared by the
+// matrix-free matvec, the dense Ewald matrix and the stored-block path:
+// the min-image squared distance and the scalars xa, ya of one pair
+// (scalars_ewald_F, src/mob/ewald.h:55-85).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sdt {
+
+// squared min-image distance with explicit fused operations, so that every
+// kernel that classifies a pair as overlapping rounds identically
+__device__ __forceinline__ double pair_rr(double dx, double dy, double dz) {
+    return fma(dx, dx, fma(dy, dy, dz * dz));
+}
+
+template <typename T>
+__device__ __forceinline__ void ewald_scalars(T xi, T r, T aa, T ab, T* xa, T* ya) {
+    const T xir = xi * r;
+    const T xir2 = xir * xir;
+    const T r2 = r * r;
+    const T r3 = r2 * r;
+    const T aa2 = aa * aa, ab2 = ab * ab;
+    const T erfcxir = erfc(xir);
+    const T expxir2 = xi * T(0.56418958354775628694807945156077) * exp(-xir2);  // xi/sqrt(pi)
+    const T a0 = T(2) * (T(-3) + xir2 * T(2)) * expxir2 + erfcxir / r;
+    const T b0 = T(2) * (T(1) + xir2 * T(-2)) * expxir2 + erfcxir / r;
+    const T a2 = T(4) / r2 * (T(1) + xir2 * (T(14) + xir2 * (T(-20) + xir2 * T(4)))) * expxir2 +
+                 T(2) / r3 * erfcxir;
+    const T b2 = T(-4) / r2 * (T(3) + xir2 * (T(2) + xir2 * (T(-16) + xir2 * T(4)))) * expxir2 -
+                 T(6) / r3 * erfcxir;
+    const T s6 = (aa2 + ab2) / T(6);
+    *ya = T(0.75) * (a0 + a2 * s6);
+    *xa = T(0.75) * (b0 + b2 * s6) + *ya;
+}
+
+}  // namespace sdt

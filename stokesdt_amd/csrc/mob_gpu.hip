@@ -222,7 +222,6 @@ struct LzGroup {
     int c0 = 0, b = 0;
     bool done = false;
     int it = -1;                  // iteration of convergence, -1 if none
-    int start = 1;                // first iteration whose criterion is evaluated
     double normz = 0.0;           // ||Z_g||_F
     std::vector<double> R, H, B;  // b x b row-major: Z = V_0 R, H_i, B_i
     std::vector<double> Tb;       // block tridiagonal T in band storage
@@ -250,7 +249,7 @@ void lz_eval(LzGroup& g, int iter, double tol) {
         }
     g.d_iter = iter;
     g.met = false;
-    if (prev_iter == iter - 1 && iter >= g.start) {
+    if (prev_iter == iter - 1 && iter >= 1) {
         double dn = 0.0;
         for (size_t t = 0; t < g.D.size(); t++) {
             const double prev = t < g.Dprev.size() ? g.Dprev[t] : 0.0;
@@ -351,8 +350,6 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
         g.H.assign((size_t)g.b * g.b, 0.0);
         g.B.assign((size_t)g.b * g.b, 0.0);
         g.Tb.assign((size_t)(max_iters_ + 1) * g.b * (2 * g.b + 1), 0.0);
-        const int hint = (gbase + gi < (int)hint_.size()) ? hint_[gbase + gi] : 0;
-        g.start = std::max(1, hint - kHintMargin);
     }
     SDT_HIP_CHECK(hipMemcpyAsync(V, Z, sizeof(double) * blk, hipMemcpyDeviceToDevice, s));
     std::vector<char> ok;
@@ -433,7 +430,7 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
         }
         std::vector<int> todo;
         for (int gi = 0; gi < ng; gi++)
-            if (!grp[gi].done && pi >= grp[gi].start - 1) todo.push_back(gi);
+            if (!grp[gi].done) todo.push_back(gi);
         const double tol = tol_;
         parallel_for((int)todo.size(), [&](int t) { lz_eval(grp[todo[t]], pi, tol); });
         bool all = true;
@@ -570,11 +567,7 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
                     k0 == 0 ? 0.0 : 1.0, Y, s);
     }
     SDT_HIP_CHECK(hipStreamSynchronize(s));
-    for (int gi = 0; gi < ng; gi++) {
-        iters.push_back(grp[gi].it);
-        if ((int)hint_.size() <= gbase + gi) hint_.resize(gbase + gi + 1, 0);
-        hint_[gbase + gi] = grp[gi].it >= 0 ? grp[gi].it : 0;
-    }
+    for (int gi = 0; gi < ng; gi++) iters.push_back(grp[gi].it);
 }
 
 void BrwnLanczosGpu::compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double* z, int ldy,

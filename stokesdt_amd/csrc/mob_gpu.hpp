@@ -79,9 +79,6 @@ class BrwnLanczosGpu {
     int max_nrhs() const { return max_nrhs_; }
     int reorth = 0;        // 0: none (reference); 1: full FP64 reorthogonalisation
     StageTimer* timer = nullptr;
-    // the convergence test of a group starts this many iterations before the
-    // count the same group needed in the previous call
-    static constexpr int kHintMargin = 3;
     // columns of one lockstep batch
     static constexpr int kMaxBatchCols = 128;
 
@@ -93,7 +90,7 @@ class BrwnLanczosGpu {
     int dim_, max_iters_, max_nrhs_;
     double tol_;
     bool inited_ = false;
-    std::vector<int> iters_, hint_;
+    std::vector<int> iters_;
     DevBuf<double> d_V_, d_G_, d_S_, d_part_, d_io_z_, d_io_y_, d_Zp_, d_Yp_;
     DevBuf<double> d_Xa_, d_Wa_;   // active columns of a partly finished batch
     DevBuf<int> d_cols_;

@@ -9,36 +9,11 @@
 #include <algorithm>
 #include <cmath>
 #include <vector>
+#include "ewald_scalars.hpp"
 #include "real_gpu.hpp"
 #include "sort_gpu.hpp"
 
 namespace sdt {
-
-// squared min-image distance with explicit fused operations, so that every
-// kernel that classifies a pair as overlapping rounds identically
-__device__ __forceinline__ double pair_rr(double dx, double dy, double dz) {
-    return fma(dx, dx, fma(dy, dy, dz * dz));
-}
-
-template <typename T>
-__device__ __forceinline__ void ewald_scalars(T xi, T r, T aa, T ab, T* xa, T* ya) {
-    const T xir = xi * r;
-    const T xir2 = xir * xir;
-    const T r2 = r * r;
-    const T r3 = r2 * r;
-    const T aa2 = aa * aa, ab2 = ab * ab;
-    const T erfcxir = erfc(xir);
-    const T expxir2 = xi * T(0.56418958354775628694807945156077) * exp(-xir2);  // xi/sqrt(pi)
-    const T a0 = T(2) * (T(-3) + xir2 * T(2)) * expxir2 + erfcxir / r;
-    const T b0 = T(2) * (T(1) + xir2 * T(-2)) * expxir2 + erfcxir / r;
-    const T a2 = T(4) / r2 * (T(1) + xir2 * (T(14) + xir2 * (T(-20) + xir2 * T(4)))) * expxir2 +
-                 T(2) / r3 * erfcxir;
-    const T b2 = T(-4) / r2 * (T(3) + xir2 * (T(2) + xir2 * (T(-16) + xir2 * T(4)))) * expxir2 -
-                 T(6) / r3 * erfcxir;
-    const T s6 = (aa2 + ab2) / T(6);
-    *ya = T(0.75) * (a0 + a2 * s6);
-    *xa = T(0.75) * (b0 + b2 * s6) + *ya;
-}
 
 // Y[(3 i + c) b + r] = beta Y + alpha sum_j B_ij X_j for r in [r0, r0 + RC),
 // over the self block and the non-overlapping pairs (r >= a_i + a_j);

@@ -14,6 +14,11 @@ namespace sdt {
 
 struct SortedSystem;   // particle order shared by SPME and real space
 
+// |b(m)|^2 = 1 / |sum_{i<P-1} M_P(i+1) e^{2 pi i m i / K}|^2 for m = 0..K-1
+// (src/mob/spme.cc:80-128), with the neighbour mean at a vanishing
+// denominator (odd P at the Nyquist index)
+void bspline_moduli(int K, int P, std::vector<double>& out);
+
 template <typename T>
 struct SpmeGpu {
     int npos = 0, K = 0, porder = 0;

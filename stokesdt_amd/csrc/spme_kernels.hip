@@ -1231,7 +1231,7 @@ interp_tile_kernel(int K, int ts, int ntile, int nsx, int sxt, int ch,
 }
 
 // ------------------------------------------------------------ host side ----
-static void b_moduli(int K, int P, std::vector<double>& out) {
+void bspline_moduli(int K, int P, std::vector<double>& out) {
     // |b(m)|^2 = 1 / |sum_{i<P-1} M_P(i+1) e^{2 pi i m i / K}|^2
     // (src/mob/spme.cc:80-128; Essmann et al. 1995)
     std::vector<double> sv(P - 1);
@@ -1323,7 +1323,7 @@ void SpmeGpu<T>::create(int npos_, double box_, double xi_, int K_, int porder_,
     d_tw.reserve(2 * K);
     SDT_HIP_CHECK(hipMemcpy(d_tw.get(), tw.data(), sizeof(double) * 2 * K, hipMemcpyHostToDevice));
     std::vector<double> bs;
-    b_moduli(K, porder, bs);
+    bspline_moduli(K, porder, bs);
     d_bsp2.reserve(K);
     SDT_HIP_CHECK(hipMemcpy(d_bsp2.get(), bs.data(), sizeof(double) * K, hipMemcpyHostToDevice));
     d_frac.reserve(3 * (size_t)npos);

@@ -111,10 +111,10 @@ def test_lanczos_many_groups_and_batches(sdt, orc):
     assert np.abs(np.asarray(bl.iters()) - oit).max() <= 3
 
 
-def test_lanczos_repeat_call_uses_previous_counts(sdt, orc):
-    # a second call starts its convergence test three iterations before the
-    # first call's counts; on the same input it stops at the same iteration
-    # with the same (deterministic) result
+def test_lanczos_repeat_call_is_history_independent(sdt, orc):
+    # a handle keeps no convergence state between calls: a second call on
+    # the same input stops at the same iteration with the same
+    # (deterministic) result, and a fresh handle agrees bit for bit
     n, box = 300, 20.0
     g, _ = _mob(sdt, orc, n, box, 32, 6, sdt.FP64, 9)
     M2 = _eig_sqrt(g.mul_vector(np.eye(3 * n)))
@@ -128,6 +128,10 @@ def test_lanczos_repeat_call_uses_previous_counts(sdt, orc):
     assert ok1 and ok2
     assert it1 == it2
     assert np.array_equal(y1, y2)
+    fresh = sdt.BrwnLanczos(3 * n, 100, 4, 1e-7)
+    assert fresh.init()
+    y3, ok3 = fresh.compute(g, z)
+    assert ok3 and list(fresh.iters()) == it1 and np.array_equal(y1, y3)
     assert orc.frob_rel_err(M2 @ z, y1) < 1e-6
 
 
