@@ -73,6 +73,29 @@ int sdt_mob_spme_dim(sdt_mob_t mob, int* dim);
 int sdt_mob_spme_real_nnz(sdt_mob_t mob, int64_t* nnz);
 int sdt_mob_spme_destroy(sdt_mob_t mob);
 
+/* ----------------------------------------------------------------- MobEwald
+ * Replaces MobEwald(npos, rdi, box_size, tol[, xi]) (src/mob/mob_ewald.h,
+ * ctor src/mob/mob_ewald.cc:19-50, Init :53-110, Update :112-130,
+ * MulVector through MobMatrix :132-152, GetMatrix): the dense 3npos x 3npos
+ * Ewald-summed RPY matrix (EwaldKernel, src/mob/ewald.cc:86-330, truncated
+ * at tol) stored on the device and applied by a GEMM; FP64, npos <= 30000.
+ * xi <= 0 selects the reference's 10^(1/6) sqrt(pi) / L (mob_ewald.cc:51).
+ * The handle is a mobility like an SPME handle: sdt_mob_init / _update /
+ * _mulvector / _dim / _destroy below (or the sdt_mob_spme_ names of the same
+ * MobBase methods) and the Brownian solvers accept either kind; the
+ * real/recip split and the real-pair count exist only for SPME handles. */
+int sdt_mob_ewald_create(int npos, const double* rdi, double box_size, double tol, double xi,
+                         sdt_mob_t* out);
+/* MobMatrix::GetMatrix: the symmetric matrix, column-major, ldm >= 3npos */
+int sdt_mob_ewald_get_matrix(sdt_mob_t mob, int ldm, double* mat);
+/* MobBase (src/mob/mob_base.h:29-69) on any mobility handle */
+int sdt_mob_init(sdt_mob_t mob);
+int sdt_mob_update(sdt_mob_t mob, const double* pos, const double* rdi);
+int sdt_mob_mulvector(sdt_mob_t mob, int num_rhs, double alpha, int ldf, const double* f,
+                      double beta, int ldv, double* v);
+int sdt_mob_dim(sdt_mob_t mob, int* dim);
+int sdt_mob_destroy(sdt_mob_t mob);
+
 /* One-shot SPME matvec with the semantics of the MATLAB entry point
  * v = rpy_spme_mex(pos, rdi, L, f, mode, xi, rmax, dim, porder)
  * (mex/rpy_spme_mex.cc:8-75): alpha = 1, beta = 0, mode "full", "real" or
@@ -162,6 +185,22 @@ int sdt_brwn_lanczos_iters(sdt_brwn_t br, int* iters, int cap, int* ngroups);
 /* 0: no reorthogonalisation (reference); 1: full FP64 reorthogonalisation */
 int sdt_brwn_lanczos_set_reorth(sdt_brwn_t br, int reorth);
 int sdt_brwn_lanczos_destroy(sdt_brwn_t br);
+
+/* --------------------------------------------------------------- BrwnChol
+ * Replaces BrwnChol(dim) (src/brwn/brwn_chol.h), Init
+ * (src/brwn/brwn_chol.cc:29-49) and Compute(mob, nrhs, ldz, z, ldy, y)
+ * (:52-83): y = L z with M = L L^T the lower Cholesky factor of a matrix
+ * mobility (a dense Ewald handle; other handles give SDT_EINVAL), factored
+ * on every call as the reference does. SDT_ECONVERGE if M is not positive
+ * definite. FP64 on the device, dim <= 90000. */
+typedef struct sdt_brwn_chol_s* sdt_brwn_chol_t;
+int sdt_brwn_chol_create(int dim, sdt_brwn_chol_t* out);
+int sdt_brwn_chol_init(sdt_brwn_chol_t br);
+int sdt_brwn_chol_compute(sdt_brwn_chol_t br, sdt_mob_t mob, int num_rhs, int ldz,
+                          const double* z, int ldy, double* y);
+int sdt_brwn_chol_compute_dev(sdt_brwn_chol_t br, sdt_mob_t mob, int num_rhs, int ldz,
+                              const double* d_z, int ldy, double* d_y);
+int sdt_brwn_chol_destroy(sdt_brwn_chol_t br);
 
 /* ------------------------------------------------------------ StericForce
  * Replaces StericForce(npos, rdi, box_size, r0, k0)

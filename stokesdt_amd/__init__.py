@@ -110,6 +110,18 @@ SIGNATURES = {
     "sdt_bd_profile": (_I, [_P, _P, _P, _I]),
     "sdt_bd_profile_columns": (_I, [_P, _P, _I]),
     "sdt_util_rsa_positions": (_I, [_I, _D, _D, ctypes.c_uint64, _I64, _P, _PI64]),
+    "sdt_mob_ewald_create": (_I, [_I, _P, _D, _D, _D, ctypes.POINTER(_P)]),
+    "sdt_mob_ewald_get_matrix": (_I, [_P, _I, _P]),
+    "sdt_mob_init": (_I, [_P]),
+    "sdt_mob_update": (_I, [_P, _P, _P]),
+    "sdt_mob_mulvector": (_I, [_P, _I, _D, _I, _P, _D, _I, _P]),
+    "sdt_mob_dim": (_I, [_P, _PI]),
+    "sdt_mob_destroy": (_I, [_P]),
+    "sdt_brwn_chol_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "sdt_brwn_chol_init": (_I, [_P]),
+    "sdt_brwn_chol_compute": (_I, [_P, _P, _I, _I, _P, _I, _P]),
+    "sdt_brwn_chol_compute_dev": (_I, [_P, _P, _I, _I, _P, _I, _P]),
+    "sdt_brwn_chol_destroy": (_I, [_P]),
     "sdt_plain_mob_create": (_I, [_I, _P, _D, _D, _D, _I, _I, ctypes.POINTER(_P)]),
     "sdt_plain_mob_update": (_I, [_P, _P]),
     "sdt_plain_mob_mulvector": (_I, [_P, _I, _I, _D, _I, _P, _D, _I, _P]),
@@ -256,6 +268,81 @@ class MobSpme:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             lib().sdt_mob_spme_destroy(self.h)
+            self.h = None
+
+
+class MobEwald:
+    """MobEwald(npos, rdi, box_size, tol, xi=None) (src/mob/mob_ewald.h): the
+    dense Ewald-summed RPY matrix on the GPU. init() -> bool; update(pos,
+    rdi); mul_vector(f, alpha, beta, v); get_matrix(). Usable wherever a
+    mobility is (BrwnLanczos, BrwnRandom, BrwnChol)."""
+
+    def __init__(self, npos, rdi, box_size, tol, xi=None):
+        self.npos = int(npos)
+        self.rdi = _f64(rdi)
+        h = _P()
+        _check(lib().sdt_mob_ewald_create(self.npos, _ptr(self.rdi), box_size, tol,
+                                          -1.0 if xi is None else xi, ctypes.byref(h)))
+        self.h = h
+
+    def init(self):
+        rc = lib().sdt_mob_init(self.h)
+        if rc != SDT_OK:
+            self.init_error = lib().sdt_last_error().decode()
+            return False
+        return True
+
+    def update(self, pos, rdi=None):
+        pos = _f64(pos)
+        r = self.rdi if rdi is None else _f64(rdi)
+        _check(lib().sdt_mob_update(self.h, _ptr(pos), _ptr(r)))
+
+    def dim(self):
+        d = ctypes.c_int(0)
+        _check(lib().sdt_mob_dim(self.h, ctypes.byref(d)))
+        return d.value
+
+    def mul_vector(self, f, alpha=1.0, beta=0.0, v=None):
+        f = _colmajor(f)
+        n, nrhs = f.shape
+        out = np.zeros((n, nrhs), order="F") if v is None else _colmajor(v).copy(order="F")
+        _check(lib().sdt_mob_mulvector(self.h, nrhs, alpha, n, _ptr(f), beta, n, _ptr(out)))
+        return out
+
+    def get_matrix(self):
+        n = 3 * self.npos
+        mat = np.zeros((n, n), order="F")
+        _check(lib().sdt_mob_ewald_get_matrix(self.h, n, _ptr(mat)))
+        return mat
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_mob_destroy(self.h)
+            self.h = None
+
+
+class BrwnChol:
+    """BrwnChol(dim) (src/brwn/brwn_chol.cc:29-83): y = L z, M = L L^T, on a
+    dense Ewald mobility."""
+
+    def __init__(self, dim):
+        h = _P()
+        _check(lib().sdt_brwn_chol_create(dim, ctypes.byref(h)))
+        self.h = h
+
+    def init(self):
+        return lib().sdt_brwn_chol_init(self.h) == SDT_OK
+
+    def compute(self, mob, z):
+        z = _colmajor(z)
+        n, nrhs = z.shape
+        y = np.zeros((n, nrhs), order="F")
+        _check(lib().sdt_brwn_chol_compute(self.h, mob.h, nrhs, n, _ptr(z), n, _ptr(y)))
+        return y
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_brwn_chol_destroy(self.h)
             self.h = None
 
 

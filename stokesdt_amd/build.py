@@ -28,6 +28,7 @@ HIP_SOURCES = [
     ("diffusion.hip", []),
     ("brwn_random.hip", []),
     ("plain_fp64.hip", []),
+    ("ewald_mob.hip", []),
     ("capi.hip", []),
 ]
 HOST_SOURCES = [("host_linalg.cpp", []), ("host_util.cpp", []), ("molecule_io.cpp", [])]

@@ -129,7 +129,7 @@ void BrwnRandomGpu::cholqr(hipStream_t s) {
     }
 }
 
-void BrwnRandomGpu::sample(MobSpmeGpu* mob) {
+void BrwnRandomGpu::sample(MobGpu* mob) {
     hipStream_t s = mob->stream();
     SDT_RB_CHECK(rocblas_set_stream(h_, s));
     const double one = 1.0, zero = 0.0;
@@ -169,7 +169,7 @@ void BrwnRandomGpu::sample(MobSpmeGpu* mob) {
     mob_ = mob;
 }
 
-void BrwnRandomGpu::compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double* z, int ldy,
+void BrwnRandomGpu::compute(MobGpu* mob, int num_rhs, int ldz, const double* z, int ldy,
                             double* y, bool dev) {
     SDT_REQUIRE(inited_, SDT_ESTATE, "compute called before init");
     SDT_REQUIRE(num_rhs >= 1, SDT_EINVAL, "num_rhs must be positive");
@@ -180,7 +180,7 @@ void BrwnRandomGpu::compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double*
         sample(mob);
     }
     SDT_REQUIRE(mob_ != nullptr, SDT_ESTATE, "no sampled subspace: call with a mobility first");
-    MobSpmeGpu* m = mob_;
+    MobGpu* m = mob_;
     hipStream_t s = m->stream();
     SDT_RB_CHECK(rocblas_set_stream(h_, s));
     const double one = 1.0, zero = 0.0, mone = -1.0;

@@ -18,19 +18,19 @@ class BrwnRandomGpu {
     // draws a new sample and rebuilds the subspace; y = first + alpha
     // (I - V V^T) z per column with |y|^2 = z^T M z. z, y column-major, host
     // or device (dev).
-    void compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double* z, int ldy, double* y,
+    void compute(MobGpu* mob, int num_rhs, int ldz, const double* z, int ldy, double* y,
                  bool dev);
     static constexpr int kMaxVecs = 128;
 
   private:
-    void sample(MobSpmeGpu* mob);
+    void sample(MobGpu* mob);
     void cholqr(hipStream_t s);
     int dim_, nv_, max_nrhs_;
     unsigned long long seed_;
     unsigned draws_ = 0;
     bool inited_ = false;
     rocblas_handle h_ = nullptr;
-    MobSpmeGpu* mob_ = nullptr;
+    MobGpu* mob_ = nullptr;
     DevBuf<double> d_V_, d_W_, d_X_, d_G_, d_T_, d_F_, d_S_, d_io_z_, d_io_y_, d_alpha_;
 };
 

@@ -6,6 +6,7 @@
 #include "../../include/stokesdt_amd_debug.h"
 #include "brwn_random.hpp"
 #include "diffusion.hpp"
+#include "ewald_gpu.hpp"
 #include "mob_gpu.hpp"
 #include "plain_gpu.hpp"
 #include "simbox.hpp"
@@ -20,7 +21,15 @@ const char* last_error() { return g_err.c_str(); }
 
 using namespace sdt;
 
-struct sdt_mob_s { MobSpmeGpu* m; };
+struct sdt_mob_s { MobGpu* m; };
+struct sdt_brwn_chol_s { BrwnCholGpu* b; };
+
+// the SPME mobility of a handle (entry points that exist only for MobSpme)
+static MobSpmeGpu* spme_of(sdt_mob_t mob) {
+    MobSpmeGpu* m = dynamic_cast<MobSpmeGpu*>(mob->m);
+    if (!m) throw Error{SDT_EINVAL, "this entry point needs an SPME mobility handle"};
+    return m;
+}
 struct sdt_brwn_s { BrwnLanczosGpu* b; };
 struct sdt_brwn_random_s { BrwnRandomGpu* b; };
 struct sdt_pairlist_s {
@@ -126,14 +135,72 @@ int sdt_mob_spme_dim(sdt_mob_t mob, int* dim) {
 int sdt_mob_spme_real_nnz(sdt_mob_t mob, int64_t* nnz) {
     SDT_CHECK_HANDLE(mob);
     if (!nnz) { set_error("null output"); return SDT_EINVAL; }
-    *nnz = mob->m->real_nnz();
-    return SDT_OK;
+    return guard([&] { *nnz = spme_of(mob)->real_nnz(); });
 }
 int sdt_mob_spme_destroy(sdt_mob_t mob) {
     if (!mob) return SDT_OK;
     return guard([&] {
         delete mob->m;
         delete mob;
+    });
+}
+
+// ------------------------------------------------ MobEwald / generic mob --
+int sdt_mob_ewald_create(int npos, const double* rdi, double box_size, double tol, double xi,
+                         sdt_mob_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] {
+        MobEwaldGpu* m = new MobEwaldGpu(npos, rdi, box_size, tol, xi);
+        *out = new sdt_mob_s{m};
+    });
+}
+int sdt_mob_ewald_get_matrix(sdt_mob_t mob, int ldm, double* mat) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        MobEwaldGpu* m = dynamic_cast<MobEwaldGpu*>(mob->m);
+        SDT_REQUIRE(m != nullptr, SDT_EINVAL, "this entry point needs a dense Ewald mobility handle");
+        m->get_matrix(ldm, mat, false);
+    });
+}
+int sdt_mob_init(sdt_mob_t mob) { return sdt_mob_spme_init(mob); }
+int sdt_mob_update(sdt_mob_t mob, const double* pos, const double* rdi) {
+    return sdt_mob_spme_update(mob, pos, rdi);
+}
+int sdt_mob_mulvector(sdt_mob_t mob, int num_rhs, double alpha, int ldf, const double* f,
+                      double beta, int ldv, double* v) {
+    return sdt_mob_spme_mulvector(mob, num_rhs, alpha, ldf, f, beta, ldv, v);
+}
+int sdt_mob_dim(sdt_mob_t mob, int* dim) { return sdt_mob_spme_dim(mob, dim); }
+int sdt_mob_destroy(sdt_mob_t mob) { return sdt_mob_spme_destroy(mob); }
+
+// ------------------------------------------------------------- BrwnChol ---
+int sdt_brwn_chol_create(int dim, sdt_brwn_chol_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] { *out = new sdt_brwn_chol_s{new BrwnCholGpu(dim)}; });
+}
+int sdt_brwn_chol_init(sdt_brwn_chol_t br) {
+    SDT_CHECK_HANDLE(br);
+    return guard([&] { br->b->init(); });
+}
+int sdt_brwn_chol_compute(sdt_brwn_chol_t br, sdt_mob_t mob, int num_rhs, int ldz,
+                          const double* z, int ldy, double* y) {
+    SDT_CHECK_HANDLE(br);
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] { br->b->compute(mob->m, num_rhs, ldz, z, ldy, y, false); });
+}
+int sdt_brwn_chol_compute_dev(sdt_brwn_chol_t br, sdt_mob_t mob, int num_rhs, int ldz,
+                              const double* d_z, int ldy, double* d_y) {
+    SDT_CHECK_HANDLE(br);
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] { br->b->compute(mob->m, num_rhs, ldz, d_z, ldy, d_y, true); });
+}
+int sdt_brwn_chol_destroy(sdt_brwn_chol_t br) {
+    if (!br) return SDT_OK;
+    return guard([&] {
+        delete br->b;
+        delete br;
     });
 }
 
@@ -604,7 +671,7 @@ int sdt_debug_band_sqrt(int m, int kd, const double* T, int kb, double* S) {
 int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh) {
     SDT_CHECK_HANDLE(mob);
     return guard([&] {
-        MobSpmeGpu* m = mob->m;
+        MobSpmeGpu* m = spme_of(mob);
         SDT_REQUIRE(m->updated(), SDT_ESTATE, "mobility not updated");
         const int n3 = m->dim();
         const size_t K3 = (size_t)m->K() * m->K() * m->K();
@@ -638,7 +705,7 @@ int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh) {
 int sdt_debug_spme_interp(sdt_mob_t mob, const double* mesh, double alpha, double* v) {
     SDT_CHECK_HANDLE(mob);
     return guard([&] {
-        MobSpmeGpu* m = mob->m;
+        MobSpmeGpu* m = spme_of(mob);
         SDT_REQUIRE(m->updated(), SDT_ESTATE, "mobility not updated");
         const int n3 = m->dim();
         const size_t K3 = (size_t)m->K() * m->K() * m->K();
@@ -726,7 +793,7 @@ extern "C" {
 int sdt_bench_stage_times(sdt_mob_t mob, int b, int reps, double* t) {
     SDT_CHECK_HANDLE(mob);
     return guard([&] {
-        MobSpmeGpu* m = mob->m;
+        MobSpmeGpu* m = spme_of(mob);
         SDT_REQUIRE(m->updated(), SDT_ESTATE, "mobility not updated");
         SDT_REQUIRE(b >= 1 && b <= 256 && reps >= 1 && t, SDT_EINVAL, "bad arguments");
         const size_t n = (size_t)m->dim() * b;

@@ -330,7 +330,7 @@ static void cholqr2_groups(long long n, int btot, int bs, double* W,
     }
 }
 
-void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const double* Z,
+void BrwnLanczosGpu::lanczos_batch(MobGpu* mob, int btot, int gbase, const double* Z,
                                    double* Y, std::vector<int>& iters) {
     const long long n = dim_;
     hipStream_t s = mob->stream();
@@ -570,7 +570,7 @@ void BrwnLanczosGpu::lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const d
     for (int gi = 0; gi < ng; gi++) iters.push_back(grp[gi].it);
 }
 
-void BrwnLanczosGpu::compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double* z, int ldy,
+void BrwnLanczosGpu::compute(MobGpu* mob, int num_rhs, int ldz, const double* z, int ldy,
                              double* y, bool dev) {
     SDT_REQUIRE(inited_, SDT_ESTATE, "compute called before init");
     SDT_REQUIRE(mob && mob->updated(), SDT_ESTATE, "mobility not updated");

@@ -13,26 +13,47 @@ namespace sdt {
 
 enum Precision : int { kFP64 = 0, kFP32 = 1 };
 
+// A device mobility (MobBase, src/mob/mob_base.h:19-80) as the Brownian
+// solvers see it. Vectors inside the library use the packed layout of the
+// mobility's own particle order: X[(3 i + c) b + r] for the particle perm()[i]
+// of the caller.
+class MobGpu {
+  public:
+    virtual ~MobGpu() {}
+    virtual void init() = 0;
+    // positions and radii: host (dev = false) or device pointers
+    virtual void update(const double* pos, const double* rdi, bool dev) = 0;
+    // v := alpha M f + beta v, reference layout; mode 0 full, 1 real, 2 recip
+    virtual void mul_vector(int mode, int nrhs, double alpha, int ldf, const double* f,
+                            double beta, int ldv, double* v, bool dev) = 0;
+    // packed FP64 vectors (row-major 3N x b) in the order of perm()
+    virtual void apply_packed(int mode, int b, double alpha, const double* X, double beta,
+                              double* Y) = 0;
+    virtual int npos() const = 0;
+    int dim() const { return 3 * npos(); }
+    virtual hipStream_t stream() const = 0;
+    virtual const int* perm() const = 0;
+    virtual bool updated() const = 0;
+};
+
 // MobSpme (src/mob/mob_spme.h:21-125) with device-resident state.
-class MobSpmeGpu {
+class MobSpmeGpu : public MobGpu {
   public:
     MobSpmeGpu(int npos, const double* rdi, double box, double xi, double rmax, int dim,
                int porder, int precision);
-    ~MobSpmeGpu();
+    ~MobSpmeGpu() override;
     // validation and allocation (src/mob/mob_spme.cc:53-132)
-    void init();
-    // positions and radii: host (dev = false) or device pointers
-    void update(const double* pos, const double* rdi, bool dev);
-    // v := alpha M f + beta v, reference layout; mode 0 full, 1 real, 2 recip
+    void init() override;
+    void update(const double* pos, const double* rdi, bool dev) override;
     void mul_vector(int mode, int nrhs, double alpha, int ldf, const double* f, double beta,
-                    int ldv, double* v, bool dev);
+                    int ldv, double* v, bool dev) override;
     // packed, sorted-order FP64 vectors (row-major 3N x b)
-    void apply_packed(int mode, int b, double alpha, const double* X, double beta, double* Y);
-    int dim() const { return 3 * npos_; }
-    int npos() const { return npos_; }
-    hipStream_t stream() const { return stream_; }
-    const int* perm() const { return d_perm_.get(); }
-    bool updated() const { return updated_; }
+    void apply_packed(int mode, int b, double alpha, const double* X, double beta,
+                      double* Y) override;
+    int npos() const override { return npos_; }
+    hipStream_t stream() const override { return stream_; }
+    const int* perm() const override { return d_perm_.get(); }
+    bool updated() const override { return updated_; }
     int precision() const { return prec_; }
     long long real_nnz() const { return real_nnz_; }
     int K() const { return K_; }
@@ -73,7 +94,7 @@ class BrwnLanczosGpu {
     void init();
     // reference layout z/y (host or device); returns the iteration count of
     // every group of max_nrhs columns in iters()
-    void compute(MobSpmeGpu* mob, int num_rhs, int ldz, const double* z, int ldy, double* y,
+    void compute(MobGpu* mob, int num_rhs, int ldz, const double* z, int ldy, double* y,
                  bool dev);
     const std::vector<int>& iters() const { return iters_; }
     int max_nrhs() const { return max_nrhs_; }
@@ -85,7 +106,7 @@ class BrwnLanczosGpu {
   private:
     // lockstep block Lanczos of the groups of a packed 3N x btot block (row
     // major, sorted particle order); gbase is the index of its first group
-    void lanczos_batch(MobSpmeGpu* mob, int btot, int gbase, const double* Z, double* Y,
+    void lanczos_batch(MobGpu* mob, int btot, int gbase, const double* Z, double* Y,
                        std::vector<int>& iters);
     int dim_, max_iters_, max_nrhs_;
     double tol_;

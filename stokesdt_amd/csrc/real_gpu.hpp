@@ -38,6 +38,20 @@ void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const doubl
                  const double* d_rdi, double box, double xi, int b, double alpha, const double* X,
                  double beta, double* Y, void* scratch, bool copied, hipStream_t s);
 
+// Dense Ewald-summed RPY matrix on the device: setup builds the image and
+// k-vector tables of (L, xi, tol) for mode 0 full, 1 real or 2 recip
+// (XitoRmax / XitoKmax truncation, src/mob/ewald.cc:19-83, 643-797); build
+// writes the symmetric 3N x 3N matrix of device positions and radii into
+// d_mat (column-major, leading dimension ld), npos <= 30000.
+struct EwaldMatrixGpu {
+    double box = 0, xi = 0, rmax = 0;
+    int parts = 0, nr = 0, nk = 0;
+    DevBuf<double> d_img, d_k;
+    void setup(double box, double xi, double tol, int mode, hipStream_t s);
+    void build(int npos, const double* d_pos, const double* d_rdi, long long ld, double* d_mat,
+               hipStream_t s) const;
+};
+
 // Dense Ewald-summed RPY matrix (MobDebug::GetMatrix): mode 0 full, 1 real
 // part, 2 reciprocal part; host pos (AoS) and rdi, host output 3N x 3N with
 // leading dimension ldm (symmetric). Synchronises.

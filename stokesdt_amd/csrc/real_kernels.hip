@@ -478,17 +478,16 @@ ewald_block_kernel(int npos, const double* __restrict__ pos, const double* __res
     if (j != i) put(j, i);
 }
 
-void ewald_matrix(int npos, const double* pos, const double* rdi, double box, double xi,
-                  double tol, int mode, int ldm, double* mat, hipStream_t s) {
-    SDT_REQUIRE(npos > 0 && npos <= 30000, SDT_EINVAL, "dense Ewald: npos must be in [1, 30000]");
-    SDT_REQUIRE(box > 0.0 && tol > 0.0 && xi > 0.0, SDT_EINVAL,
+void EwaldMatrixGpu::setup(double box_, double xi_, double tol, int mode, hipStream_t s) {
+    SDT_REQUIRE(box_ > 0.0 && tol > 0.0 && xi_ > 0.0, SDT_EINVAL,
                 "dense Ewald: L, tol and xi must be positive");
-    SDT_REQUIRE(ldm >= 3 * npos, SDT_EINVAL, "dense Ewald: ldm must be at least 3*npos");
     SDT_REQUIRE(mode >= 0 && mode <= 2, SDT_EINVAL, "dense Ewald: mode must be full, real or recip");
-    const int parts = mode == 0 ? 3 : (mode == 1 ? 1 : 2);
+    box = box_;
+    xi = xi_;
+    parts = mode == 0 ? 3 : (mode == 1 ? 1 : 2);
     // periodic images whose shifted cell can hold a pair within rmax
     std::vector<double> img;
-    double rmax = 0.0;
+    rmax = 0.0;
     if (parts & 1) {
         rmax = ewald_rmax(xi, tol, box);
         const int mi = (int)(rmax / box) + 1;
@@ -525,27 +524,44 @@ void ewald_matrix(int npos, const double* pos, const double* rdi, double box, do
                     kt.insert(kt.end(), row, row + kEwaldK);
                 }
     }
-    const int nr = (int)(img.size() / 3), nk = (int)(kt.size() / kEwaldK);
-    const size_t dim = (size_t)3 * npos;
-    DevBuf<double> d_pos, d_rdi, d_img, d_k, d_mat;
-    d_pos.reserve(dim);
-    d_rdi.reserve(npos);
+    nr = (int)(img.size() / 3);
+    nk = (int)(kt.size() / kEwaldK);
     d_img.reserve(std::max<size_t>(1, img.size()));
     d_k.reserve(std::max<size_t>(1, kt.size()));
-    d_mat.reserve(dim * dim);
-    SDT_HIP_CHECK(hipMemcpyAsync(d_pos.get(), pos, sizeof(double) * dim, hipMemcpyHostToDevice, s));
-    SDT_HIP_CHECK(hipMemcpyAsync(d_rdi.get(), rdi, sizeof(double) * npos, hipMemcpyHostToDevice, s));
     if (nr)
         SDT_HIP_CHECK(hipMemcpyAsync(d_img.get(), img.data(), sizeof(double) * img.size(),
                                      hipMemcpyHostToDevice, s));
     if (nk)
         SDT_HIP_CHECK(hipMemcpyAsync(d_k.get(), kt.data(), sizeof(double) * kt.size(),
                                      hipMemcpyHostToDevice, s));
+    SDT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void EwaldMatrixGpu::build(int npos, const double* d_pos, const double* d_rdi, long long ld,
+                           double* d_mat, hipStream_t s) const {
+    SDT_REQUIRE(parts != 0, SDT_ESTATE, "dense Ewald: build before setup");
+    SDT_REQUIRE(npos > 0 && npos <= 30000, SDT_EINVAL, "dense Ewald: npos must be in [1, 30000]");
+    SDT_REQUIRE(ld >= 3LL * npos, SDT_EINVAL, "dense Ewald: ldm must be at least 3*npos");
     const dim3 grid((unsigned)((npos + 255) / 256), (unsigned)npos);
-    ewald_block_kernel<<<grid, 256, 0, s>>>(npos, d_pos.get(), d_rdi.get(), box, xi, rmax * rmax,
-                                            nr, d_img.get(), nk, d_k.get(), parts,
-                                            (long long)dim, d_mat.get());
+    ewald_block_kernel<<<grid, 256, 0, s>>>(npos, d_pos, d_rdi, box, xi, rmax * rmax, nr,
+                                            d_img.get(), nk, d_k.get(), parts, ld, d_mat);
     SDT_HIP_CHECK(hipGetLastError());
+}
+
+void ewald_matrix(int npos, const double* pos, const double* rdi, double box, double xi,
+                  double tol, int mode, int ldm, double* mat, hipStream_t s) {
+    SDT_REQUIRE(npos > 0 && npos <= 30000, SDT_EINVAL, "dense Ewald: npos must be in [1, 30000]");
+    SDT_REQUIRE(ldm >= 3 * npos, SDT_EINVAL, "dense Ewald: ldm must be at least 3*npos");
+    EwaldMatrixGpu ew;
+    ew.setup(box, xi, tol, mode, s);
+    const size_t dim = (size_t)3 * npos;
+    DevBuf<double> d_pos, d_rdi, d_mat;
+    d_pos.reserve(dim);
+    d_rdi.reserve(npos);
+    d_mat.reserve(dim * dim);
+    SDT_HIP_CHECK(hipMemcpyAsync(d_pos.get(), pos, sizeof(double) * dim, hipMemcpyHostToDevice, s));
+    SDT_HIP_CHECK(hipMemcpyAsync(d_rdi.get(), rdi, sizeof(double) * npos, hipMemcpyHostToDevice, s));
+    ew.build(npos, d_pos.get(), d_rdi.get(), (long long)dim, d_mat.get(), s);
     SDT_HIP_CHECK(hipMemcpy2DAsync(mat, sizeof(double) * ldm, d_mat.get(), sizeof(double) * dim,
                                    sizeof(double) * dim, dim, hipMemcpyDeviceToHost, s));
     SDT_HIP_CHECK(hipStreamSynchronize(s));

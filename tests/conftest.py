@@ -96,3 +96,28 @@ def poly_system(n, box, seed):
     sizes = np.geomspace(21.4, 115.2, 15)
     rdi = rng.choice(sizes, size=n)
     return rng.uniform(0.0, box, 3 * n), rdi
+
+
+def direct_steric(pos, rdi, box, cutoff, k0):
+    """O(N^2) steric force of tests/steric/teststeric.cc:34-79: pairs with
+    2r/(a_i + a_j) < cutoff (r = 0 counts as 1e-10) push apart with
+    k0 ((a_i + a_j) cutoff / 2 - r) along the min-image separation."""
+    n = rdi.size
+    P = pos.reshape(n, 3)
+    f = np.zeros((n, 3))
+    for i in range(n):
+        d = np.remainder(P[i] - P + box / 2.0, box) - box / 2.0
+        r = np.sqrt((d ** 2).sum(1))
+        r = np.where(r == 0.0, 1e-10, r)
+        s = 2.0 * r / (rdi[i] + rdi)
+        sel = (s - cutoff < 0.0)
+        sel[i] = False
+        force = k0 * ((rdi[i] + rdi) * cutoff / 2.0 - r)
+        f[i] = ((force / r)[sel, None] * d[sel]).sum(0)
+    return f.reshape(-1)
+
+
+def box_for(n, phi, a=1.0):
+    """Side of the periodic cube holding n spheres of radius a at volume
+    fraction phi."""
+    return (4.0 * np.pi * n * a ** 3 / (3.0 * phi)) ** (1.0 / 3.0)

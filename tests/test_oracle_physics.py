@@ -20,7 +20,7 @@ sums approach as 1/L in growing boxes.
 import numpy as np
 import pytest
 
-from conftest import poly_system, reference_system, uniform_system
+from conftest import direct_steric, poly_system, reference_system, uniform_system
 
 L1000 = 34.729314   # box of the reference's N1000 phi=0.1 system
 
@@ -278,29 +278,12 @@ def test_pairlist_matches_brute_force(orc, mode, cutoff):
         assert i in set(got.tolist())   # self pair included
 
 
-def _direct_steric(pos, rdi, box, cutoff, k0):
-    # O(N^2) force of tests/steric/teststeric.cc:34-79
-    n = rdi.size
-    P = pos.reshape(n, 3)
-    f = np.zeros((n, 3))
-    for i in range(n):
-        d = np.remainder(P[i] - P + box / 2.0, box) - box / 2.0
-        r = np.sqrt((d ** 2).sum(1))
-        r = np.where(r == 0.0, 1e-10, r)
-        s = 2.0 * r / (rdi[i] + rdi)
-        sel = (s - cutoff < 0.0)
-        sel[i] = False
-        force = k0 * ((rdi[i] + rdi) * cutoff / 2.0 - r)
-        f[i] = ((force / r)[sel, None] * d[sel]).sum(0)
-    return f.reshape(-1)
-
-
 def test_steric_matches_direct(orc):
     # tests/steric/test.sh:2 on the reference's own input: N412.xyz with
     # poly.mod, cutoff 2.3, k0 125
     pos, rdi, box = reference_system("N412.xyz", "poly.mod")
     f = orc.steric(pos, rdi, box, 2.3, 125.0)
-    fd = _direct_steric(pos, rdi, box, 2.3, 125.0)
+    fd = direct_steric(pos, rdi, box, 2.3, 125.0)
     assert np.linalg.norm(f) > 0
     assert np.linalg.norm(f - fd) / np.linalg.norm(fd) < 1e-6
     # Newton's third law
