@@ -126,6 +126,17 @@ int sdt_rpy_ewald_matrix(const double* pos, const double* rdi, int npos, double 
  * (every particle counts with radius 1, as there). *nlags receives
  * nframes/2 - 1; D must hold that many values. FP64 on the device. */
 int sdt_diffusion(const double* pos, int nframes, int npos, double dt, double* D, int* nlags);
+/* No counterpart in the reference: sdt_diffusion for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops). d_pos and d_D are device pointers on the current device; the work
+ * is enqueued on stream (a hipStream_t, NULL for the default stream) and the
+ * call returns without waiting. No hipMalloc / hipFree: the partial sums live
+ * in a static device array of 2^20 doubles (8 MiB, held on every GPU the
+ * library runs on until the process ends), so ceil(npos / 256) *
+ * (nframes / 2 - 1) must not exceed 2^20 (else SDT_ECAP). Calls that
+ * overlap on one GPU share that array and race. */
+int sdt_diffusion_dev_noalloc(const double* d_pos, int nframes, int npos, double dt, double* d_D,
+                              int* nlags, void* stream);
 
 /* ------------------------------------------------------------- BrwnRandom
  * Replaces BrwnRandom(dim, num_vecs, max_nrhs) (src/brwn/brwn_random.h:16-86),

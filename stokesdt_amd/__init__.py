@@ -80,6 +80,7 @@ SIGNATURES = {
     "sdt_force_steric": (_I, [_P, _P, _I, _D, _D, _D, _P]),
     "sdt_rpy_ewald_matrix": (_I, [_P, _P, _I, _D, _D, ctypes.c_char_p, _D, _P, _I]),
     "sdt_diffusion": (_I, [_P, _I, _I, _D, _P, ctypes.POINTER(ctypes.c_int)]),
+    "sdt_diffusion_dev_noalloc": (_I, [_P, _I, _I, _D, _P, ctypes.POINTER(ctypes.c_int), _P]),
     "sdt_brwn_random_create": (_I, [_I, _I, _I, ctypes.c_uint64, ctypes.POINTER(_P)]),
     "sdt_brwn_random_init": (_I, [_P]),
     "sdt_brwn_random_compute": (_I, [_P, _P, _I, _I, _P, _I, _P]),

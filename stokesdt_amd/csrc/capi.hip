@@ -282,6 +282,17 @@ int sdt_diffusion(const double* pos, int nframes, int npos, double dt, double* D
     });
 }
 
+int sdt_diffusion_dev_noalloc(const double* d_pos, int nframes, int npos, double dt, double* d_D,
+                              int* nlags, void* stream) {
+    if (!d_pos || !d_D || !nlags || npos <= 0) { set_error("invalid array arguments"); return SDT_EINVAL; }
+    if (nframes <= 2) { set_error("need more than two frames"); return SDT_EINVAL; }
+    if (dt == 0.) { set_error("the frame interval must be non-zero"); return SDT_EINVAL; }
+    *nlags = diffusion_lags(nframes);
+    return guard([&] {
+        diffusion_noalloc(d_pos, nframes, npos, dt, d_D, static_cast<hipStream_t>(stream));
+    });
+}
+
 // ------------------------------------------------------------- BrwnRandom --
 int sdt_brwn_random_create(int dim, int num_vecs, int max_nrhs, uint64_t seed,
                            sdt_brwn_random_t* out) {

@@ -11,4 +11,14 @@ int diffusion_lags(int nframes);
 // Synchronises.
 void diffusion(const double* traj, int nframes, int npos, double dt, double* D, hipStream_t s);
 
+// Partial sums the allocation-free variant holds in its static device array
+// (8 MiB per GPU): ceil(npos / 256) * (nframes / 2 - 1) must not exceed it.
+constexpr size_t kDiffusionScratchDoubles = (size_t)1 << 20;
+constexpr int kMaxDevices = 64;
+// The same on device pointers with no device allocation: d_traj and d_D on
+// the current device; enqueued on s without waiting. Calls that overlap on
+// one GPU share the scratch array and race.
+void diffusion_noalloc(const double* d_traj, int nframes, int npos, double dt, double* d_D,
+                       hipStream_t s);
+
 }  // namespace sdt
