@@ -385,6 +385,30 @@ int sdt_pairlist_destroy(sdt_pairlist_t pl) {
         delete pl;
     });
 }
+// The list of a size query is kept (per thread) and handed out by the next
+// call on the same inputs, so the two-call protocol builds the list once.
+namespace {
+struct PairlistQuery {
+    sdt_pairlist_t h = nullptr;
+    std::vector<double> pos, rdi;
+    double L = 0.0, cutoff = 0.0;
+    char mode = 0;
+    int64_t nnz = -1;
+    ~PairlistQuery() { drop(); }
+    void drop() {
+        if (h) sdt_pairlist_destroy(h);
+        h = nullptr;
+        nnz = -1;
+    }
+    bool same(const double* p, const double* r, int npos, double L_, double c, char m) const {
+        return h && (int)rdi.size() == npos && L == L_ && cutoff == c && mode == m &&
+               std::memcmp(pos.data(), p, sizeof(double) * 3 * (size_t)npos) == 0 &&
+               std::memcmp(rdi.data(), r, sizeof(double) * (size_t)npos) == 0;
+    }
+};
+thread_local PairlistQuery g_pl_query;
+}  // namespace
+
 int sdt_pairlist(const double* pos, const double* rdi, int npos, double L, double cutoff,
                  char mode, int32_t* ia, int32_t* ja, int64_t ja_cap, int64_t* nnz) {
     // argument checks of mex/pairlist_mex.cc:13-53
@@ -392,23 +416,38 @@ int sdt_pairlist(const double* pos, const double* rdi, int npos, double L, doubl
     if (L <= 0. || cutoff <= 0.) { set_error("Scalar arguments must be positive."); return SDT_EINVAL; }
     if (2. * cutoff >= L) { set_error("cutoff must be less than L/2."); return SDT_EINVAL; }
     if (mode != 'R' && mode != 'S') { set_error("mode must be R or S."); return SDT_EINVAL; }
-    sdt_pairlist_t h = nullptr;
-    int rc = sdt_pairlist_create(npos, rdi, L, cutoff, mode, &h);
-    if (rc) return rc;
-    int64_t est = 0, n = 0;
-    rc = sdt_pairlist_init(h, &est);
-    if (!rc) rc = sdt_pairlist_build(h, pos, &n);
-    if (!rc) {
-        if (nnz) *nnz = n;
-        if (n > ja_cap || !ia || (n > 0 && !ja)) {
-            // a size query (null outputs) or too small a ja: report nnz only
-            set_error("ja capacity is smaller than nnz");
-            rc = SDT_ECAP;
-        } else {
-            rc = sdt_pairlist_get_pairs(h, ia, ja);
+    PairlistQuery& q = g_pl_query;
+    int rc = SDT_OK;
+    int64_t n = 0;
+    if (q.same(pos, rdi, npos, L, cutoff, mode)) {
+        n = q.nnz;   // built by the size query before
+    } else {
+        q.drop();
+        rc = sdt_pairlist_create(npos, rdi, L, cutoff, mode, &q.h);
+        if (rc) return rc;
+        int64_t est = 0;
+        rc = sdt_pairlist_init(q.h, &est);
+        if (!rc) rc = sdt_pairlist_build(q.h, pos, &n);
+        if (rc) {
+            q.drop();
+            return rc;
         }
     }
-    sdt_pairlist_destroy(h);
+    if (nnz) *nnz = n;
+    if (n > ja_cap || !ia || (n > 0 && !ja)) {
+        // a size query (null outputs) or too small a ja: report nnz only and
+        // keep the list for the call with buffers
+        q.pos.assign(pos, pos + 3 * (size_t)npos);
+        q.rdi.assign(rdi, rdi + npos);
+        q.L = L;
+        q.cutoff = cutoff;
+        q.mode = mode;
+        q.nnz = n;
+        set_error("ja capacity is smaller than nnz");
+        return SDT_ECAP;
+    }
+    rc = sdt_pairlist_get_pairs(q.h, ia, ja);
+    q.drop();
     return rc;
 }
 

@@ -210,7 +210,10 @@ void BrwnLanczosGpu::init() {
     SDT_REQUIRE(max_nrhs_ > 0, SDT_EINVAL, "max_nrhs must be positive");
     SDT_REQUIRE(tol_ > 0.0, SDT_EINVAL, "tolerance must be positive");
     if (max_nrhs_ > dim_ / 3) max_nrhs_ = dim_ / 3;
-    if (max_nrhs_ > 16) max_nrhs_ = 16;   // the device block width limit
+    // groups are at most 16 columns wide on the device (one 16 x 16 Gram
+    // tile per group); wider groups would change the reference's grouping
+    SDT_REQUIRE(max_nrhs_ <= 16, SDT_EINVAL,
+                "max_nrhs must be at most 16 on the device (groups of max_nrhs columns)");
     d_S_.reserve(8192);
     inited_ = true;
 }

@@ -3,6 +3,7 @@
 #include "simbox.hpp"
 #include <cfloat>
 #include <climits>
+#include <cstdio>
 #include <cstring>
 
 namespace sdt {
@@ -75,9 +76,19 @@ SimBoxGpu::~SimBoxGpu() {
 void SimBoxGpu::init() {
     SDT_REQUIRE(!bd_, SDT_ESTATE, "init called twice");
     const int n = io_.npos();
+    int maxnrhs = prm_.lanczos_maxnrhs;
+    if (!prm_.random && maxnrhs > 16) {
+        // the device's groups are at most 16 columns wide
+        std::fprintf(stderr,
+                     "WARNING: lanczos-maxnrhs %d > 16: the displacements are computed in groups "
+                     "of 16 columns (the reference's groups of %d give other samples of the same "
+                     "distribution)\n",
+                     maxnrhs, maxnrhs);
+        maxnrhs = 16;
+    }
     bd_.reset(new BDGpu(n, pos_.data(), io_.rdi().data(), io_.box[0], prm_.spme_xi, prm_.spme_rmax,
                         prm_.spme_dim, prm_.spme_porder, prm_.precision, prm_.lanczos_maxiters,
-                        prm_.lanczos_maxnrhs, prm_.lanczos_tol,
+                        maxnrhs, prm_.lanczos_tol,
                         prm_.steric ? prm_.steric_r0 : 0.0, prm_.steric ? prm_.steric_k0 : 0.0,
                         prm_.delta_t, prm_.mob_interval, prm_.seed));
     if (prm_.random) bd_->use_random(prm_.random_numvecs, prm_.random_maxnrhs);

@@ -1302,10 +1302,16 @@ static FFTArgs fft_args(const FFTPlan& p) {
 template <typename T>
 void SpmeGpu<T>::create(int npos_, double box_, double xi_, int K_, int porder_, bool poly_) {
     npos = npos_; box = box_; xi = xi_; K = K_; porder = porder_; poly = poly_;
-    SDT_REQUIRE(porder >= 4 && porder <= 8, SDT_EINVAL, "porder must be in [4, 8] on the GPU path");
-    SDT_REQUIRE(K % 2 == 0, SDT_EINVAL, "mesh dimension must be even");
+    // narrower than the reference (porder > 1 with 4 porder <= dim,
+    // src/mob/mob_spme.cc:79-90): the kernels are instantiated for these
+    SDT_REQUIRE(porder >= 4 && porder <= 8, SDT_EINVAL,
+                "porder must be in [4, 8] on the GPU path (the reference accepts any porder > 1 "
+                "with 4 porder <= dim)");
+    SDT_REQUIRE(K % 2 == 0, SDT_EINVAL,
+                "mesh dimension must be even on the GPU path (the reference's odd-dim wavenumber "
+                "map is incomplete)");
     SDT_REQUIRE(make_fft_plan(K, &plan), SDT_EINVAL,
-                "mesh dimension must factor into 2, 3 and 5");
+                "mesh dimension must factor into 2, 3 and 5 on the GPU path");
     SDT_REQUIRE(K <= 1024, SDT_EINVAL, "mesh dimension must be <= 1024");
     // tile side: a divisor of K in [P-1, 8], preferring 8 (one wave owns
     // ts x ts mesh rows)
