@@ -43,8 +43,96 @@ CFG = dict(n=100_000, phi=0.3, K=128, p=6, rmax=5.0, xi=0.896, b=8, tol=1e-6, ma
            mob_interval=100, dt=1e-3, r0=2.0, k0=125.0, seed=2024)
 
 
+# batched ensemble (64 independent systems)
+ENS = dict(nsys=64, nmin=500, nmax=20_000, phimin=0.05, phimax=0.35, kmin=24, kmax=96, p=6,
+           xi=0.896, rmax=5.0, b=4, tol=1e-6, max_iters=100, mob_interval=100, dt=1e-3, r0=2.0,
+           k0=125.0, seed=33)
+
+
 def box_size(n, phi):
     return (4.0 * np.pi * n / (3.0 * phi)) ** (1.0 / 3.0)
+
+
+def smooth_mesh(L, kmin, kmax, h=0.8):
+    """The even 2^a 3^b 5^c in [kmin, kmax] nearest to L / h (larger on a tie)."""
+    def smooth(k):
+        for f in (2, 3, 5):
+            while k % f == 0:
+                k //= f
+        return k == 1
+    cand = [k for k in range(kmin, kmax + 1) if k % 2 == 0 and smooth(k)]
+    t = L / h
+    return min(cand, key=lambda k: (abs(k - t), -k))
+
+
+def ensemble_systems(S):
+    """64 systems: N log-uniform in [500, 20000], phi uniform in [0.05, 0.35],
+    mesh the nearest even 2^a 3^b 5^c to L / 0.8 in [24, 96] (seeded)."""
+    e = ENS
+    rng = np.random.default_rng(e["seed"])
+    out = []
+    for k in range(e["nsys"]):
+        n = int(round(np.exp(rng.uniform(np.log(e["nmin"]), np.log(e["nmax"])))))
+        phi = float(rng.uniform(e["phimin"], e["phimax"]))
+        L = box_size(n, phi)
+        out.append(dict(pos=S.rsa_positions(n, L, 1.0, e["seed"] + k), rdi=np.ones(n), box=L,
+                        xi=e["xi"], rmax=e["rmax"], dim=smooth_mesh(L, e["kmin"], e["kmax"]),
+                        phi=phi))
+    return out
+
+
+def run_ensemble(S, steps, warmup):
+    """Config-3 ensemble (FP32-mixed, b = 4 per system, autotuned): warmup
+    steps and on to an interval boundary untimed, then `steps` timed steps
+    (host wall time per step, every system advanced); then the same number of
+    tail steps again with eager launches instead of graphs, for comparison."""
+    import tempfile
+    e = ENS
+    mi = e["mob_interval"]
+    systems = ensemble_systems(S)
+    ens = S.Ensemble(systems, e["p"], S.FP32, e["max_iters"], e["b"], e["tol"], e["r0"], e["k0"],
+                     e["dt"], mi, e["seed"])
+    t = time.perf_counter()
+    with tempfile.TemporaryDirectory() as tmp:
+        ens.init(autotune=True, cache=os.path.join(tmp, "tune.txt"))
+    t_tune = time.perf_counter() - t
+    first = warmup + (-warmup) % mi
+    ens.advance(first)
+    lat = ens.advance(steps)
+    value, how = amortise(list(lat / 1000.0), first, mi)
+    tails = [x for j, x in enumerate(lat) if (first + j) % mi != 0]
+    # eager tails over the same kind of steps (the window ends inside an
+    # interval unless it holds whole intervals)
+    n_eager = min(len(tails), mi - 1 - ((first + steps - 1) % mi)) if tails else 0
+    eager = []
+    if n_eager > 0:
+        ens.set_graphs(False)
+        eager = list(ens.advance(n_eager))
+        ens.set_graphs(True)
+    ntot = int(sum(len(sy["rdi"]) for sy in systems))
+    gain_n = [ens.tuning(s)[2] for s in range(len(systems))]
+    ms = np.array(gain_n)
+    return dict(
+        systems=len(systems), particles=ntot,
+        n_range=[int(min(len(sy["rdi"]) for sy in systems)),
+                 int(max(len(sy["rdi"]) for sy in systems))],
+        meshes=sorted(set(int(sy["dim"]) for sy in systems)),
+        steps_per_s=round(value, 4), particle_steps_per_s=round(value * ntot, 1),
+        amortisation=how, first_step=first, steps=steps,
+        step_latency_ms_p50=round(float(np.percentile(lat, 50)), 4),
+        step_latency_ms_p99=round(float(np.percentile(lat, 99)), 4),
+        interval_start_step_ms=round(float(np.mean([x for j, x in enumerate(lat)
+                                                    if (first + j) % mi == 0])), 3),
+        tail_step_ms_graph=round(float(np.mean(tails)), 4) if tails else None,
+        tail_step_ms_eager=round(float(np.mean(eager)), 4) if eager else None,
+        autotune_s=round(t_tune, 2),
+        autotune_matvec_ms_sum={"narrow_default": round(float(ms[:, 0].sum()), 3),
+                                "narrow_tuned": round(float(ms[:, 1].sum()), 3),
+                                "wide_default": round(float(ms[:, 2].sum()), 3),
+                                "wide_tuned": round(float(ms[:, 3].sum()), 3)},
+        workload="64 systems, N log-uniform [500, 20000], phi U[0.05, 0.35], mesh nearest even "
+                 "2^a3^b5^c to L/0.8 in [24, 96], p=6 rc=5 xi=0.896, FP32-mixed, block Lanczos "
+                 "b=4 tol=1e-6, mob-interval=100, dt=1e-3, steric r0=2 k0=125")
 
 
 def alg_bytes(stage, b, n, K, e, poly=False, nnz=0):
@@ -338,6 +426,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU paths")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU path")
     ap.add_argument("--no-err", action="store_true", help="skip the FP32-vs-FP64 error check")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-system ensemble")
     ap.add_argument("--kernel-sample", action="store_true",
                     help="run only the short kernel sample used for counter collection")
     args = ap.parse_args()
@@ -410,6 +499,11 @@ def main():
                      "(rocSOLVER Householder QR, rocBLAS products, Y every iteration)")
             res["speedup_vs_plain_fp64_gpu"] = round(value / rp["value"], 2)
             res["north_star_speedup_target"] = 10.0
+    if not args.no_ensemble:
+        ens = run_ensemble(S, args.steps, args.warmup)
+        res["ensemble"] = ens
+        res["ensemble_step_latency_ms_p50"] = ens["step_latency_ms_p50"]
+        res["ensemble_step_latency_ms_p99"] = ens["step_latency_ms_p99"]
     if not args.no_err:
         e_mf, e_d = displacement_error(S, O, pos, rdi, L)
         res["mf_err_fp32_vs_fp64"] = e_mf

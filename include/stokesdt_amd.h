@@ -247,6 +247,44 @@ int sdt_bd_get_positions(sdt_bd_t bd, double* pos);
 int sdt_bd_last_iters(sdt_bd_t bd, int* iters, int cap, int* ngroups);
 int sdt_bd_destroy(sdt_bd_t bd);
 
+/* Tails of the BD step as one replayed HIP graph per mobility interval
+ * (default on): the forces, M f and position update of a step are captured
+ * once per interval and replayed; results are bit-identical to eager launches
+ * (off). No counterpart in the reference. */
+int sdt_bd_set_graphs(sdt_bd_t bd, int on);
+
+/* -------------------------------------------------------------- Ensemble
+ * An ensemble of independent BD systems of uneven size, box and mesh
+ * advanced together (no counterpart in the reference, whose BDSimBox
+ * advances one system, src/simbox/simbox_bd.cc:296-366; each member is that
+ * step). System s has npos[s] particles (positions and radii of all systems
+ * one after another in pos / rdi), box box_size[s], Ewald xi[s], real-space
+ * cutoff rmax[s] and mesh dim[s]; porder, precision, Lanczos, steric, dt and
+ * mob_interval are shared; system s draws its Gaussian blocks from seed + s.
+ * Each mobility interval's updates and Lanczos solves run side by side (one
+ * HIP stream per system); every other step is one HIP graph launch holding
+ * every system's step tail. init with autotune != 0 picks each system's SPME
+ * launch choices (FFT radix order, interp strip and chunk, spread stage) by
+ * timing them on the system, for single-column and Lanczos-batch launches;
+ * tune_cache (may be NULL) names a text file of earlier choices.
+ * advance writes each step's host wall time in ms to step_ms (may be NULL).
+ * sdt_ensemble_tuning: choices[8] = {fft_order, interp_sxt, interp_ch,
+ * spread_stage_kb} narrow then wide (0 = heuristic); ms[4] = default and
+ * tuned matvec times, narrow then wide. */
+typedef struct sdt_ensemble_s* sdt_ensemble_t;
+int sdt_ensemble_create(int nsys, const int* npos, const double* pos, const double* rdi,
+                        const double* box_size, const double* xi, const double* rmax,
+                        const int* dim, int porder, int precision, int max_iters, int max_nrhs,
+                        double tol, double steric_r0, double steric_k0, double dt,
+                        int mob_interval, uint64_t seed, sdt_ensemble_t* out);
+int sdt_ensemble_init(sdt_ensemble_t e, int autotune, const char* tune_cache);
+int sdt_ensemble_set_graphs(sdt_ensemble_t e, int on);
+int sdt_ensemble_advance(sdt_ensemble_t e, int nsteps, double* step_ms);
+int sdt_ensemble_get_positions(sdt_ensemble_t e, int sys, double* pos);
+int sdt_ensemble_tuning(sdt_ensemble_t e, int sys, int* choices, double* ms);
+int sdt_ensemble_last_iters(sdt_ensemble_t e, int sys, int* iters, int cap, int* ngroups);
+int sdt_ensemble_destroy(sdt_ensemble_t e);
+
 /* ---------------------------------------------------------------- SimBox
  * Replaces BDSimBox(config_file, model_file, xyz_file)
  * (src/simbox/simbox_bd.cc:31-294) with Init and Advance (:296-366), and the

@@ -123,6 +123,16 @@ SIGNATURES = {
     "sdt_brwn_chol_compute": (_I, [_P, _P, _I, _I, _P, _I, _P]),
     "sdt_brwn_chol_compute_dev": (_I, [_P, _P, _I, _I, _P, _I, _P]),
     "sdt_brwn_chol_destroy": (_I, [_P]),
+    "sdt_bd_set_graphs": (_I, [_P, _I]),
+    "sdt_ensemble_create": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _D, _D, _D, _D,
+                                 _I, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "sdt_ensemble_init": (_I, [_P, _I, ctypes.c_char_p]),
+    "sdt_ensemble_set_graphs": (_I, [_P, _I]),
+    "sdt_ensemble_advance": (_I, [_P, _I, _P]),
+    "sdt_ensemble_get_positions": (_I, [_P, _I, _P]),
+    "sdt_ensemble_tuning": (_I, [_P, _I, _P, _P]),
+    "sdt_ensemble_last_iters": (_I, [_P, _I, _P, _I, _PI]),
+    "sdt_ensemble_destroy": (_I, [_P]),
     "sdt_plain_mob_create": (_I, [_I, _P, _D, _D, _D, _I, _I, ctypes.POINTER(_P)]),
     "sdt_plain_mob_update": (_I, [_P, _P]),
     "sdt_plain_mob_mulvector": (_I, [_P, _I, _I, _D, _I, _P, _D, _I, _P]),
@@ -656,6 +666,10 @@ class BDSim:
     def set_profiling(self, on):
         _check(lib().sdt_bd_set_profiling(self.h, 1 if on else 0))
 
+    def set_graphs(self, on):
+        """Step tails as one replayed HIP graph per mobility interval (default)."""
+        _check(lib().sdt_bd_set_graphs(self.h, 1 if on else 0))
+
     def profile(self):
         """{stage: (total ms, launches, right-hand sides over the launches)}
         recorded since profiling was enabled."""
@@ -814,4 +828,64 @@ class PlainBDSim:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             lib().sdt_plain_bd_destroy(self.h)
+            self.h = None
+
+
+class Ensemble:
+    """Independent BD systems of uneven size, box and mesh advanced together:
+    each mobility interval's updates and Lanczos solves side by side, every
+    other step one HIP graph launch over all systems' step tails.
+    systems: list of dicts with keys pos (flat AoS), rdi, box, xi, rmax, dim."""
+
+    def __init__(self, systems, porder, precision, max_iters, max_nrhs, tol, steric_r0,
+                 steric_k0, dt, mob_interval, seed):
+        self.npos = np.array([len(s["rdi"]) for s in systems], dtype=np.int32)
+        pos = _f64(np.concatenate([np.asarray(s["pos"], dtype=np.float64).reshape(-1)
+                                   for s in systems]))
+        rdi = _f64(np.concatenate([np.asarray(s["rdi"], dtype=np.float64) for s in systems]))
+        box = _f64([s["box"] for s in systems])
+        xi = _f64([s["xi"] for s in systems])
+        rmax = _f64([s["rmax"] for s in systems])
+        dim = np.array([s["dim"] for s in systems], dtype=np.int32)
+        h = _P()
+        _check(lib().sdt_ensemble_create(len(systems), _ptr(self.npos), _ptr(pos), _ptr(rdi),
+                                         _ptr(box), _ptr(xi), _ptr(rmax), _ptr(dim), porder,
+                                         precision, max_iters, max_nrhs, tol, steric_r0,
+                                         steric_k0, dt, mob_interval, seed, ctypes.byref(h)))
+        self.h = h
+
+    def init(self, autotune=False, cache=None):
+        _check(lib().sdt_ensemble_init(self.h, 1 if autotune else 0,
+                                       None if cache is None else str(cache).encode()))
+
+    def set_graphs(self, on):
+        _check(lib().sdt_ensemble_set_graphs(self.h, 1 if on else 0))
+
+    def advance(self, nsteps):
+        """Returns each step's host wall time in ms."""
+        ms = np.zeros(max(int(nsteps), 1))
+        _check(lib().sdt_ensemble_advance(self.h, int(nsteps), _ptr(ms)))
+        return ms[:nsteps]
+
+    def positions(self, s):
+        out = np.zeros(3 * int(self.npos[s]))
+        _check(lib().sdt_ensemble_get_positions(self.h, s, _ptr(out)))
+        return out
+
+    def tuning(self, s):
+        """((narrow choices), (wide choices), [default ms, tuned ms] x 2)."""
+        c = np.zeros(8, dtype=np.int32)
+        ms = np.zeros(4)
+        _check(lib().sdt_ensemble_tuning(self.h, s, _ptr(c), _ptr(ms)))
+        return tuple(c[:4]), tuple(c[4:]), ms
+
+    def last_iters(self, s):
+        out = np.zeros(4096, dtype=np.int32)
+        ng = ctypes.c_int(0)
+        _check(lib().sdt_ensemble_last_iters(self.h, s, _ptr(out), 4096, ctypes.byref(ng)))
+        return out[:ng.value].copy()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_ensemble_destroy(self.h)
             self.h = None

@@ -29,6 +29,8 @@ HIP_SOURCES = [
     ("brwn_random.hip", []),
     ("plain_fp64.hip", []),
     ("ewald_mob.hip", []),
+    ("tune.hip", []),
+    ("ensemble.hip", []),
     ("capi.hip", []),
 ]
 HOST_SOURCES = [("host_linalg.cpp", []), ("host_util.cpp", []), ("molecule_io.cpp", [])]

@@ -6,6 +6,7 @@
 #include "../../include/stokesdt_amd_debug.h"
 #include "brwn_random.hpp"
 #include "diffusion.hpp"
+#include "ensemble.hpp"
 #include "ewald_gpu.hpp"
 #include "mob_gpu.hpp"
 #include "plain_gpu.hpp"
@@ -23,6 +24,7 @@ using namespace sdt;
 
 struct sdt_mob_s { MobGpu* m; };
 struct sdt_brwn_chol_s { BrwnCholGpu* b; };
+struct sdt_ensemble_s { EnsembleGpu* e; };
 
 // the SPME mobility of a handle (entry points that exist only for MobSpme)
 static MobSpmeGpu* spme_of(sdt_mob_t mob) {
@@ -1080,6 +1082,91 @@ int sdt_plain_bd_destroy(sdt_plain_bd_t bd) {
         delete bd->bd;
         delete bd;
     });
+}
+
+// ------------------------------------------------------------- Ensemble ---
+int sdt_ensemble_create(int nsys, const int* npos, const double* pos, const double* rdi,
+                        const double* box_size, const double* xi, const double* rmax,
+                        const int* dim, int porder, int precision, int max_iters, int max_nrhs,
+                        double tol, double steric_r0, double steric_k0, double dt,
+                        int mob_interval, uint64_t seed, sdt_ensemble_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    if (nsys <= 0 || !npos || !pos || !rdi || !box_size || !xi || !rmax || !dim) {
+        set_error("invalid array arguments");
+        return SDT_EINVAL;
+    }
+    return guard([&] {
+        std::vector<EnsembleSystem> sys(nsys);
+        for (int s = 0; s < nsys; s++) {
+            sys[s].npos = npos[s];
+            sys[s].box = box_size[s];
+            sys[s].xi = xi[s];
+            sys[s].rmax = rmax[s];
+            sys[s].dim = dim[s];
+        }
+        EnsembleGpu* e = new EnsembleGpu(sys, pos, rdi, porder, precision, max_iters, max_nrhs,
+                                         tol, steric_r0, steric_k0, dt, mob_interval, seed);
+        *out = new sdt_ensemble_s{e};
+    });
+}
+int sdt_ensemble_init(sdt_ensemble_t e, int autotune, const char* tune_cache) {
+    SDT_CHECK_HANDLE(e);
+    return guard([&] { e->e->init(autotune != 0, tune_cache ? tune_cache : ""); });
+}
+int sdt_ensemble_set_graphs(sdt_ensemble_t e, int on) {
+    SDT_CHECK_HANDLE(e);
+    e->e->set_graphs(on != 0);
+    return SDT_OK;
+}
+int sdt_ensemble_advance(sdt_ensemble_t e, int nsteps, double* step_ms) {
+    SDT_CHECK_HANDLE(e);
+    return guard([&] { e->e->advance(nsteps, step_ms); });
+}
+int sdt_ensemble_get_positions(sdt_ensemble_t e, int sys, double* pos) {
+    SDT_CHECK_HANDLE(e);
+    return guard([&] { e->e->get_positions(sys, pos); });
+}
+int sdt_ensemble_tuning(sdt_ensemble_t e, int sys, int* choices, double* ms) {
+    SDT_CHECK_HANDLE(e);
+    return guard([&] {
+        SDT_REQUIRE(sys >= 0 && sys < e->e->size(), SDT_EINVAL, "system index out of range");
+        const std::vector<TuneResult>& t = e->e->tuning();
+        for (int w = 0; w < 2; w++) {
+            const TuneResult& r = t[2 * sys + w];
+            if (choices) {
+                choices[4 * w] = r.tuning.fft_order;
+                choices[4 * w + 1] = r.tuning.interp_sxt;
+                choices[4 * w + 2] = r.tuning.interp_ch;
+                choices[4 * w + 3] = r.tuning.spread_stage_kb;
+            }
+            if (ms) {
+                ms[2 * w] = r.ms_default;
+                ms[2 * w + 1] = r.ms_best;
+            }
+        }
+    });
+}
+int sdt_ensemble_last_iters(sdt_ensemble_t e, int sys, int* iters, int cap, int* ngroups) {
+    SDT_CHECK_HANDLE(e);
+    return guard([&] {
+        SDT_REQUIRE(sys >= 0 && sys < e->e->size(), SDT_EINVAL, "system index out of range");
+        const std::vector<int>& it = e->e->system(sys)->last_iters();
+        if (ngroups) *ngroups = (int)it.size();
+        for (int i = 0; i < (int)it.size() && i < cap && iters; i++) iters[i] = it[i];
+    });
+}
+int sdt_ensemble_destroy(sdt_ensemble_t e) {
+    if (!e) return SDT_OK;
+    return guard([&] {
+        delete e->e;
+        delete e;
+    });
+}
+int sdt_bd_set_graphs(sdt_bd_t bd, int on) {
+    SDT_CHECK_HANDLE(bd);
+    bd->bd->set_graphs(on != 0);
+    return SDT_OK;
 }
 
 }  // extern "C"

@@ -75,6 +75,20 @@ void MobSpmeGpu::init() {
     inited_ = true;
 }
 
+void MobSpmeGpu::set_tuning(const SpmeTuning& t, int b) {
+    SDT_REQUIRE(inited_, SDT_ESTATE, "set_tuning called before init");
+    const int w = SpmeGpu<float>::width_class(b);
+    if (spme32_) spme32_->set_tuning(t, w);
+    else spme64_->set_tuning(t, w);
+}
+
+SpmeTuning MobSpmeGpu::tuning(int b) const {
+    const int w = SpmeGpu<float>::width_class(b);
+    if (spme32_) return spme32_->tune[w];
+    if (spme64_) return spme64_->tune[w];
+    return SpmeTuning();
+}
+
 void MobSpmeGpu::set_timer(StageTimer* t) {
     timer_ = t;
     if (spme32_) spme32_->timer = t;
@@ -151,6 +165,16 @@ void MobSpmeGpu::apply_packed(int mode, int b, double alpha, const double* X, do
 
 void MobSpmeGpu::mul_vector(int mode, int nrhs, double alpha, int ldf, const double* f,
                             double beta, int ldv, double* v, bool dev) {
+    mul_vector_impl(mode, nrhs, alpha, ldf, f, beta, ldv, v, dev, true);
+}
+
+void MobSpmeGpu::enqueue_mul_vector(int mode, int nrhs, double alpha, int ldf, const double* d_f,
+                                    double beta, int ldv, double* d_v) {
+    mul_vector_impl(mode, nrhs, alpha, ldf, d_f, beta, ldv, d_v, true, false);
+}
+
+void MobSpmeGpu::mul_vector_impl(int mode, int nrhs, double alpha, int ldf, const double* f,
+                                 double beta, int ldv, double* v, bool dev, bool sync) {
     SDT_REQUIRE(updated_, SDT_ESTATE, "mulvector called before update");
     SDT_REQUIRE(mode >= 0 && mode <= 2, SDT_EINVAL, "mode must be full, real or recip");
     SDT_REQUIRE(nrhs >= 1, SDT_EINVAL, "num_rhs must be positive");
@@ -188,7 +212,7 @@ void MobSpmeGpu::mul_vector(int mode, int nrhs, double alpha, int ldf, const dou
         SDT_HIP_CHECK(hipMemcpy2DAsync(v, sizeof(double) * ldv, d_io_v_.get(), sizeof(double) * ldv,
                                        sizeof(double) * 3 * npos_, nrhs, hipMemcpyDeviceToHost, s));
     }
-    SDT_HIP_CHECK(hipStreamSynchronize(s));
+    if (sync || !dev) SDT_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // ----------------------------------------------------------- Lanczos -------
@@ -263,10 +287,10 @@ void lz_eval(LzGroup& g, int iter, double tol) {
     }
 }
 
-// runs f(0 .. n-1) on up to 16 host threads
+// runs f(0 .. n-1) on up to max_threads host threads
 template <typename F>
-void parallel_for(int n, F f) {
-    const int nt = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
+void parallel_for(int n, int max_threads, F f) {
+    const int nt = std::max(1, std::min({n, max_threads, (int)std::thread::hardware_concurrency()}));
     if (nt == 1) {
         for (int i = 0; i < n; i++) f(i);
         return;
@@ -435,7 +459,7 @@ void BrwnLanczosGpu::lanczos_batch(MobGpu* mob, int btot, int gbase, const doubl
         for (int gi = 0; gi < ng; gi++)
             if (!grp[gi].done) todo.push_back(gi);
         const double tol = tol_;
-        parallel_for((int)todo.size(), [&](int t) { lz_eval(grp[todo[t]], pi, tol); });
+        parallel_for((int)todo.size(), eval_threads, [&](int t) { lz_eval(grp[todo[t]], pi, tol); });
         bool all = true;
         for (LzGroup& g : grp) {
             if (!g.done && g.d_iter == pi && g.met) {
@@ -710,7 +734,26 @@ BDGpu::BDGpu(int npos, const double* pos, const double* rdi, double box, double 
     steric_.reset(new StericGpu(npos, rdi, box, r0, k0));
 }
 
-BDGpu::~BDGpu() {}
+BDGpu::~BDGpu() { release_graph(); }
+
+void BDGpu::release_graph() {
+    if (tail_exec_) (void)hipGraphExecDestroy(tail_exec_);
+    if (tail_graph_) (void)hipGraphDestroy(tail_graph_);
+    tail_exec_ = nullptr;
+    tail_graph_ = nullptr;
+    graph_block_ = -1;
+}
+
+// out = D[:, *col] (the displacement column of the step), then the tail
+// advances *col after the position update
+__global__ void select_column_kernel(long long n, const double* __restrict__ D, long long ld,
+                                     const int* __restrict__ col, double* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = D[(size_t)(*col) * ld + i];
+}
+
+__global__ void set_counter_kernel(int* c, int v) { *c = v; }
+__global__ void next_counter_kernel(int* c) { *c += 1; }
 
 void BDGpu::use_random(int num_vecs, int max_nrhs) {
     SDT_REQUIRE(!d_pos_.get(), SDT_ESTATE, "use_random after init");
@@ -732,30 +775,77 @@ void BDGpu::init() {
     d_Z_.reserve((size_t)ldm_ * mob_interval_);
     d_D_.reserve((size_t)ldm_ * mob_interval_);
     d_f_.reserve((size_t)ldm_);
+    d_cur_.reserve((size_t)ldm_);
+    d_irhs_.reserve(1);
     SDT_HIP_CHECK(hipMemcpy(d_pos_.get(), h_pos0_.data(), sizeof(double) * ldm_,
                             hipMemcpyHostToDevice));
+}
+
+bool BDGpu::begin_step() {
+    hipStream_t s = mob_->stream();
+    const int irhs = (int)(istep_ % mob_interval_);
+    const long long blk = istep_ / mob_interval_;
+    // a new interval, or the first step after a start inside an interval
+    // (then D of this interval is computed from the current positions)
+    if (irhs != 0 && blk == d_block_) return false;
+    gaussian_fill_device(seed_, (unsigned)blk, (long long)mob_interval_ * ldm_, d_Z_.get(), s);
+    mob_->update(d_pos_.get(), nullptr, true);
+    if (rnd_) rnd_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
+    else brwn_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
+    d_block_ = blk;
+    hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(1), 0, s, d_irhs_.get(), irhs);
+    SDT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+// disp = dt M f + sqrt(2 dt) D[:, irhs]; pos += disp (simbox_bd.cc:323-339)
+void BDGpu::enqueue_tail(hipStream_t s) {
+    hipLaunchKernelGGL(select_column_kernel, dim3(ceil_div(ldm_, 256)), dim3(256), 0, s,
+                       (long long)ldm_, d_D_.get(), (long long)ldm_, d_irhs_.get(), d_cur_.get());
+    SDT_HIP_CHECK(hipGetLastError());
+    fill(ldm_, 0.0, d_f_.get(), s);
+    if (steric_on_) steric_->accumulate(d_pos_.get(), d_f_.get(), true, s);
+    mob_->enqueue_mul_vector(0, 1, dt_, ldm_, d_f_.get(), sqrt(2.0 * dt_), ldm_, d_cur_.get());
+    axpby(ldm_, 1.0, d_cur_.get(), 1.0, d_pos_.get(), s);
+    hipLaunchKernelGGL(next_counter_kernel, dim3(1), dim3(1), 0, s, d_irhs_.get());
+    SDT_HIP_CHECK(hipGetLastError());
+    warm_block_ = d_block_;
+}
+
+// The first tail of an interval runs eagerly (its buffers then exist at
+// their final sizes); the second captures the tail into a graph, which the
+// rest of the interval replays. A mobility update can move buffers and
+// changes launch sizes, so each interval captures its own graph.
+void BDGpu::tail_step() {
+    hipStream_t s = mob_->stream();
+    if (!use_graph_ || timer_.on || warm_block_ != d_block_) {
+        enqueue_tail(s);
+        return;
+    }
+    if (graph_block_ != d_block_) {
+        release_graph();
+        SDT_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_tail(s);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        SDT_HIP_CHECK(hipStreamEndCapture(s, &tail_graph_));
+        SDT_HIP_CHECK(hipGraphInstantiate(&tail_exec_, tail_graph_, nullptr, nullptr, 0));
+        graph_block_ = d_block_;
+    }
+    SDT_HIP_CHECK(hipGraphLaunch(tail_exec_, s));
 }
 
 void BDGpu::advance(int nsteps) {
     hipStream_t s = mob_->stream();
     for (int k = 0; k < nsteps; k++) {
-        const int irhs = (int)(istep_ % mob_interval_);
-        const long long blk = istep_ / mob_interval_;
-        // a new interval, or the first step after a start inside an interval
-        // (then D of this interval is computed from the current positions)
-        if (irhs == 0 || blk != d_block_) {
-            gaussian_fill_device(seed_, (unsigned)blk, (long long)mob_interval_ * ldm_, d_Z_.get(), s);
-            mob_->update(d_pos_.get(), nullptr, true);
-            if (rnd_) rnd_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
-            else brwn_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
-            d_block_ = blk;
-        }
-        double* disp = d_D_.get() + (size_t)irhs * ldm_;
-        fill(ldm_, 0.0, d_f_.get(), s);
-        if (steric_on_) steric_->accumulate(d_pos_.get(), d_f_.get(), true, s);
-        mob_->mul_vector(0, 1, dt_, ldm_, d_f_.get(), sqrt(2.0 * dt_), ldm_, disp, true);
-        axpby(ldm_, 1.0, disp, 1.0, d_pos_.get(), s);
-        istep_++;
+        begin_step();
+        tail_step();
+        end_step();
     }
     SDT_HIP_CHECK(hipStreamSynchronize(s));
     timer_.collect();
@@ -765,6 +855,8 @@ void BDGpu::set_step(long long istep) {
     SDT_REQUIRE(istep >= 0, SDT_EINVAL, "step must be non-negative");
     istep_ = istep;
     d_block_ = -1;
+    warm_block_ = -1;
+    release_graph();
 }
 
 void BDGpu::get_positions(double* pos) {

@@ -47,6 +47,10 @@ class MobSpmeGpu : public MobGpu {
     void update(const double* pos, const double* rdi, bool dev) override;
     void mul_vector(int mode, int nrhs, double alpha, int ldf, const double* f, double beta,
                     int ldv, double* v, bool dev) override;
+    // the same on device vectors, enqueued on stream() without waiting (no
+    // host synchronisation: capturable into a graph once its buffers exist)
+    void enqueue_mul_vector(int mode, int nrhs, double alpha, int ldf, const double* d_f,
+                            double beta, int ldv, double* d_v);
     // packed, sorted-order FP64 vectors (row-major 3N x b)
     void apply_packed(int mode, int b, double alpha, const double* X, double beta,
                       double* Y) override;
@@ -64,8 +68,16 @@ class MobSpmeGpu : public MobGpu {
     SpmeGpu<double>* spme64() { return spme64_.get(); }
     void set_timer(StageTimer* t);
     StageTimer* timer() const { return timer_; }
+    // launch choices of the reciprocal-space engine for launches of b
+    // columns (the narrow class b < 8 or the wide class b >= 8; after init)
+    void set_tuning(const SpmeTuning& t, int b);
+    SpmeTuning tuning(int b) const;
+    double xi() const { return xi_; }
+    bool mono() const { return mono_; }
 
   private:
+    void mul_vector_impl(int mode, int nrhs, double alpha, int ldf, const double* f, double beta,
+                         int ldv, double* v, bool dev, bool sync);
     template <typename T>
     void apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const double* X, double beta,
                   double* Y);
@@ -99,6 +111,7 @@ class BrwnLanczosGpu {
     const std::vector<int>& iters() const { return iters_; }
     int max_nrhs() const { return max_nrhs_; }
     int reorth = 0;        // 0: none (reference); 1: full FP64 reorthogonalisation
+    int eval_threads = 16; // host threads of the per-group convergence tests
     StageTimer* timer = nullptr;
     // columns of one lockstep batch
     static constexpr int kMaxBatchCols = 128;
@@ -160,14 +173,35 @@ class BDGpu {
     void get_positions(double* pos);
     const std::vector<int>& last_iters() const { return brwn_->iters(); }
     MobSpmeGpu* mob() { return mob_.get(); }
+    BrwnLanczosGpu* lanczos() { return brwn_.get(); }
     long long istep() const { return istep_; }
     // continue from step istep (a restart); a start inside a mobility
     // interval computes that interval's D from the positions at the start
     void set_step(long long istep);
     int npos() const { return npos_; }
     StageTimer& timer() { return timer_; }
+    hipStream_t stream() const { return mob_->stream(); }
+
+    // The parts of one step. begin_step: when the step starts a mobility
+    // interval (or follows a restart), the Gaussian block, the mobility
+    // update and the interval's displacements (synchronous); returns true if
+    // it ran. enqueue_tail: the step's forces, M f and position update on s,
+    // with no host synchronisation (the displacement column comes from a
+    // device counter that the tail advances), so it can be captured into a
+    // graph once its buffers exist. end_step: the step counter.
+    bool begin_step();
+    void enqueue_tail(hipStream_t s);
+    void end_step() { istep_++; }
+    // interval whose buffers an eager tail has touched (-1: none yet)
+    long long warm_block() const { return warm_block_; }
+    long long block() const { return d_block_; }
+    // tails as one replayed HIP graph per interval (default on; profiling
+    // runs them eagerly, with per-stage events)
+    void set_graphs(bool on) { use_graph_ = on; }
 
   private:
+    void tail_step();
+    void release_graph();
     int npos_;
     double box_, dt_;
     int mob_interval_;
@@ -177,12 +211,18 @@ class BDGpu {
     std::unique_ptr<BrwnLanczosGpu> brwn_;
     std::unique_ptr<BrwnRandomGpu> rnd_;
     std::unique_ptr<StericGpu> steric_;
-    DevBuf<double> d_pos_, d_Z_, d_D_, d_f_;
+    DevBuf<double> d_pos_, d_Z_, d_D_, d_f_, d_cur_;
+    DevBuf<int> d_irhs_;       // displacement column of the next tail
     long long istep_ = 0;
     long long d_block_ = -1;   // interval whose displacements d_D_ holds
+    long long warm_block_ = -1;
     bool steric_on_ = true;    // r0 > 0 and k0 != 0
     int ldm_ = 0;
     StageTimer timer_;
+    bool use_graph_ = true;
+    hipGraph_t tail_graph_ = nullptr;
+    hipGraphExec_t tail_exec_ = nullptr;
+    long long graph_block_ = -1;   // interval the instantiated tail graph belongs to
 };
 
 // Philox-4x32-10 Gaussian block on the device (same stream as the oracle):

@@ -122,7 +122,9 @@ struct FFTPlan {
     int nst = 0;
     int radix[16] = {0};
 };
-bool make_fft_plan(int K, FFTPlan* plan);
+// order: 0 radix 8 first, then 4, 2, 3, 5; 1 radix 4 first (no radix 8);
+// 2 radices 3 and 5 first, then 8, 4, 2
+bool make_fft_plan(int K, FFTPlan* plan, int order = 0);
 
 }  // namespace sdt
 

@@ -19,6 +19,21 @@ struct SortedSystem;   // particle order shared by SPME and real space
 // denominator (odd P at the Nyquist index)
 void bspline_moduli(int K, int P, std::vector<double>& out);
 
+// Launch choices of one SPME engine that leave the operator unchanged (up to
+// the rounding of the FFT's stage order): picked by the autotuner, 0 = the
+// built-in heuristic.
+struct SpmeTuning {
+    int fft_order = 0;        // generic radix chain: 0 radix 8 first, then 4, 2, 3, 5;
+                              // 1 radix 4 first (no 8); 2 radices 3 and 5 first
+    int interp_sxt = 0;       // interp strip width in tiles (1, 2, 4)
+    int interp_ch = 0;        // interp particle chunk per pass (32 .. 128)
+    int spread_stage_kb = 0;  // spread candidate stage in KB (32, 48, 64)
+    bool operator==(const SpmeTuning& o) const {
+        return fft_order == o.fft_order && interp_sxt == o.interp_sxt &&
+               interp_ch == o.interp_ch && spread_stage_kb == o.spread_stage_kb;
+    }
+};
+
 template <typename T>
 struct SpmeGpu {
     int npos = 0, K = 0, porder = 0;
@@ -27,7 +42,14 @@ struct SpmeGpu {
     int ts = 8;            // spread/interp tile side (mesh points)
     int ntile = 0;         // tiles per dimension
     int bcap = 0;          // RHS capacity of the mesh buffers
-    FFTPlan plan;
+    FFTPlan plan;          // the default radix plan
+    // launch choices and FFT plans of narrow (width class 0: b < 8) and wide
+    // (1: b >= 8) launches
+    SpmeTuning tune[2];
+    FFTPlan plans[2];
+    static int width_class(int b) { return b >= 8 ? 1 : 0; }
+    // replaces the launch choices of a width class (re-plans the FFT order)
+    void set_tuning(const SpmeTuning& t, int width);
     DevBuf<double> d_tw;   // K roots of unity (cos, sin) of -2*pi*j/K
     DevBuf<double> d_bsp2; // |b(m)|^2 per wavenumber index, K entries
     DevBuf<double> d_frac; // 3N fractional offsets (sorted order)

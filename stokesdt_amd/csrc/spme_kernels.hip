@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include "fft_lds.hpp"
 #include "real_gpu.hpp"
 #include "spme_gpu.hpp"
@@ -1267,16 +1269,36 @@ void bspline_moduli(int K, int P, std::vector<double>& out) {
     for (int m = 0; m < K; m++) out[m] = 1.0 / den2[m];
 }
 
-bool make_fft_plan(int K, FFTPlan* plan) {
+bool make_fft_plan(int K, FFTPlan* plan, int order) {
     plan->K = K;
     plan->nst = 0;
     int m = K;
-    while (m % 8 == 0) { plan->radix[plan->nst++] = 8; m /= 8; }
-    while (m % 4 == 0) { plan->radix[plan->nst++] = 4; m /= 4; }
-    while (m % 2 == 0) { plan->radix[plan->nst++] = 2; m /= 2; }
-    while (m % 3 == 0) { plan->radix[plan->nst++] = 3; m /= 3; }
-    while (m % 5 == 0) { plan->radix[plan->nst++] = 5; m /= 5; }
-    return m == 1 && plan->nst <= 16;
+    auto take = [&](int r) {
+        while (m % r == 0 && plan->nst < 16) { plan->radix[plan->nst++] = r; m /= r; }
+    };
+    if (order == 2) { take(3); take(5); }
+    if (order != 1) take(8);
+    take(4);
+    take(2);
+    take(3);
+    take(5);
+    return m == 1;
+}
+
+template <typename T>
+void SpmeGpu<T>::set_tuning(const SpmeTuning& t, int width) {
+    FFTPlan p;
+    SDT_REQUIRE(t.fft_order >= 0 && t.fft_order <= 2 && make_fft_plan(K, &p, t.fft_order),
+                SDT_EINVAL, "tuning: invalid FFT stage order");
+    SDT_REQUIRE(t.spread_stage_kb == 0 || (t.spread_stage_kb >= 16 && t.spread_stage_kb <= 64),
+                SDT_EINVAL, "tuning: spread stage must be 16 .. 64 KB");
+    SDT_REQUIRE(t.interp_sxt == 0 || t.interp_sxt == 1 || t.interp_sxt == 2 || t.interp_sxt == 4,
+                SDT_EINVAL, "tuning: interp strip width must be 1, 2 or 4 tiles");
+    SDT_REQUIRE(t.interp_ch == 0 || (t.interp_ch >= 32 && t.interp_ch <= 128), SDT_EINVAL,
+                "tuning: interp chunk must be 32 .. 128");
+    SDT_REQUIRE(width == 0 || width == 1, SDT_EINVAL, "tuning: width class must be 0 or 1");
+    plans[width] = p;
+    tune[width] = t;
 }
 
 static FFTArgs fft_args(const FFTPlan& p) {
@@ -1313,6 +1335,8 @@ void SpmeGpu<T>::create(int npos_, double box_, double xi_, int K_, int porder_,
     SDT_REQUIRE(make_fft_plan(K, &plan), SDT_EINVAL,
                 "mesh dimension must factor into 2, 3 and 5 on the GPU path");
     SDT_REQUIRE(K <= 1024, SDT_EINVAL, "mesh dimension must be <= 1024");
+    plans[0] = plans[1] = plan;
+    tune[0] = tune[1] = SpmeTuning();
     // tile side: a divisor of K in [P-1, 8], preferring 8 (one wave owns
     // ts x ts mesh rows)
     ts = 0;
@@ -1407,6 +1431,7 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
     SDT_REQUIRE(xb == nullptr || blocked_cols(b) > 0, SDT_EINVAL,
                 "spread: no blocked input for this configuration");
     if (point_owner()) {
+        const SpmeTuning& tn = tune[width_class(b)];
         constexpr int NQL = sizeof(T) == 4 ? 48 : 24;
         const int NQ = pt_meshes<T>(b, NC);
         // FP32 reads the blocked forces through the scalar path (measured
@@ -1414,7 +1439,8 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
         const bool sgf = xb != nullptr && sizeof(T) == 4;
         const int rec = ((3 * (porder + 1) + 3 + 3) & ~3) + (sgf ? 0 : NQ);
         // few meshes: a 48 KB stage leaves room for three workgroups per CU
-        const int stage_bytes = NQ <= 12 ? 48 * 1024 : 65536;
+        const int stage_bytes =
+            tn.spread_stage_kb > 0 ? tn.spread_stage_kb * 1024 : (NQ <= 12 ? 48 * 1024 : 65536);
         const int cap = (int)(stage_bytes / (rec * sizeof(T) + 9 * sizeof(int)));
         const size_t lds = (size_t)cap * (rec * sizeof(T) + 9 * sizeof(int));
         const int nsx = (K + 31) / 32;
@@ -1524,13 +1550,21 @@ static int plane_path(int K) {
 
 template <typename T>
 static void allow_lds(const void* fn, size_t bytes) {
-    if (bytes > 64 * 1024)
-        SDT_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    if (bytes <= 64 * 1024) return;
+    // raised once per kernel and size: later launches (and graph captures of
+    // them) make no attribute call
+    static std::mutex mu;
+    static std::map<const void*, size_t> raised;
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& have = raised[fn];
+    if (have >= bytes) return;
+    SDT_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    have = bytes;
 }
 
 template <typename T>
 void SpmeGpu<T>::fft_forward(int b, hipStream_t s) {
-    const FFTArgs fa = fft_args(plan);
+    const FFTArgs fa = fft_args(plans[width_class(b)]);
     const int Kh = K / 2 + 1;
     const long long nrows = (long long)nmesh(b) * K * K;
     if (const int pv = plane_path<T>(K)) {
@@ -1571,7 +1605,7 @@ void SpmeGpu<T>::fft_forward(int b, hipStream_t s) {
 
 template <typename T>
 void SpmeGpu<T>::influence(int b, hipStream_t s) {
-    const FFTArgs fa = fft_args(plan);
+    const FFTArgs fa = fft_args(plans[width_class(b)]);
     const int Kh = K / 2 + 1;
     const int NC = poly ? 6 : 3;
     const int nj = K * Kh;
@@ -1608,7 +1642,7 @@ void SpmeGpu<T>::influence(int b, hipStream_t s) {
 
 template <typename T>
 void SpmeGpu<T>::fft_backward(int b, hipStream_t s) {
-    const FFTArgs fa = fft_args(plan);
+    const FFTArgs fa = fft_args(plans[width_class(b)]);
     const int Kh = K / 2 + 1;
     const long long nrows = (long long)nmesh(b) * K * K;
     if (const int pv = plane_path<T>(K)) {
@@ -1659,6 +1693,9 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
     // strips; FP64 two meshes (16-byte LDS vectors) over 2-tile strips
     constexpr int MV = sizeof(T) == 8 ? 2 : 1;
     int sxt = sizeof(T) == 8 ? 2 : 4, ch = sizeof(T) == 8 ? 64 : 128;
+    const SpmeTuning& tn = tune[width_class(b)];
+    if (tn.interp_sxt > 0) sxt = tn.interp_sxt;
+    if (tn.interp_ch > 0) ch = tn.interp_ch;
     auto lds_of = [&](int sx, int c) {
         const size_t nreg = (size_t)(sx * ts + R) * (ts + R) * (ts + R);
         return nreg * MV * sizeof(T) + (size_t)c * porder * MV * sizeof(T) +
