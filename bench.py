@@ -49,8 +49,24 @@ ENS = dict(nsys=64, nmin=500, nmax=20_000, phimin=0.05, phimax=0.35, kmin=24, km
            k0=125.0, seed=33)
 
 
+# serving loop: seeded Poisson arrivals
+SERVE = dict(rate=50.0, nmin=1000, nmax=200_000, phimin=0.05, phimax=0.3, h=0.8, p=6, xi=0.896,
+             rmax=5.0, b=8, tol=1e-6, max_iters=100, steps=100, dt=1e-3, mob_interval=100,
+             r0=2.0, k0=125.0, seed=55, horizon=6.0, deadline=20.0, max_active=16,
+             capacity=288e9)
+
+
 def box_size(n, phi):
     return (4.0 * np.pi * n / (3.0 * phi)) ** (1.0 / 3.0)
+
+
+T_START = time.perf_counter()
+
+
+def note(msg):
+    """A progress line on stderr (the JSON result goes to stdout)."""
+    print("[bench %6.1f s] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr,
+          flush=True)
 
 
 def smooth_mesh(L, kmin, kmax, h=0.8):
@@ -79,6 +95,76 @@ def ensemble_systems(S):
                         xi=e["xi"], rmax=e["rmax"], dim=smooth_mesh(L, e["kmin"], e["kmax"]),
                         phi=phi))
     return out
+
+
+def lattice_positions(rng, n, L):
+    """n non-overlapping unit spheres: a random subset of the sites of an
+    m^3 simple cubic lattice (m = ceil(n^(1/3)), spacing L/m > 2 for phi <=
+    0.3 and n >= 1000), each jittered within the free gap."""
+    m = int(np.ceil(n ** (1.0 / 3.0) - 1e-9))
+    while m ** 3 < n:
+        m += 1
+    a = L / m
+    if a <= 2.0:
+        raise ValueError("lattice spacing %.3f leaves no room for the spheres" % a)
+    sites = rng.choice(m ** 3, size=n, replace=False)
+    ijk = np.stack([sites // (m * m), (sites // m) % m, sites % m], axis=1).astype(np.float64)
+    j = 0.45 * (a - 2.0)
+    return ((ijk + 0.5) * a + rng.uniform(-j, j, (n, 3))).reshape(-1)
+
+
+def serve_requests(horizon):
+    """Requests of the serving run: Poisson arrivals at 50 per second over
+    `horizon` seconds; N log-uniform in [1000, 200000], phi uniform in
+    [0.05, 0.3], mesh the even 2^a 3^b 5^c nearest to L / 0.8 (at least
+    24), 100 steps each (seeded)."""
+    c = SERVE
+    rng = np.random.default_rng(c["seed"])
+    out = []
+    t = 0.0
+    while True:
+        t += float(rng.exponential(1.0 / c["rate"]))
+        if t >= horizon:
+            break
+        n = int(round(np.exp(rng.uniform(np.log(c["nmin"]), np.log(c["nmax"])))))
+        phi = float(rng.uniform(c["phimin"], c["phimax"]))
+        L = box_size(n, phi)
+        out.append(dict(arrival=t, n=n, L=L, dim=smooth_mesh(L, 24, 512, c["h"]),
+                        pos=lattice_positions(rng, n, L), seed=int(rng.integers(1 << 62))))
+    return out
+
+
+def run_serving(S, kind, requests):
+    """One serving run over the request trace until the deadline."""
+    c = SERVE
+    free, total = S.device_memory()
+    cap = min(c["capacity"], 0.92 * free)
+    sv = S.Server(kind, cap, c["max_active"], c["p"], c["max_iters"], c["b"], c["tol"], c["r0"],
+                  c["k0"], c["dt"], c["mob_interval"])
+    for r in requests:
+        sv.submit(r["pos"], np.ones(r["n"]), r["L"], c["xi"], c["rmax"], r["dim"], c["steps"],
+                  r["seed"], r["arrival"])
+    sv.run(c["deadline"])
+    st = sv.stats()
+    jobs = [sv.job(k) for k in range(len(requests))]
+    lat = np.concatenate([sv.step_ms(k) for k in range(len(requests))] + [np.zeros(0)])
+    steps = sum(j["steps"] for j in jobs)
+    psteps = sum(j["steps"] * r["n"] for j, r in zip(jobs, requests))
+    done = [j for j in jobs if j["state"] == "done"]
+    adm = [j for j in jobs if j["admit"] >= 0 and j["state"] != "failed"]
+    wait = [j["admit"] - j["arrival"] for j in adm]
+    return dict(
+        requests=len(requests), completed=len(done),
+        completed_per_s=round(len(done) / st["wall_s"], 3),
+        admitted=len(adm), refused=sum(j["state"] == "failed" for j in jobs),
+        still_queued=sum(j["state"] == "queued" for j in jobs),
+        steps_per_s=round(steps / st["wall_s"], 2),
+        particle_steps_per_s=round(psteps / st["wall_s"], 1),
+        step_latency_ms_p50=round(float(np.percentile(lat, 50)), 3) if lat.size else None,
+        step_latency_ms_p99=round(float(np.percentile(lat, 99)), 3) if lat.size else None,
+        queue_wait_s_p50=round(float(np.percentile(wait, 50)), 3) if wait else None,
+        wall_s=round(st["wall_s"], 3), capacity_gb=round(cap / 1e9, 1),
+        peak_reserved_gb=round(st["peak_bytes"] / 1e9, 2), peak_jobs_on_device=st["peak_active"])
 
 
 def run_ensemble(S, steps, warmup):
@@ -427,6 +513,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU path")
     ap.add_argument("--no-err", action="store_true", help="skip the FP32-vs-FP64 error check")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-system ensemble")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
     ap.add_argument("--kernel-sample", action="store_true",
                     help="run only the short kernel sample used for counter collection")
     args = ap.parse_args()
@@ -451,8 +538,10 @@ def main():
     pos = S.rsa_positions(n, L, 1.0, c["seed"])
     rdi = np.ones(n)
 
+    note("FP32-mixed window")
     r32 = run_gpu(S, pos, rdi, L, S.FP32, args.steps, args.warmup)
     value = r32["value"]
+    note("FP32-mixed: %.2f steps/s" % value)
     res = dict(metric=METRIC, value=round(value, 4), unit="steps/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(1000.0 / value, 4),
                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="fp32",
@@ -480,6 +569,7 @@ def main():
             O = None
             res["oracle_error"] = str(exc)
     if not args.no_fp64:
+        note("FP64 window")
         r64 = run_gpu(S, pos, rdi, L, S.FP64, args.steps, args.warmup)
         res["fp64_gpu_steps_per_s"] = round(r64["value"], 4)
         res["speedup_fp32_vs_fp64_gpu"] = round(value / r64["value"], 3)
@@ -489,6 +579,7 @@ def main():
         res["trajectory_disp_err_fp32_vs_fp64"] = float(np.linalg.norm(d32 - d64) /
                                                         np.linalg.norm(d64))
         if not args.no_plain:
+            note("plain FP64 window")
             rp = run_plain(S, pos, rdi, L, args.steps)
             res["plain_fp64_gpu"] = dict(
                 steps_per_s=round(rp["value"], 4), window=window_info(rp, mi),
@@ -500,15 +591,33 @@ def main():
             res["speedup_vs_plain_fp64_gpu"] = round(value / rp["value"], 2)
             res["north_star_speedup_target"] = 10.0
     if not args.no_ensemble:
+        note("ensemble")
         ens = run_ensemble(S, args.steps, args.warmup)
         res["ensemble"] = ens
         res["ensemble_step_latency_ms_p50"] = ens["step_latency_ms_p50"]
         res["ensemble_step_latency_ms_p99"] = ens["step_latency_ms_p99"]
+    if not args.no_serve:
+        note("serving requests")
+        reqs = serve_requests(SERVE["horizon"])
+        note("serving FP32-mixed (%d requests)" % len(reqs))
+        sv32 = run_serving(S, S.SERVE_FP32, reqs)
+        res["serving"] = dict(
+            fp32_mixed=sv32,
+            workload="Poisson arrivals at 50 req/s for %.0f s (deadline %.0f s): N log-uniform "
+                     "[1000, 200000], phi U[0.05, 0.3], mesh nearest even 2^a3^b5^c to L/0.8, "
+                     "p=6 rc=5 xi=0.896, 100 steps dt=1e-3, block Lanczos b=8 tol=1e-6, "
+                     "mob-interval=100; admission by predicted footprint against min(288 GB, "
+                     "free device memory)" % (SERVE["horizon"], SERVE["deadline"]))
+        if not args.no_plain:
+            note("serving plain FP64")
+            res["serving"]["plain_fp64"] = run_serving(S, S.SERVE_PLAIN, reqs)
     if not args.no_err:
+        note("FP32-vs-FP64 errors")
         e_mf, e_d = displacement_error(S, O, pos, rdi, L)
         res["mf_err_fp32_vs_fp64"] = e_mf
         res["brownian_disp_err_fp32_vs_fp64"] = e_d
     if not args.no_cpu and O is not None:
+        note("CPU baseline")
         cb = cpu_baseline(O, pos, rdi, L, r32["iters"])
         res["cpu_baseline"] = dict(value=round(cb["value"], 6), unit=cb["unit"], cores=cb["cores"],
                                    kind=cb["kind"], sample=cb["sample"], model=cb["model"],

@@ -285,6 +285,47 @@ int sdt_ensemble_tuning(sdt_ensemble_t e, int sys, int* choices, double* ms);
 int sdt_ensemble_last_iters(sdt_ensemble_t e, int sys, int* iters, int cap, int* ngroups);
 int sdt_ensemble_destroy(sdt_ensemble_t e);
 
+/* --------------------------------------------------------------- Serving
+ * A request loop for BD jobs (no counterpart in the reference, whose only
+ * run loop is bin/main/main.cc:129-158): continuous batching on one GPU.
+ * kind: 0 optimised FP64, 1 FP32-mixed, 2 plain FP64 path. A job is a
+ * system (npos particles, box, xi, rmax, mesh dim) asking for nsteps BD
+ * steps (the step of src/simbox/simbox_bd.cc:296-366 with the shared
+ * porder, Lanczos, steric, dt and mob_interval), its Gaussian stream from
+ * seed, arriving arrival_s seconds after sdt_serve_run starts.
+ * sdt_serve_run admits arrived jobs first come first served while a slot
+ * (max_active) is free and the job's predicted device footprint
+ * (sdt_serve_footprint) fits what is left of capacity_bytes; a job whose
+ * footprint exceeds the whole capacity is refused (state 3). Every admitted
+ * job steps on its own stream and host thread with its state resident on
+ * the device, so jobs admitted while others run overlap with them; its
+ * reservation is released when it ends. Returns when every job has ended, or
+ * at deadline_s (> 0), when running jobs stop after their current step
+ * (state 4) and queued ones stay queued (state 0).
+ * sdt_serve_job: state (0 queued, 1 running, 2 done, 3 failed, 4 stopped),
+ * times[3] = {arrival, admission, end} in seconds (-1: not yet), the steps
+ * done and the reserved footprint; a failed job's message goes to
+ * sdt_last_error. sdt_serve_job_positions: the positions after the last
+ * step (3 npos). sdt_serve_job_step_ms: host wall time of every step.
+ * sdt_serve_stats: stats[3] = {wall seconds of the run, peak reserved bytes,
+ * peak jobs on the device}. */
+typedef struct sdt_serve_s* sdt_serve_t;
+int sdt_serve_create(int kind, double capacity_bytes, int max_active, int porder, int max_iters,
+                     int max_nrhs, double tol, double steric_r0, double steric_k0, double dt,
+                     int mob_interval, sdt_serve_t* out);
+int sdt_serve_footprint(sdt_serve_t sv, int npos, int dim, double box_size, double rmax,
+                        double* bytes);
+int sdt_serve_submit(sdt_serve_t sv, int npos, const double* pos, const double* rdi,
+                     double box_size, double xi, double rmax, int dim, int nsteps, uint64_t seed,
+                     double arrival_s, int* job);
+int sdt_serve_run(sdt_serve_t sv, double deadline_s);
+int sdt_serve_job(sdt_serve_t sv, int job, int* state, double* times, int* steps_done,
+                  double* footprint_bytes);
+int sdt_serve_job_positions(sdt_serve_t sv, int job, double* pos);
+int sdt_serve_job_step_ms(sdt_serve_t sv, int job, double* ms, int cap, int* n);
+int sdt_serve_stats(sdt_serve_t sv, double* stats);
+int sdt_serve_destroy(sdt_serve_t sv);
+
 /* ---------------------------------------------------------------- SimBox
  * Replaces BDSimBox(config_file, model_file, xyz_file)
  * (src/simbox/simbox_bd.cc:31-294) with Init and Advance (:296-366), and the
@@ -324,6 +365,8 @@ int sdt_simbox_destroy(sdt_simbox_t sb);
  * element e of (seed, stream); replaces RndStream::Gaussian
  * (src/util/rnd_stream.cc:44-65), whose MKL stream cannot be reproduced. */
 int sdt_gaussian(uint64_t seed, uint32_t stream, int64_t n, double* out);
+/* Free and total memory of the current device, in bytes. */
+int sdt_device_memory(double* free_bytes, double* total_bytes);
 /* Block until all work of every handle finished. */
 int sdt_device_synchronize(void);
 

@@ -133,6 +133,16 @@ SIGNATURES = {
     "sdt_ensemble_tuning": (_I, [_P, _I, _P, _P]),
     "sdt_ensemble_last_iters": (_I, [_P, _I, _P, _I, _PI]),
     "sdt_ensemble_destroy": (_I, [_P]),
+    "sdt_serve_create": (_I, [_I, _D, _I, _I, _I, _I, _D, _D, _D, _D, _I, ctypes.POINTER(_P)]),
+    "sdt_serve_footprint": (_I, [_P, _I, _I, _D, _D, ctypes.POINTER(_D)]),
+    "sdt_serve_submit": (_I, [_P, _I, _P, _P, _D, _D, _D, _I, _I, ctypes.c_uint64, _D, _PI]),
+    "sdt_serve_run": (_I, [_P, _D]),
+    "sdt_serve_job": (_I, [_P, _I, _PI, _P, _PI, ctypes.POINTER(_D)]),
+    "sdt_serve_job_positions": (_I, [_P, _I, _P]),
+    "sdt_serve_job_step_ms": (_I, [_P, _I, _P, _I, _PI]),
+    "sdt_serve_stats": (_I, [_P, _P]),
+    "sdt_serve_destroy": (_I, [_P]),
+    "sdt_device_memory": (_I, [ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "sdt_plain_mob_create": (_I, [_I, _P, _D, _D, _D, _I, _I, ctypes.POINTER(_P)]),
     "sdt_plain_mob_update": (_I, [_P, _P]),
     "sdt_plain_mob_mulvector": (_I, [_P, _I, _I, _D, _I, _P, _D, _I, _P]),
@@ -888,4 +898,82 @@ class Ensemble:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             lib().sdt_ensemble_destroy(self.h)
+            self.h = None
+
+
+SERVE_FP64, SERVE_FP32, SERVE_PLAIN = 0, 1, 2
+
+
+def device_memory():
+    """(free, total) bytes of the current device."""
+    f, t = _D(), _D()
+    _check(lib().sdt_device_memory(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+JOB_STATES = ("queued", "running", "done", "failed", "stopped")
+
+
+class Server:
+    """Request loop for BD jobs with continuous batching (sdt_serve_*):
+    jobs are admitted first come first served while a slot is free and their
+    predicted device footprint fits the capacity left; each runs on its own
+    stream, resident on the device."""
+
+    def __init__(self, kind, capacity_bytes, max_active, porder=6, max_iters=100, max_nrhs=8,
+                 tol=1e-6, steric_r0=2.0, steric_k0=125.0, dt=1e-3, mob_interval=100):
+        h = _P()
+        _check(lib().sdt_serve_create(kind, float(capacity_bytes), max_active, porder, max_iters,
+                                      max_nrhs, tol, steric_r0, steric_k0, dt, mob_interval,
+                                      ctypes.byref(h)))
+        self.h = h
+        self.npos = []
+
+    def footprint(self, npos, dim, box, rmax):
+        b = _D()
+        _check(lib().sdt_serve_footprint(self.h, npos, dim, box, rmax, ctypes.byref(b)))
+        return b.value
+
+    def submit(self, pos, rdi, box, xi, rmax, dim, nsteps, seed, arrival):
+        pos = _f64(pos)
+        rdi = _f64(rdi)
+        j = ctypes.c_int(0)
+        _check(lib().sdt_serve_submit(self.h, rdi.size, _ptr(pos), _ptr(rdi), box, xi, rmax, dim,
+                                      nsteps, seed, arrival, ctypes.byref(j)))
+        self.npos.append(rdi.size)
+        return j.value
+
+    def run(self, deadline=0.0):
+        _check(lib().sdt_serve_run(self.h, float(deadline)))
+
+    def job(self, j):
+        st, nd = ctypes.c_int(0), ctypes.c_int(0)
+        t = np.zeros(3)
+        b = _D()
+        _check(lib().sdt_serve_job(self.h, j, ctypes.byref(st), _ptr(t), ctypes.byref(nd),
+                                   ctypes.byref(b)))
+        out = dict(state=JOB_STATES[st.value], arrival=t[0], admit=t[1], finish=t[2],
+                   steps=nd.value, bytes=b.value)
+        if st.value == 3:
+            out["error"] = lib().sdt_last_error().decode()
+        return out
+
+    def positions(self, j):
+        out = np.zeros(3 * self.npos[j])
+        _check(lib().sdt_serve_job_positions(self.h, j, _ptr(out)))
+        return out
+
+    def step_ms(self, j):
+        n = ctypes.c_int(0)
+        _check(lib().sdt_serve_job_step_ms(self.h, j, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1))
+        _check(lib().sdt_serve_job_step_ms(self.h, j, _ptr(out), n.value, ctypes.byref(n)))
+        return out[:n.value]
+
+    def stats(self):
+        s = np.zeros(3)
+        _check(lib().sdt_serve_stats(self.h, _ptr(s)))
+        return dict(wall_s=s[0], peak_bytes=s[1], peak_active=int(s[2]))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            lib().sdt_serve_destroy(self.h)
             self.h = None

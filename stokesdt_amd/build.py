@@ -31,6 +31,7 @@ HIP_SOURCES = [
     ("ewald_mob.hip", []),
     ("tune.hip", []),
     ("ensemble.hip", []),
+    ("serve.hip", []),
     ("capi.hip", []),
 ]
 HOST_SOURCES = [("host_linalg.cpp", []), ("host_util.cpp", []), ("molecule_io.cpp", [])]

@@ -10,6 +10,7 @@
 #include "ewald_gpu.hpp"
 #include "mob_gpu.hpp"
 #include "plain_gpu.hpp"
+#include "serve.hpp"
 #include "simbox.hpp"
 
 namespace sdt {
@@ -25,6 +26,7 @@ using namespace sdt;
 struct sdt_mob_s { MobGpu* m; };
 struct sdt_brwn_chol_s { BrwnCholGpu* b; };
 struct sdt_ensemble_s { EnsembleGpu* e; };
+struct sdt_serve_s { ServeGpu* s; ServeParams p; };
 
 // the SPME mobility of a handle (entry points that exist only for MobSpme)
 static MobSpmeGpu* spme_of(sdt_mob_t mob) {
@@ -1167,6 +1169,107 @@ int sdt_bd_set_graphs(sdt_bd_t bd, int on) {
     SDT_CHECK_HANDLE(bd);
     bd->bd->set_graphs(on != 0);
     return SDT_OK;
+}
+
+// ---------------------------------------------------------------- Serving --
+int sdt_serve_create(int kind, double capacity_bytes, int max_active, int porder, int max_iters,
+                     int max_nrhs, double tol, double steric_r0, double steric_k0, double dt,
+                     int mob_interval, sdt_serve_t* out) {
+    if (!out) { set_error("null output handle"); return SDT_EINVAL; }
+    *out = nullptr;
+    return guard([&] {
+        ServeParams p;
+        p.kind = kind;
+        p.capacity = capacity_bytes;
+        p.max_active = max_active;
+        p.porder = porder;
+        p.max_iters = max_iters;
+        p.max_nrhs = max_nrhs;
+        p.tol = tol;
+        p.r0 = steric_r0;
+        p.k0 = steric_k0;
+        p.dt = dt;
+        p.mob_interval = mob_interval;
+        *out = new sdt_serve_s{new ServeGpu(p), p};
+    });
+}
+int sdt_serve_footprint(sdt_serve_t sv, int npos, int dim, double box_size, double rmax,
+                        double* bytes) {
+    SDT_CHECK_HANDLE(sv);
+    if (!bytes || npos <= 0 || dim <= 0 || box_size <= 0.0) {
+        set_error("invalid arguments");
+        return SDT_EINVAL;
+    }
+    *bytes = (double)serve_footprint(sv->p, npos, dim, box_size, rmax);
+    return SDT_OK;
+}
+int sdt_serve_submit(sdt_serve_t sv, int npos, const double* pos, const double* rdi,
+                     double box_size, double xi, double rmax, int dim, int nsteps, uint64_t seed,
+                     double arrival_s, int* job) {
+    SDT_CHECK_HANDLE(sv);
+    return guard([&] {
+        const int id = sv->s->submit(npos, pos, rdi, box_size, xi, rmax, dim, nsteps, seed,
+                                     arrival_s);
+        if (job) *job = id;
+    });
+}
+int sdt_serve_run(sdt_serve_t sv, double deadline_s) {
+    SDT_CHECK_HANDLE(sv);
+    return guard([&] { sv->s->run(deadline_s); });
+}
+int sdt_serve_job(sdt_serve_t sv, int job, int* state, double* times, int* steps_done,
+                  double* footprint_bytes) {
+    SDT_CHECK_HANDLE(sv);
+    if (job < 0 || job >= sv->s->njobs()) { set_error("job index out of range"); return SDT_EINVAL; }
+    const ServeJob& j = sv->s->job(job);
+    if (state) *state = j.state;
+    if (times) {
+        times[0] = j.arrival;
+        times[1] = j.admit;
+        times[2] = j.finish;
+    }
+    if (steps_done) *steps_done = j.steps_done;
+    if (footprint_bytes) *footprint_bytes = (double)j.bytes;
+    if (j.state == ServeJob::kFailed) set_error(j.error);
+    return SDT_OK;
+}
+int sdt_serve_job_positions(sdt_serve_t sv, int job, double* pos) {
+    SDT_CHECK_HANDLE(sv);
+    if (job < 0 || job >= sv->s->njobs() || !pos) { set_error("invalid arguments"); return SDT_EINVAL; }
+    const ServeJob& j = sv->s->job(job);
+    std::memcpy(pos, j.pos.data(), sizeof(double) * j.pos.size());
+    return SDT_OK;
+}
+int sdt_serve_job_step_ms(sdt_serve_t sv, int job, double* ms, int cap, int* n) {
+    SDT_CHECK_HANDLE(sv);
+    if (job < 0 || job >= sv->s->njobs()) { set_error("job index out of range"); return SDT_EINVAL; }
+    const ServeJob& j = sv->s->job(job);
+    if (n) *n = (int)j.step_ms.size();
+    for (int i = 0; i < (int)j.step_ms.size() && i < cap && ms; i++) ms[i] = j.step_ms[i];
+    return SDT_OK;
+}
+int sdt_serve_stats(sdt_serve_t sv, double* stats) {
+    SDT_CHECK_HANDLE(sv);
+    if (!stats) { set_error("null output"); return SDT_EINVAL; }
+    stats[0] = sv->s->wall();
+    stats[1] = (double)sv->s->peak_bytes();
+    stats[2] = sv->s->peak_active();
+    return SDT_OK;
+}
+int sdt_device_memory(double* free_bytes, double* total_bytes) {
+    return guard([&] {
+        size_t f = 0, t = 0;
+        SDT_HIP_CHECK(hipMemGetInfo(&f, &t));
+        if (free_bytes) *free_bytes = (double)f;
+        if (total_bytes) *total_bytes = (double)t;
+    });
+}
+int sdt_serve_destroy(sdt_serve_t sv) {
+    if (!sv) return SDT_OK;
+    return guard([&] {
+        delete sv->s;
+        delete sv;
+    });
 }
 
 }  // extern "C"

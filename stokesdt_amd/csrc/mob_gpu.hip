@@ -474,6 +474,10 @@ void BrwnLanczosGpu::lanczos_batch(MobGpu* mob, int btot, int gbase, const doubl
     int iter;
     bool all = false;
     for (iter = 0; iter < max_iters_ && !all; iter++) {
+        if (cancel && cancel->load()) {
+            SDT_HIP_CHECK(hipStreamSynchronize(s));
+            throw Error{SDT_ESTATE, "cancelled"};
+        }
         double* Vi = V + (size_t)iter * blk;
         double* W = V + (size_t)(iter + 1) * blk;
         double* rec = d_rec_.get() + (size_t)(iter & 1) * rsz;

@@ -1,6 +1,7 @@
 // GPU mobility (MobSpme), block Lanczos (BrwnLanczos), steric force and the
 // BD step driver (BDSimBox::Advance) behind the C ABI.
 #pragma once
+#include <atomic>
 #include <memory>
 #include "block_gpu.hpp"
 #include "host_linalg.hpp"
@@ -112,6 +113,8 @@ class BrwnLanczosGpu {
     int max_nrhs() const { return max_nrhs_; }
     int reorth = 0;        // 0: none (reference); 1: full FP64 reorthogonalisation
     int eval_threads = 16; // host threads of the per-group convergence tests
+    // when set, a solve stops between iterations with SDT_ESTATE "cancelled"
+    const std::atomic<bool>* cancel = nullptr;
     StageTimer* timer = nullptr;
     // columns of one lockstep batch
     static constexpr int kMaxBatchCols = 128;
@@ -198,6 +201,8 @@ class BDGpu {
     // tails as one replayed HIP graph per interval (default on; profiling
     // runs them eagerly, with per-stage events)
     void set_graphs(bool on) { use_graph_ = on; }
+    // a set flag stops an interval's displacement solve between iterations
+    void set_cancel(const std::atomic<bool>* c) { brwn_->cancel = c; }
 
   private:
     void tail_step();

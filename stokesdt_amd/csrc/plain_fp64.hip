@@ -491,6 +491,10 @@ int PlainLanczosGpu::block(PlainMobGpu* mob, int b, int ldz, const double* z, in
     int iter;
     bool conv = false;
     for (iter = 0; iter < max_iters_; iter++) {
+        if (cancel && cancel->load()) {
+            SDT_HIP_CHECK(hipStreamSynchronize(s));
+            throw Error{SDT_ESTATE, "cancelled"};
+        }
         double* Vi = V + (size_t)iter * blk;
         double* W = V + (size_t)(iter + 1) * blk;
         double beta = 0.0;

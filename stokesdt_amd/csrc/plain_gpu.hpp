@@ -66,6 +66,8 @@ class PlainLanczosGpu {
     void compute(PlainMobGpu* mob, int num_rhs, int ldz, const double* d_z, int ldy, double* d_y);
     const std::vector<int>& iters() const { return iters_; }
     StageTimer* timer = nullptr;
+    // when set, a solve stops between iterations with SDT_ESTATE "cancelled"
+    const std::atomic<bool>* cancel = nullptr;
 
   private:
     int block(PlainMobGpu* mob, int b, int ldz, const double* z, int ldy, double* y);
@@ -90,6 +92,7 @@ class PlainBDGpu {
     const std::vector<int>& last_iters() const { return brwn_->iters(); }
     StageTimer& timer() { return timer_; }
     long long istep() const { return istep_; }
+    void set_cancel(const std::atomic<bool>* c) { brwn_->cancel = c; }
 
   private:
     int npos_;
