@@ -322,14 +322,16 @@ def amortise(lat, first_step, mi):
 
 
 def run_gpu(S, pos, rdi, L, prec, steps, warmup):
-    """Optimised path: warmup steps, then untimed steps up to the next
-    mobility-interval boundary, then `steps` timed steps (the first of them
-    holds the update and the Lanczos solve of all mob_interval columns)."""
+    """Optimised path: the autotuner's launch choices, warmup steps, then
+    untimed steps up to the next mobility-interval boundary, then `steps`
+    timed steps (the first of them holds the update and the Lanczos solve of
+    all mob_interval columns)."""
     c = CFG
     mi = c["mob_interval"]
     bd = S.BDSim(pos, rdi, L, c["xi"], c["rmax"], c["K"], c["p"], prec, c["max_iters"], c["b"],
                  c["tol"], c["r0"], c["k0"], c["dt"], mi, c["seed"])
     bd.init()
+    tuned = bd.autotune()
     first = warmup + (-warmup) % mi
     if first:
         bd.advance(first)
@@ -341,7 +343,9 @@ def run_gpu(S, pos, rdi, L, prec, steps, warmup):
     bd.set_profiling(False)
     value, how = amortise(lat, first, mi)
     return dict(value=value, how=how, lat=lat, first=first, prof=prof,
-                iters=[int(x) for x in iters], pos=bd.positions())
+                iters=[int(x) for x in iters], pos=bd.positions(),
+                tuning=dict(narrow=tuned[0], wide=tuned[1],
+                            matvec_ms=[round(float(x), 4) for x in tuned[2]]))
 
 
 def run_plain(S, pos, rdi, L, steps):
@@ -558,6 +562,9 @@ def main():
     res["step_latency_ms_p50"] = round(float(np.percentile(lat, 50)), 4)
     res["step_latency_ms_p99"] = round(float(np.percentile(lat, 99)), 4)
     res["lanczos_iters_fp32"] = r32["iters"]
+    res["autotune"] = dict(r32["tuning"], note="(fft_order, interp_sxt, interp_ch, "
+                           "spread_stage_kb), 0 = built-in heuristic; matvec_ms = default and "
+                           "tuned for 1 column, then for the Lanczos batch")
     res["stage_device_ms"] = stage_dict(r32["prof"])
     res["roofline"] = roofline(S, r32, pos, rdi, L)
     O = None

@@ -252,6 +252,12 @@ int sdt_bd_destroy(sdt_bd_t bd);
  * once per interval and replayed; results are bit-identical to eager launches
  * (off). No counterpart in the reference. */
 int sdt_bd_set_graphs(sdt_bd_t bd, int on);
+/* The project autotuner on a BD handle (after init; no counterpart in the
+ * reference): picks the SPME launch choices (FFT radix order, interp strip
+ * and chunk, spread stage) for single-column and Lanczos-batch launches by
+ * timing them on the current positions; tune_cache (may be NULL) as for
+ * sdt_ensemble_init. choices[8] and ms[4] as sdt_ensemble_tuning. */
+int sdt_bd_autotune(sdt_bd_t bd, const char* tune_cache, int* choices, double* ms);
 
 /* -------------------------------------------------------------- Ensemble
  * An ensemble of independent BD systems of uneven size, box and mesh

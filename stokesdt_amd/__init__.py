@@ -124,6 +124,7 @@ SIGNATURES = {
     "sdt_brwn_chol_compute_dev": (_I, [_P, _P, _I, _I, _P, _I, _P]),
     "sdt_brwn_chol_destroy": (_I, [_P]),
     "sdt_bd_set_graphs": (_I, [_P, _I]),
+    "sdt_bd_autotune": (_I, [_P, ctypes.c_char_p, _P, _P]),
     "sdt_ensemble_create": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _D, _D, _D, _D,
                                  _I, ctypes.c_uint64, ctypes.POINTER(_P)]),
     "sdt_ensemble_init": (_I, [_P, _I, ctypes.c_char_p]),
@@ -679,6 +680,15 @@ class BDSim:
     def set_graphs(self, on):
         """Step tails as one replayed HIP graph per mobility interval (default)."""
         _check(lib().sdt_bd_set_graphs(self.h, 1 if on else 0))
+
+    def autotune(self, cache=None):
+        """Autotuned SPME launch choices for single-column and Lanczos-batch
+        launches: ((narrow), (wide), [default ms, tuned ms] x 2)."""
+        c = np.zeros(8, dtype=np.int32)
+        ms = np.zeros(4)
+        _check(lib().sdt_bd_autotune(self.h, None if cache is None else str(cache).encode(),
+                                     _ptr(c), _ptr(ms)))
+        return tuple(int(x) for x in c[:4]), tuple(int(x) for x in c[4:]), ms
 
     def profile(self):
         """{stage: (total ms, launches, right-hand sides over the launches)}

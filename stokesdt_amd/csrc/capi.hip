@@ -11,6 +11,7 @@
 #include "mob_gpu.hpp"
 #include "plain_gpu.hpp"
 #include "serve.hpp"
+#include "tune.hpp"
 #include "simbox.hpp"
 
 namespace sdt {
@@ -1163,6 +1164,31 @@ int sdt_ensemble_destroy(sdt_ensemble_t e) {
     return guard([&] {
         delete e->e;
         delete e;
+    });
+}
+int sdt_bd_autotune(sdt_bd_t bd, const char* tune_cache, int* choices, double* ms) {
+    SDT_CHECK_HANDLE(bd);
+    return guard([&] {
+        BDGpu* b = bd->bd;
+        MobSpmeGpu* m = b->mob();
+        std::vector<double> pos(3 * (size_t)b->npos());
+        b->get_positions(pos.data());
+        m->update(pos.data(), nullptr, false);
+        const int wide = std::min(b->mob_interval(), BrwnLanczosGpu::kMaxBatchCols);
+        const std::string cache = tune_cache ? tune_cache : "";
+        const TuneResult r[2] = {autotune_spme(m, 1, cache), autotune_spme(m, wide, cache)};
+        for (int w = 0; w < 2; w++) {
+            if (choices) {
+                choices[4 * w] = r[w].tuning.fft_order;
+                choices[4 * w + 1] = r[w].tuning.interp_sxt;
+                choices[4 * w + 2] = r[w].tuning.interp_ch;
+                choices[4 * w + 3] = r[w].tuning.spread_stage_kb;
+            }
+            if (ms) {
+                ms[2 * w] = r[w].ms_default;
+                ms[2 * w + 1] = r[w].ms_best;
+            }
+        }
     });
 }
 int sdt_bd_set_graphs(sdt_bd_t bd, int on) {

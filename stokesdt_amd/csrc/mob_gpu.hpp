@@ -182,6 +182,7 @@ class BDGpu {
     // interval computes that interval's D from the positions at the start
     void set_step(long long istep);
     int npos() const { return npos_; }
+    int mob_interval() const { return mob_interval_; }
     StageTimer& timer() { return timer_; }
     hipStream_t stream() const { return mob_->stream(); }
 
