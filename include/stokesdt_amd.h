@@ -336,11 +336,14 @@ int sdt_serve_destroy(sdt_serve_t sv);
  * Replaces BDSimBox(config_file, model_file, xyz_file)
  * (src/simbox/simbox_bd.cc:31-294) with Init and Advance (:296-366), and the
  * file formats of MoleculeIO (src/util/molecule_io.cc:25-414):
- *  - config "key value" lines with the reference keys (mob-method must be
- *    spme, brownian-method lanczos; spme-xi, spme-rmax, spme-nfft-grids,
- *    spme-porder, mob-interval, lanczos-maxiters/-maxnrhs/-tol, delta-t,
+ *  - config "key value" lines with the reference keys (mob-method spme, or
+ *    the dense Ewald sum for any other value, as the reference; ewald-tol;
+ *    brownian-method lanczos, random (SPME only) or Cholesky for any other
+ *    value (dense mobility only, as the reference); spme-xi, spme-rmax,
+ *    spme-nfft-grids, spme-porder, mob-interval,
+ *    lanczos-maxiters/-maxnrhs/-tol, random-numvecs/-maxnrhs, delta-t,
  *    traj-interval, traj-startstep, traj-output) plus "precision fp64|fp32"
- *    and "seed <uint64>";
+ *    (SPME) and "seed <uint64>";
  *  - model lines "par type radius charge" and "steric r0 k0" (bond lines
  *    are rejected);
  *  - XYZ: npos, "step time Lx Ly Lz [seed]", "type x y z"; the box must be
@@ -351,9 +354,10 @@ int sdt_serve_destroy(sdt_serve_t sv);
  * run (a start inside a mobility interval recomputes that interval's
  * displacements from the restart positions).
  * sdt_simbox_params: dparams[7] = {spme-xi, spme-rmax, lanczos-tol,
- * steric r0, steric k0, delta-t, 0}; iparams[9] = {spme-nfft-grids,
+ * steric r0, steric k0, delta-t, ewald-tol}; iparams[11] = {spme-nfft-grids,
  * spme-porder, mob-interval, lanczos-maxiters, lanczos-maxnrhs,
- * traj-interval, traj-startstep, precision, seed}. */
+ * traj-interval, traj-startstep, precision, seed, mobility (0 SPME, 1 dense
+ * Ewald), Brownian method (0 Lanczos, 1 random, 2 Cholesky)}. */
 int sdt_simbox_create(const char* config_file, const char* model_file, const char* xyz_file,
                       sdt_simbox_t* out);
 int sdt_simbox_init(sdt_simbox_t sb);

@@ -585,9 +585,10 @@ class SimBox:
     with the reference's .cfg / .mod / XYZ formats, on the GPU BD step.
     Construction parses the files only; init() allocates the device state."""
 
-    DKEYS = ("spme_xi", "spme_rmax", "lanczos_tol", "steric_r0", "steric_k0", "delta_t")
+    DKEYS = ("spme_xi", "spme_rmax", "lanczos_tol", "steric_r0", "steric_k0", "delta_t",
+             "ewald_tol")
     IKEYS = ("spme_dim", "spme_porder", "mob_interval", "lanczos_maxiters", "lanczos_maxnrhs",
-             "traj_interval", "traj_start", "precision", "seed")
+             "traj_interval", "traj_start", "precision", "seed", "mob_method", "brownian_method")
 
     def __init__(self, config_file, model_file, xyz_file):
         h = _P()
@@ -611,7 +612,7 @@ class SimBox:
 
     def params(self):
         d = np.zeros(7)
-        i = np.zeros(9, dtype=np.int64)
+        i = np.zeros(11, dtype=np.int64)
         _check(lib().sdt_simbox_params(self.h, _ptr(d), _ptr(i)))
         out = dict(zip(self.DKEYS, d.tolist()))
         out.update(zip(self.IKEYS, i.tolist()))

@@ -657,13 +657,14 @@ int sdt_simbox_params(sdt_simbox_t sb, double* dparams, int64_t* iparams) {
     const SimBoxParams& p = sb->sb->params();
     if (dparams) {
         const double d[7] = {p.spme_xi, p.spme_rmax, p.lanczos_tol, p.steric_r0, p.steric_k0,
-                             p.delta_t, 0.0};
+                             p.delta_t, p.ewald_tol};
         memcpy(dparams, d, sizeof(d));
     }
     if (iparams) {
-        const int64_t v[9] = {p.spme_dim, p.spme_porder, p.mob_interval, p.lanczos_maxiters,
-                              p.lanczos_maxnrhs, p.traj_interval, p.traj_start, p.precision,
-                              (int64_t)p.seed};
+        const int64_t v[11] = {p.spme_dim, p.spme_porder, p.mob_interval, p.lanczos_maxiters,
+                               p.lanczos_maxnrhs, p.traj_interval, p.traj_start, p.precision,
+                               (int64_t)p.seed, p.ewald ? 1 : 0,
+                               p.chol ? 2 : (p.random ? 1 : 0)};
         memcpy(iparams, v, sizeof(v));
     }
     return SDT_OK;
@@ -903,7 +904,7 @@ int sdt_bench_copy(int64_t n, int reps, double* ms_avg) {
 int sdt_bench_bd_steps(sdt_bd_t bd, int nsteps, double* ms_total) {
     SDT_CHECK_HANDLE(bd);
     return guard([&] {
-        hipStream_t s = bd->bd->mob()->stream();
+        hipStream_t s = bd->bd->stream();
         hipEvent_t a, b;
         SDT_HIP_CHECK(hipEventCreate(&a));
         SDT_HIP_CHECK(hipEventCreate(&b));
@@ -1171,6 +1172,7 @@ int sdt_bd_autotune(sdt_bd_t bd, const char* tune_cache, int* choices, double* m
     return guard([&] {
         BDGpu* b = bd->bd;
         MobSpmeGpu* m = b->mob();
+        SDT_REQUIRE(m != nullptr, SDT_EINVAL, "autotune needs the SPME mobility");
         std::vector<double> pos(3 * (size_t)b->npos());
         b->get_positions(pos.data());
         m->update(pos.data(), nullptr, false);

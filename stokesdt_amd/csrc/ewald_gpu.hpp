@@ -22,6 +22,8 @@ class MobEwaldGpu : public MobGpu {
     // mode must be 0 (full): the reference's MobEwald has no split
     void mul_vector(int mode, int nrhs, double alpha, int ldf, const double* f, double beta,
                     int ldv, double* v, bool dev) override;
+    void enqueue_mul_vector(int mode, int nrhs, double alpha, int ldf, const double* d_f,
+                            double beta, int ldv, double* d_v) override;
     void apply_packed(int mode, int b, double alpha, const double* X, double beta,
                       double* Y) override;
     int npos() const override { return npos_; }

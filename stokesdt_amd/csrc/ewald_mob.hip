@@ -120,6 +120,17 @@ void MobEwaldGpu::mul_vector(int mode, int nrhs, double alpha, int ldf, const do
     SDT_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+void MobEwaldGpu::enqueue_mul_vector(int mode, int nrhs, double alpha, int ldf, const double* d_f,
+                                     double beta, int ldv, double* d_v) {
+    SDT_REQUIRE(updated_, SDT_ESTATE, "mulvector called before update");
+    SDT_REQUIRE(mode == 0, SDT_EINVAL, "the dense Ewald mobility has no real/recip split");
+    const int n = 3 * npos_;
+    SDT_REQUIRE(nrhs >= 1 && ldf >= n && ldv >= n, SDT_EINVAL, "bad shapes");
+    SDT_EW_BLAS(rocblas_set_pointer_mode(blas_, rocblas_pointer_mode_host));
+    SDT_EW_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, n, nrhs, n,
+                              &alpha, d_mat_.get(), n, d_f, ldf, &beta, d_v, ldv));
+}
+
 void MobEwaldGpu::get_matrix(int ld, double* mat, bool dev) {
     SDT_REQUIRE(updated_, SDT_ESTATE, "get_matrix called before update");
     const int n = 3 * npos_;

@@ -6,6 +6,7 @@
 #include <thread>
 #include "mob_gpu.hpp"
 #include "brwn_random.hpp"
+#include "ewald_gpu.hpp"
 
 namespace sdt {
 
@@ -733,9 +734,27 @@ BDGpu::BDGpu(int npos, const double* pos, const double* rdi, double box, double 
     h_pos0_.assign(pos, pos + 3 * (size_t)npos);
     rdi_.assign(rdi, rdi + npos);
     steric_on_ = r0 > 0.0 && k0 != 0.0;
-    mob_.reset(new MobSpmeGpu(npos, rdi, box, xi, rmax, dim, porder, precision));
+    spme_ = new MobSpmeGpu(npos, rdi, box, xi, rmax, dim, porder, precision);
+    mob_.reset(spme_);
     brwn_.reset(new BrwnLanczosGpu(3 * npos, max_iters, max_nrhs, tol));
     steric_.reset(new StericGpu(npos, rdi, box, r0, k0));
+}
+
+BDGpu::BDGpu(int npos, const double* pos, const double* rdi, double box, double ewald_tol,
+             bool chol, int max_iters, int max_nrhs, double tol, double r0, double k0, double dt,
+             int mob_interval, unsigned long long seed)
+    : npos_(npos), box_(box), dt_(dt), mob_interval_(mob_interval), seed_(seed) {
+    SDT_REQUIRE(npos > 0 && pos && rdi, SDT_EINVAL, "bad particle arrays");
+    SDT_REQUIRE(mob_interval >= 1, SDT_EINVAL, "mob_interval must be positive");
+    SDT_REQUIRE(dt > 0.0, SDT_EINVAL, "dt must be positive");
+    h_pos0_.assign(pos, pos + 3 * (size_t)npos);
+    rdi_.assign(rdi, rdi + npos);
+    steric_on_ = r0 > 0.0 && k0 != 0.0;
+    mob_.reset(new MobEwaldGpu(npos, rdi, box, ewald_tol, -1.0));
+    brwn_.reset(new BrwnLanczosGpu(3 * npos, max_iters, max_nrhs, tol));
+    if (chol) chol_.reset(new BrwnCholGpu(3 * npos));
+    steric_.reset(new StericGpu(npos, rdi, box, r0, k0));
+    use_graph_ = false;   // the tail's GEMM is a library call: launched eagerly
 }
 
 BDGpu::~BDGpu() { release_graph(); }
@@ -761,6 +780,7 @@ __global__ void next_counter_kernel(int* c) { *c += 1; }
 
 void BDGpu::use_random(int num_vecs, int max_nrhs) {
     SDT_REQUIRE(!d_pos_.get(), SDT_ESTATE, "use_random after init");
+    SDT_REQUIRE(!chol_, SDT_EINVAL, "use_random with Cholesky displacements");
     // its own Gaussian stream, as the reference's BrwnRandom has its own
     // RndStream (src/brwn/brwn_random.cc:109)
     rnd_.reset(new BrwnRandomGpu(3 * npos_, num_vecs, max_nrhs, seed_ ^ 0x5678ull));
@@ -768,10 +788,11 @@ void BDGpu::use_random(int num_vecs, int max_nrhs) {
 
 void BDGpu::init() {
     mob_->init();
-    if (rnd_) rnd_->init();
+    if (chol_) chol_->init();
+    else if (rnd_) rnd_->init();
     else brwn_->init();
     if (steric_on_) steric_->init();
-    mob_->set_timer(&timer_);
+    if (spme_) spme_->set_timer(&timer_);
     brwn_->timer = &timer_;
     steric_->timer = &timer_;
     ldm_ = 3 * npos_;
@@ -794,7 +815,8 @@ bool BDGpu::begin_step() {
     if (irhs != 0 && blk == d_block_) return false;
     gaussian_fill_device(seed_, (unsigned)blk, (long long)mob_interval_ * ldm_, d_Z_.get(), s);
     mob_->update(d_pos_.get(), nullptr, true);
-    if (rnd_) rnd_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
+    if (chol_) chol_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
+    else if (rnd_) rnd_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
     else brwn_->compute(mob_.get(), mob_interval_, ldm_, d_Z_.get(), ldm_, d_D_.get(), true);
     d_block_ = blk;
     hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(1), 0, s, d_irhs_.get(), irhs);

@@ -27,6 +27,9 @@ class MobGpu {
     // v := alpha M f + beta v, reference layout; mode 0 full, 1 real, 2 recip
     virtual void mul_vector(int mode, int nrhs, double alpha, int ldf, const double* f,
                             double beta, int ldv, double* v, bool dev) = 0;
+    // the same on device vectors, enqueued on stream() without waiting
+    virtual void enqueue_mul_vector(int mode, int nrhs, double alpha, int ldf, const double* d_f,
+                                    double beta, int ldv, double* d_v) = 0;
     // packed FP64 vectors (row-major 3N x b) in the order of perm()
     virtual void apply_packed(int mode, int b, double alpha, const double* X, double beta,
                               double* Y) = 0;
@@ -51,7 +54,7 @@ class MobSpmeGpu : public MobGpu {
     // the same on device vectors, enqueued on stream() without waiting (no
     // host synchronisation: capturable into a graph once its buffers exist)
     void enqueue_mul_vector(int mode, int nrhs, double alpha, int ldf, const double* d_f,
-                            double beta, int ldv, double* d_v);
+                            double beta, int ldv, double* d_v) override;
     // packed, sorted-order FP64 vectors (row-major 3N x b)
     void apply_packed(int mode, int b, double alpha, const double* X, double beta,
                       double* Y) override;
@@ -162,11 +165,20 @@ class StericGpu {
 // BDSimBox::Advance (src/simbox/simbox_bd.cc:296-366) with device-resident
 // positions, Gaussian block, displacements and forces.
 class BrwnRandomGpu;
+class BrwnCholGpu;
 class BDGpu {
   public:
+    // SPME mobility (InitMob's spme branch, simbox_bd.cc:103-111) with block
+    // Lanczos displacements
     BDGpu(int npos, const double* pos, const double* rdi, double box, double xi, double rmax,
           int dim, int porder, int precision, int max_iters, int max_nrhs, double tol, double r0,
           double k0, double dt, int mob_interval, unsigned long long seed);
+    // the dense Ewald mobility (the ewald branch, simbox_bd.cc:99-102, tolerance
+    // ewald_tol) with Cholesky displacements (chol = true, BrwnChol) or block
+    // Lanczos; FP64
+    BDGpu(int npos, const double* pos, const double* rdi, double box, double ewald_tol, bool chol,
+          int max_iters, int max_nrhs, double tol, double r0, double k0, double dt,
+          int mob_interval, unsigned long long seed);
     ~BDGpu();
     // Brownian displacements by randomised sampling (BrwnRandom) instead of
     // block Lanczos; before init
@@ -175,7 +187,9 @@ class BDGpu {
     void advance(int nsteps);
     void get_positions(double* pos);
     const std::vector<int>& last_iters() const { return brwn_->iters(); }
-    MobSpmeGpu* mob() { return mob_.get(); }
+    // the SPME mobility (null for the dense Ewald mobility)
+    MobSpmeGpu* mob() { return spme_; }
+    MobGpu* mobility() { return mob_.get(); }
     BrwnLanczosGpu* lanczos() { return brwn_.get(); }
     long long istep() const { return istep_; }
     // continue from step istep (a restart); a start inside a mobility
@@ -213,9 +227,11 @@ class BDGpu {
     int mob_interval_;
     unsigned long long seed_;
     std::vector<double> h_pos0_, rdi_;
-    std::unique_ptr<MobSpmeGpu> mob_;
+    std::unique_ptr<MobGpu> mob_;
+    MobSpmeGpu* spme_ = nullptr;   // mob_ when it is the SPME mobility
     std::unique_ptr<BrwnLanczosGpu> brwn_;
     std::unique_ptr<BrwnRandomGpu> rnd_;
+    std::unique_ptr<BrwnCholGpu> chol_;
     std::unique_ptr<StericGpu> steric_;
     DevBuf<double> d_pos_, d_Z_, d_D_, d_f_, d_cur_;
     DevBuf<int> d_irhs_;       // displacement column of the next tail

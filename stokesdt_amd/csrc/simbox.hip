@@ -16,25 +16,32 @@ SimBoxGpu::SimBoxGpu(const char* config_file, const char* model_file, const char
     SDT_REQUIRE(io_.parse_xyz(xyz_file), SDT_EINVAL, io_.err());
     SDT_REQUIRE(io_.box[0] == io_.box[1] && io_.box[1] == io_.box[2], SDT_EINVAL,
                 "the periodic box must be cubic (Lx = Ly = Lz)");
-    // InitMob (:88-124): only the SPME mobility runs on the device path
+    // InitMob (:88-124): "spme", else the dense Ewald sum (the reference's
+    // default for any other value, :91-98)
     const char* mob = io_.get_string_key("mob-method");
-    SDT_REQUIRE(mob && strcmp(mob, "spme") == 0, SDT_EINVAL,
-                "mob-method must be spme (the dense Ewald mobility is not on the device path)");
-    prm_.spme_xi = io_.get_float_key("spme-xi", 0.0, DBL_MAX, 0.5);
-    prm_.spme_rmax = io_.get_float_key("spme-rmax", 0.0, DBL_MAX, 4.0);
-    prm_.spme_dim = io_.get_int_key("spme-nfft-grids", 1, 1024, 128);
-    prm_.spme_porder = io_.get_int_key("spme-porder", 4, 6, 4);
+    prm_.ewald = !(mob && strcmp(mob, "spme") == 0);
+    if (prm_.ewald) {
+        prm_.ewald_tol = io_.get_float_key("ewald-tol", 0.0, DBL_MAX, 1e-4);
+    } else {
+        prm_.spme_xi = io_.get_float_key("spme-xi", 0.0, DBL_MAX, 0.5);
+        prm_.spme_rmax = io_.get_float_key("spme-rmax", 0.0, DBL_MAX, 4.0);
+        prm_.spme_dim = io_.get_int_key("spme-nfft-grids", 1, 1024, 128);
+        prm_.spme_porder = io_.get_int_key("spme-porder", 4, 6, 4);
+    }
     prm_.mob_interval = io_.get_int_key("mob-interval", 1, INT_MAX, 100);
-    // InitBrwn (:127-182): block Lanczos or randomised sampling
+    // InitBrwn (:127-182): "lanczos", "random", else Cholesky (the
+    // reference's default, :130-139), which needs the dense mobility
     const char* brwn = io_.get_string_key("brownian-method");
-    SDT_REQUIRE(brwn && (strcmp(brwn, "lanczos") == 0 || strcmp(brwn, "random") == 0),
-                SDT_EINVAL,
-                "brownian-method must be lanczos or random (Cholesky needs the dense mobility)");
-    prm_.random = strcmp(brwn, "random") == 0;
+    prm_.random = brwn && strcmp(brwn, "random") == 0;
+    prm_.chol = !(brwn && (strcmp(brwn, "lanczos") == 0 || prm_.random));
+    SDT_REQUIRE(!(prm_.chol && !prm_.ewald), SDT_EINVAL,
+                "Cholesky requires using the explicit matrix approaches (mob-method ewald)");
+    SDT_REQUIRE(!(prm_.random && prm_.ewald), SDT_EINVAL,
+                "brownian-method random runs on the SPME mobility here");
     if (prm_.random) {
         prm_.random_numvecs = io_.get_int_key("random-numvecs", 1, INT_MAX, 30);
         prm_.random_maxnrhs = io_.get_int_key("random-maxnrhs", 1, INT_MAX, 50);
-    } else {
+    } else if (!prm_.chol) {
         prm_.lanczos_maxiters = io_.get_int_key("lanczos-maxiters", 1, INT_MAX, 50);
         prm_.lanczos_maxnrhs = io_.get_int_key("lanczos-maxnrhs", 1, INT_MAX, 20);
         prm_.lanczos_tol = io_.get_float_key("lanczos-tol", 0.0, DBL_MAX, 1.0e-3);
@@ -77,7 +84,7 @@ void SimBoxGpu::init() {
     SDT_REQUIRE(!bd_, SDT_ESTATE, "init called twice");
     const int n = io_.npos();
     int maxnrhs = prm_.lanczos_maxnrhs;
-    if (!prm_.random && maxnrhs > 16) {
+    if (!prm_.random && !prm_.chol && maxnrhs > 16) {
         // the device's groups are at most 16 columns wide
         std::fprintf(stderr,
                      "WARNING: lanczos-maxnrhs %d > 16: the displacements are computed in groups "
@@ -86,11 +93,19 @@ void SimBoxGpu::init() {
                      maxnrhs, maxnrhs);
         maxnrhs = 16;
     }
-    bd_.reset(new BDGpu(n, pos_.data(), io_.rdi().data(), io_.box[0], prm_.spme_xi, prm_.spme_rmax,
-                        prm_.spme_dim, prm_.spme_porder, prm_.precision, prm_.lanczos_maxiters,
-                        maxnrhs, prm_.lanczos_tol,
-                        prm_.steric ? prm_.steric_r0 : 0.0, prm_.steric ? prm_.steric_k0 : 0.0,
-                        prm_.delta_t, prm_.mob_interval, prm_.seed));
+    if (prm_.ewald)
+        bd_.reset(new BDGpu(n, pos_.data(), io_.rdi().data(), io_.box[0], prm_.ewald_tol,
+                            prm_.chol, prm_.lanczos_maxiters, maxnrhs, prm_.lanczos_tol,
+                            prm_.steric ? prm_.steric_r0 : 0.0,
+                            prm_.steric ? prm_.steric_k0 : 0.0, prm_.delta_t, prm_.mob_interval,
+                            prm_.seed));
+    else
+        bd_.reset(new BDGpu(n, pos_.data(), io_.rdi().data(), io_.box[0], prm_.spme_xi,
+                            prm_.spme_rmax, prm_.spme_dim, prm_.spme_porder, prm_.precision,
+                            prm_.lanczos_maxiters, maxnrhs, prm_.lanczos_tol,
+                            prm_.steric ? prm_.steric_r0 : 0.0,
+                            prm_.steric ? prm_.steric_k0 : 0.0, prm_.delta_t, prm_.mob_interval,
+                            prm_.seed));
     if (prm_.random) bd_->use_random(prm_.random_numvecs, prm_.random_maxnrhs);
     bd_->init();
     bd_->set_step(step_);

@@ -11,13 +11,16 @@
 namespace sdt {
 
 struct SimBoxParams {
-    // mobility (InitMob, simbox_bd.cc:88-124)
+    // mobility (InitMob, simbox_bd.cc:88-124): SPME or the dense Ewald sum
+    bool ewald = false;            // mob-method ewald (MobEwald)
+    double ewald_tol = 1e-4;
     double spme_xi = 0.5, spme_rmax = 4.0;
     int spme_dim = 128, spme_porder = 4, mob_interval = 100;
     // Brownian displacements (InitBrwn, :127-182)
     int lanczos_maxiters = 50, lanczos_maxnrhs = 20;
     double lanczos_tol = 1.0e-3;
     bool random = false;           // brownian-method random (BrwnRandom)
+    bool chol = false;             // brownian-method chol (BrwnChol, dense mobility only)
     int random_numvecs = 30, random_maxnrhs = 50;
     // forces (InitForces, :185-234)
     bool steric = false;

@@ -166,3 +166,41 @@ def test_simbox_random_method_matches_composed_oracle(sdt, orc, tmp_path):
     disp = sb.positions() - pos
     assert np.abs(disp - (expect - pos)).max() < 1e-9 * np.abs(expect - pos).max()
 
+
+
+def test_reference_example_config_runs(sdt):
+    # the reference's data/example.cfg + example.mod on its N1000_Phi0.1.xyz:
+    # the dense Ewald mobility (tol 1e-4) with Cholesky displacements. Three
+    # steps composed from the pieces: D = L Z for the interval's Gaussian
+    # block (simbox_bd.cc:323-331), then per step f = steric force,
+    # pos += dt M f + sqrt(2 dt) D[:, k] with M at the interval's positions
+    ref = os.path.join(ROOT, "tests", "golden", "reference_data")
+    sb = sdt.SimBox(os.path.join(ref, "example.cfg"), os.path.join(ref, "example.mod"),
+                    os.path.join(ref, "N1000_Phi0.1.xyz"))
+    sb.init()
+    pos0 = sb.positions()
+    rdi = sb.radii()
+    p = sb.params()
+    sb.advance(3)
+    n, L = rdi.size, None
+    with open(os.path.join(ref, "N1000_Phi0.1.xyz")) as fh:
+        fh.readline()
+        L = float(fh.readline().split()[2])
+    me = sdt.MobEwald(n, rdi, L, p["ewald_tol"])
+    assert me.init()
+    me.update(pos0)
+    mi = p["mob_interval"]
+    Z = sdt.gaussian(p["seed"], 0, 3 * n * mi).reshape(mi, 3 * n).T.copy()
+    ch = sdt.BrwnChol(3 * n)
+    assert ch.init()
+    D = ch.compute(me, Z)
+    st = sdt.StericForce(n, rdi, L, 2.0, 125.0)
+    assert st.init()
+    pos = pos0.copy()
+    dt = p["delta_t"]
+    for k in range(3):
+        f = st.compute(pos)
+        disp = me.mul_vector(f, dt, np.sqrt(2.0 * dt), D[:, k:k + 1])[:, 0]
+        pos = pos + disp
+    got = sb.positions()
+    assert np.linalg.norm(got - pos) <= 1e-12 * np.linalg.norm(pos - pos0)

@@ -94,15 +94,19 @@ def test_rejected_inputs(sdt, tmp_path, cfg_extra, mod, msg):
     assert msg in str(e.value)
 
 
-def test_methods_off_the_device_path_rejected(sdt, tmp_path):
-    # the reference example uses mob-method ewald / brownian-method cholesky
-    with pytest.raises(sdt.SdtError) as e:
-        sdt.SimBox(os.path.join(REF, "example.cfg"), os.path.join(REF, "example.mod"),
-                   os.path.join(REF, "N1000_Phi0.1.xyz"))
-    assert "spme" in str(e.value)
+def test_reference_example_config_accepted(sdt, tmp_path):
+    # the reference's own data/example.cfg: mob-method ewald, brownian-method
+    # cholesky (any other value than lanczos / random selects Cholesky, as
+    # simbox_bd.cc:130-139), ewald-tol 1e-4, lanczos-maxnrhs 20
+    sb = sdt.SimBox(os.path.join(REF, "example.cfg"), os.path.join(REF, "example.mod"),
+                    os.path.join(REF, "N1000_Phi0.1.xyz"))
+    p = sb.params()
+    assert p["mob_method"] == 1 and p["brownian_method"] == 2 and p["ewald_tol"] == 1e-4
+    # Cholesky needs the dense mobility (simbox_bd.cc:141-144)
     cfg = _write(tmp_path, "d.cfg", BASE_CFG.replace("lanczos\n", "chol\n", 1))
-    with pytest.raises(sdt.SdtError):
+    with pytest.raises(sdt.SdtError) as e:
         sdt.SimBox(cfg, os.path.join(REF, "mono.mod"), os.path.join(REF, "N3_Phi0.1.xyz"))
+    assert "explicit matrix" in str(e.value)
 
 
 def test_bad_xyz(sdt, tmp_path):
