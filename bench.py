@@ -19,7 +19,8 @@ interval-start step and the mean of its other steps.
 
 Prints one JSON line: value = FP32-mixed steps/s; also the optimised FP64
 GPU path and the plain FP64 GPU path (the reference algorithm as structured
-on the CPU) on the same amortisation, the FP32-vs-FP64 errors, per-stage
+on the CPU) on the same amortisation, the FP32-vs-FP64 errors, the error of M f against
+the dense Ewald sum on 2000 sampled particles, per-stage
 device times (HIP events on the launching stream), the roofline of the
 dominant kernel and the CPU baseline (the FP64 C++/OpenMP oracle timed on a
 bounded sample).
@@ -60,6 +61,9 @@ def box_size(n, phi):
     return (4.0 * np.pi * n / (3.0 * phi)) ** (1.0 / 3.0)
 
 
+# the sampled dense-Ewald check of M f: rows and truncation tolerance
+EWALD_ROWS, EWALD_TOL = 2000, 1e-10
+ACCURATE = dict(K=256, p=8, xi=0.5, rmax=9.5)
 T_START = time.perf_counter()
 
 
@@ -271,7 +275,9 @@ def read_traffic(stage):
 def kernel_sample():
     """Short fixed kernel sample for counter runs: a calibration copy of
     2^26 doubles, then the FP32 matvec pipeline at the workload shape with
-    1 and with the Lanczos batch width of columns (2 repetitions each)."""
+    1 and with the Lanczos batch width of columns (2 repetitions each), then
+    one block-Lanczos solve of a mobility interval's columns (the Gram and
+    block-product kernels at their real shapes)."""
     import stokesdt_amd as S
     c = CFG
     n = c["n"]
@@ -289,6 +295,14 @@ def kernel_sample():
                                            zip(["spread", "fft_xy_fwd", "fft_z_influence",
                                                 "fft_xy_inv", "interp", "real_space", "whole"],
                                                t)), flush=True)
+    nc = c["mob_interval"]
+    z = S.gaussian(c["seed"], 0, 3 * n * nc).reshape(nc, 3 * n).T.copy()
+    bl = S.BrwnLanczos(3 * n, c["max_iters"], c["b"], c["tol"])
+    assert bl.init()
+    t0 = time.perf_counter()
+    y, ok = bl.compute(m, z)
+    print("block Lanczos, %d columns in groups of %d: %.3f s, iterations %s" %
+          (nc, c["b"], time.perf_counter() - t0, list(bl.iters())), flush=True)
 
 
 def step_window(bd, steps):
@@ -392,7 +406,31 @@ def displacement_error(S, O, pos, rdi, L):
         out[prec] = (mf, y)
     e_mf = float(np.linalg.norm(out[0][0] - out[1][0]) / np.linalg.norm(out[0][0]))
     e_d = float(np.linalg.norm(out[0][1] - out[1][1]) / np.linalg.norm(out[0][1]))
-    return e_mf, e_d
+    # the dense Ewald sum (EwaldVectorKernel) on 2000 seeded particles
+    rows = np.sort(np.random.default_rng(12).choice(n, EWALD_ROWS, replace=False))
+    t0 = time.perf_counter()
+    v_ew, xi_ew = S.rpy_ewald_rows(pos, rdi, L, EWALD_TOL, f, rows)
+    t_ew = time.perf_counter() - t0
+    ew = dict(rows=EWALD_ROWS, tol=EWALD_TOL, xi=round(xi_ew, 5), seconds=round(t_ew, 3))
+    for prec, name in ((S.FP64, "fp64_spme"), (S.FP32, "fp32_mixed_spme")):
+        mf = out[prec][0].reshape(-1, 3)[rows]
+        ew["mf_err_" + name] = float(np.linalg.norm(mf - v_ew) / np.linalg.norm(v_ew))
+    # the same rows from an SPME operator converged to the Ewald sum (finer
+    # mesh, smaller xi, wider real space): the gap above is the workload's
+    # discretisation, not the implementation
+    m = S.MobSpme(n, rdi, L, ACCURATE["xi"], ACCURATE["rmax"], ACCURATE["K"], ACCURATE["p"],
+                  S.FP64)
+    assert m.init()
+    m.update(pos)
+    mf = m.mul_vector(f).reshape(-1, 3)[rows]
+    ew["mf_err_converged_fp64_spme"] = float(np.linalg.norm(mf - v_ew) / np.linalg.norm(v_ew))
+    ew["converged_spme"] = "mesh %d^3 p=%d xi=%.2f rc=%.1f" % (ACCURATE["K"], ACCURATE["p"],
+                                                              ACCURATE["xi"], ACCURATE["rmax"])
+    ew["note"] = ("the workload's mesh spacing L/128 = 0.87 at xi = 0.896 ends the reciprocal "
+                  "sum where exp(-k^2/4xi^2) is still 1.8 % of its peak, so its SPME operator "
+                  "(FP64 or FP32) is mf_err_fp64_spme from the Ewald sum; FP32 vs FP64 differ "
+                  "by mf_err_fp32_vs_fp64")
+    return e_mf, e_d, ew
 
 
 def cpu_model():
@@ -620,9 +658,10 @@ def main():
             res["serving"]["plain_fp64"] = run_serving(S, S.SERVE_PLAIN, reqs)
     if not args.no_err:
         note("FP32-vs-FP64 errors")
-        e_mf, e_d = displacement_error(S, O, pos, rdi, L)
+        e_mf, e_d, ew = displacement_error(S, O, pos, rdi, L)
         res["mf_err_fp32_vs_fp64"] = e_mf
         res["brownian_disp_err_fp32_vs_fp64"] = e_d
+        res["ewald_sampled"] = ew
     if not args.no_cpu and O is not None:
         note("CPU baseline")
         cb = cpu_baseline(O, pos, rdi, L, r32["iters"])

@@ -118,6 +118,18 @@ int sdt_rpy_spme_ex(const double* pos, const double* rdi, int npos, double L, co
 int sdt_rpy_ewald_matrix(const double* pos, const double* rdi, int npos, double L, double tol,
                          const char* mode, double xi, double* mat, int ldm);
 
+/* EwaldVectorKernel (src/mob/ewald.cc:333-536), v = M f, on a sample of
+ * rows: the Ewald-summed RPY mobility to tolerance tol (the tables and
+ * truncation of sdt_rpy_ewald_matrix) applied to f (3 npos values) and
+ * returned for the particles rows[0 .. nrows-1] (v: 3 nrows values), with
+ * no limit on npos: the matrix is never formed and the reciprocal sum runs
+ * over structure factors of f. xi <= 0 picks the splitting that makes the
+ * sample cheapest (the sum depends on xi only within tol); *xi_used (may be
+ * NULL) receives the one used. FP64 on the device; host arrays. */
+int sdt_rpy_ewald_rows(const double* pos, const double* rdi, int npos, double L, double tol,
+                       double xi, int nrows, const int* rows, const double* f, double* v,
+                       double* xi_used);
+
 /* Diffusion coefficients of tools/calc_diff (calc_diff.cc:100-140): pos
  * holds nframes > 2 frames of npos AoS positions, frame after frame, dt the
  * time between frames. For lag i = 1 .. nframes/2 - 1,

@@ -35,6 +35,10 @@ int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh);
  * interpolation step of the reference's recip driver (src/mob/spme.cc:
  * 163-198). */
 int sdt_debug_spme_interp(sdt_mob_t mob, const double* mesh, double alpha, double* v);
+/* Meshes per LDS pass of the interp kernel, for tests and measurements:
+ * 0 the built-in choice (1), else 1, 2 or 4 (4: FP32 only; an
+ * error at apply time otherwise). */
+int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form);
 /* Mean device time (ms, HIP events on the handle's stream) of each stage of
  * one packed matvec with b right-hand sides over `reps` repetitions:
  * t[0] spread, t[1] forward x/y FFT, t[2] z FFT + influence + inverse z,

@@ -79,6 +79,7 @@ SIGNATURES = {
     "sdt_bd_destroy": (_I, [_P]),
     "sdt_force_steric": (_I, [_P, _P, _I, _D, _D, _D, _P]),
     "sdt_rpy_ewald_matrix": (_I, [_P, _P, _I, _D, _D, ctypes.c_char_p, _D, _P, _I]),
+    "sdt_rpy_ewald_rows": (_I, [_P, _P, _I, _D, _D, _D, _I, _P, _P, _P, _P]),
     "sdt_diffusion": (_I, [_P, _I, _I, _D, _P, ctypes.POINTER(ctypes.c_int)]),
     "sdt_diffusion_dev_noalloc": (_I, [_P, _I, _I, _D, _P, ctypes.POINTER(ctypes.c_int), _P]),
     "sdt_brwn_random_create": (_I, [_I, _I, _I, ctypes.c_uint64, ctypes.POINTER(_P)]),
@@ -104,6 +105,7 @@ SIGNATURES = {
     "sdt_debug_band_sqrt": (_I, [_I, _I, _P, _I, _P]),
     "sdt_debug_spme_spread": (_I, [_P, _P, _P]),
     "sdt_debug_spme_interp": (_I, [_P, _P, _D, _P]),
+    "sdt_debug_spme_forms": (_I, [_P, _I]),
     "sdt_bench_stage_times": (_I, [_P, _I, _I, _P]),
     "sdt_bench_copy": (_I, [_I64, _I, ctypes.POINTER(_D)]),
     "sdt_bench_bd_steps": (_I, [_P, _I, ctypes.POINTER(_D)]),
@@ -282,6 +284,11 @@ class MobSpme:
         _check(lib().sdt_debug_spme_interp(self.h, _ptr(m), alpha, _ptr(v)))
         return v
 
+    def set_interp_form(self, form):
+        """Interp meshes per LDS pass: 0 built-in, 1, 2 or 4 (FP32) (testing
+        and measurement)."""
+        _check(lib().sdt_debug_spme_forms(self.h, int(form)))
+
     def stage_times(self, b, reps=5):
         t = np.zeros(7)
         _check(lib().sdt_bench_stage_times(self.h, b, reps, _ptr(t)))
@@ -395,6 +402,24 @@ def rpy_ewald_matrix(pos, rdi, L, tol, mode="full", xi=None):
     _check(lib().sdt_rpy_ewald_matrix(_ptr(pos), _ptr(rdi), rdi.size, L, tol, mode.encode(),
                                       -1.0 if xi is None else xi, _ptr(mat), n))
     return mat
+
+
+def rpy_ewald_rows(pos, rdi, L, tol, f, rows, xi=None):
+    """EwaldVectorKernel (src/mob/ewald.cc:333-536), v = M f, for the
+    particles in rows only, with no size limit. pos: flat AoS or (npos, 3);
+    f: 3 npos values. Returns (v as (len(rows), 3), xi used)."""
+    pos = _f64(pos).reshape(-1)
+    rdi = _f64(rdi).reshape(-1)
+    f = _f64(f).reshape(-1)
+    rows = np.ascontiguousarray(rows, dtype=np.int32).reshape(-1)
+    if f.size != 3 * rdi.size or pos.size != 3 * rdi.size:
+        raise ValueError("pos and f need 3 values per particle")
+    v = np.zeros(3 * rows.size)
+    x = ctypes.c_double(0.0)
+    _check(lib().sdt_rpy_ewald_rows(_ptr(pos), _ptr(rdi), rdi.size, L, tol,
+                                    -1.0 if xi is None else xi, rows.size,
+                                    rows.ctypes.data_as(_P), _ptr(f), _ptr(v), ctypes.byref(x)))
+    return v.reshape(-1, 3), x.value
 
 
 def diffusion(pos, dt):

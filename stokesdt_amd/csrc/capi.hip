@@ -268,6 +268,29 @@ int sdt_rpy_ewald_matrix(const double* pos, const double* rdi, int npos, double 
     });
 }
 
+int sdt_rpy_ewald_rows(const double* pos, const double* rdi, int npos, double L, double tol,
+                       double xi, int nrows, const int* rows, const double* f, double* v,
+                       double* xi_used) {
+    if (!pos || !rdi || !rows || !f || !v || npos <= 0 || nrows <= 0) {
+        set_error("invalid array arguments");
+        return SDT_EINVAL;
+    }
+    if (L <= 0. || tol <= 0.) { set_error("Scalar arguments must be positive."); return SDT_EINVAL; }
+    return guard([&] {
+        const double x = xi > 0. ? xi : ewald_rows_xi(npos, nrows, L, tol);
+        if (xi_used) *xi_used = x;
+        hipStream_t s;
+        SDT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            ewald_rows(npos, pos, rdi, L, x, tol, nrows, rows, f, v, s);
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        (void)hipStreamDestroy(s);
+    });
+}
+
 int sdt_diffusion(const double* pos, int nframes, int npos, double dt, double* D, int* nlags) {
     if (!pos || !D || !nlags || npos <= 0) { set_error("invalid array arguments"); return SDT_EINVAL; }
     // calc_diff.cc:47 requires more than two frames
@@ -755,6 +778,17 @@ int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh) {
                                          hipMemcpyDeviceToHost, s));
             SDT_HIP_CHECK(hipStreamSynchronize(s));
         }
+    });
+}
+
+int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        SDT_REQUIRE(interp_form == 0 || interp_form == 1 || interp_form == 2 || interp_form == 4,
+                    SDT_EINVAL, "interp meshes per pass must be 0 (built-in), 1, 2 or 4");
+        MobSpmeGpu* m = spme_of(mob);
+        if (m->spme32()) m->spme32()->interp_form = interp_form;
+        if (m->spme64()) m->spme64()->interp_form = interp_form;
     });
 }
 

@@ -50,6 +50,12 @@ struct EwaldMatrixGpu {
     void setup(double box, double xi, double tol, int mode, hipStream_t s);
     void build(int npos, const double* d_pos, const double* d_rdi, long long ld, double* d_mat,
                hipStream_t s) const;
+    // v_r = sum_j M(rows[r], j) f_j for nrows rows (3 values each), the
+    // matrix never formed: the real sum per pair over the image table, the
+    // reciprocal sum through structure factors of f; d_sk holds 12 nk doubles
+    void rows_mul(int npos, const double* d_pos, const double* d_rdi, int nrows,
+                  const int* d_rows, const double* d_f, double* d_sk, double* d_v,
+                  hipStream_t s) const;
 };
 
 // Dense Ewald-summed RPY matrix (MobDebug::GetMatrix): mode 0 full, 1 real
@@ -57,5 +63,14 @@ struct EwaldMatrixGpu {
 // leading dimension ldm (symmetric). Synchronises.
 void ewald_matrix(int npos, const double* pos, const double* rdi, double box, double xi,
                   double tol, int mode, int ldm, double* mat, hipStream_t s);
+
+// EwaldVectorKernel (src/mob/ewald.cc:333-536) on a sample of rows: host
+// pos (AoS), rdi, rows (nrows indices) and f (3 npos); v receives 3 nrows
+// values of M f. xi <= 0 picks the splitting that balances the real-space
+// pairs of the sampled rows against the structure factors over all npos
+// particles (the sum does not depend on xi beyond tol). Synchronises.
+void ewald_rows(int npos, const double* pos, const double* rdi, double box, double xi, double tol,
+                int nrows, const int* rows, const double* f, double* v, hipStream_t s);
+double ewald_rows_xi(int npos, int nrows, double box, double tol);
 
 }  // namespace sdt

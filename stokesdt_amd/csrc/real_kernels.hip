@@ -548,6 +548,201 @@ void EwaldMatrixGpu::build(int npos, const double* d_pos, const double* d_rdi, l
     SDT_HIP_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------------- Ewald rows, no matrix --
+// EwaldVectorKernel (src/mob/ewald.cc:333-536) evaluates sum_j M_ij f_j pair
+// by pair, both sums per pair. For a sample of rows the reciprocal sum is
+// rewritten over structure factors of f, which needs each k vector once per
+// particle instead of once per pair:
+//   sum_j cos(k.(r_j - r_i)) ya_ij f_j = Re[e^{-i k.r_i} (c0 S0(k) + c2 S2(k))]
+// with ya_ij = 6 (1 - k^2 (a_i^2 + a_j^2) / 6) kexp, c0 = (6 - k^2 a_i^2) kexp,
+// c2 = -k^2 kexp, S0(k) = sum_j f_j e^{i k.r_j}, S2(k) = sum_j a_j^2 f_j
+// e^{i k.r_j}; the projector I - k^ k^T acts on the summed vector. The real
+// sum is the per-pair sum of the dense kernel over the same image table.
+
+constexpr int kSk = 12;   // Re S0, Im S0, Re S2, Im S2 (3 components each)
+
+template <int N>
+__device__ inline void block_sum_256(double (&a)[N], double* lds) {
+#pragma unroll
+    for (int c = 0; c < N; c++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a[c] += __shfl_xor(a[c], off, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < N; c++) lds[w * N + c] = a[c];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < N; c++) a[c] = lds[c] + lds[N + c] + lds[2 * N + c] + lds[3 * N + c];
+}
+
+// one workgroup per k vector: its structure factors over all particles
+__global__ void __launch_bounds__(256)
+ewald_sk_kernel(int npos, const double* __restrict__ pos, const double* __restrict__ rdi,
+                const double* __restrict__ f, const double* __restrict__ ktab,
+                double* __restrict__ sk) {
+    __shared__ double lds[4 * kSk];
+    const double* k = ktab + (size_t)blockIdx.x * kEwaldK;
+    const double k1 = k[0], k2 = k[1], k3 = k[2];
+    double a[kSk];
+#pragma unroll
+    for (int c = 0; c < kSk; c++) a[c] = 0.0;
+    for (int j = threadIdx.x; j < npos; j += 256) {
+        double sn, cs;
+        sincos(k1 * pos[3 * j] + k2 * pos[3 * j + 1] + k3 * pos[3 * j + 2], &sn, &cs);
+        const double a2 = rdi[j] * rdi[j];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double fc = f[3 * j + c];
+            a[c] += cs * fc;
+            a[3 + c] += sn * fc;
+            a[6 + c] += a2 * cs * fc;
+            a[9 + c] += a2 * sn * fc;
+        }
+    }
+    block_sum_256<kSk>(a, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < kSk; c++) sk[(size_t)blockIdx.x * kSk + c] = a[c];
+}
+
+// one workgroup per sampled row
+__global__ void __launch_bounds__(256)
+ewald_rows_kernel(int npos, const double* __restrict__ pos, const double* __restrict__ rdi,
+                  const double* __restrict__ f, double box, double xi, double rmax2, int nr,
+                  const double* __restrict__ img, int nk, const double* __restrict__ ktab,
+                  const double* __restrict__ sk, const int* __restrict__ rows,
+                  double* __restrict__ v) {
+    __shared__ double lds[4 * 3];
+    const int i = rows[blockIdx.x];
+    const double aa = rdi[i], aa2 = aa * aa;
+    const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < npos; j += 256) {
+        const double ab = rdi[j], ab2 = ab * ab;
+        const double x0 = remainder(pos[3 * j] - px, box);
+        const double y0 = remainder(pos[3 * j + 1] - py, box);
+        const double z0 = remainder(pos[3 * j + 2] - pz, box);
+        double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        auto add = [&](double xa, double ya, double ex, double ey, double ez) {
+            const double d = xa - ya;
+            s[0] += d * ex * ex + ya;
+            s[1] += d * ex * ey;
+            s[2] += d * ex * ez;
+            s[3] += d * ey * ey + ya;
+            s[4] += d * ey * ez;
+            s[5] += d * ez * ez + ya;
+        };
+        if (j == i) {
+            const double self_a =
+                1.0 / aa - xi / sqrt(M_PI) * (6.0 - 40.0 / 3.0 * xi * xi * aa2);
+            s[0] = s[3] = s[5] = self_a;
+        }
+        for (int m = 0; m < nr; m++) {
+            const double xx = x0 + img[3 * m], yy = y0 + img[3 * m + 1], zz = z0 + img[3 * m + 2];
+            const double rr = xx * xx + yy * yy + zz * zz;
+            if (rr == 0.0 || rr > rmax2) continue;
+            const double r = sqrt(rr);
+            const double ex = xx / r, ey = yy / r, ez = zz / r;
+            if (r < aa + ab) {
+                const double s2 = aa2 + ab2;
+                add(-(1.5 - s2 * 0.5 / rr) / r + 2.0 / (aa + ab) - 3.0 * r / 8.0 / s2,
+                    -(0.75 + s2 * 0.25 / rr) / r + 2.0 / (aa + ab) - 9.0 * r / 16.0 / s2, ex, ey,
+                    ez);
+            }
+            double xa, ya;
+            ewald_scalars<double>(xi, r, aa, ab, &xa, &ya);
+            add(xa, ya, ex, ey, ez);
+        }
+        const double f0 = f[3 * j], f1 = f[3 * j + 1], f2 = f[3 * j + 2];
+        acc[0] += s[0] * f0 + s[1] * f1 + s[2] * f2;
+        acc[1] += s[1] * f0 + s[3] * f1 + s[4] * f2;
+        acc[2] += s[2] * f0 + s[4] * f1 + s[5] * f2;
+    }
+    for (int m = threadIdx.x; m < nk; m += 256) {
+        const double* k = ktab + (size_t)m * kEwaldK;
+        const double* S = sk + (size_t)m * kSk;
+        double sn, cs;
+        sincos(k[0] * px + k[1] * py + k[2] * pz, &sn, &cs);
+        const double c0 = (6.0 - k[6] * aa2) * k[7], c2 = -k[6] * k[7];
+        double w[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            w[c] = cs * (c0 * S[c] + c2 * S[6 + c]) + sn * (c0 * S[3 + c] + c2 * S[9 + c]);
+        const double dot = k[3] * w[0] + k[4] * w[1] + k[5] * w[2];
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[c] += w[c] - k[3 + c] * dot;
+    }
+    block_sum_256<3>(acc, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[3 * blockIdx.x + c] = acc[c];
+}
+
+void EwaldMatrixGpu::rows_mul(int npos, const double* d_pos, const double* d_rdi, int nrows,
+                              const int* d_rows, const double* d_f, double* d_sk, double* d_v,
+                              hipStream_t s) const {
+    SDT_REQUIRE(parts == 3, SDT_ESTATE, "Ewald rows: set up for the full sum first");
+    if (nk > 0) {
+        ewald_sk_kernel<<<nk, 256, 0, s>>>(npos, d_pos, d_rdi, d_f, d_k.get(), d_sk);
+        SDT_HIP_CHECK(hipGetLastError());
+    }
+    ewald_rows_kernel<<<nrows, 256, 0, s>>>(npos, d_pos, d_rdi, d_f, box, xi, rmax * rmax, nr,
+                                            d_img.get(), nk, d_k.get(), d_sk, d_rows, d_v);
+    SDT_HIP_CHECK(hipGetLastError());
+}
+
+// The split costs nrows npos n_images distance tests plus the erfc terms of
+// the pairs within rmax, against (npos + nrows) n_k phases: scan xi on a
+// geometric grid and keep the cheapest.
+double ewald_rows_xi(int npos, int nrows, double box, double tol) {
+    const double rho = npos / (box * box * box), kcell = box / (2.0 * M_PI);
+    double best = 0.0, best_cost = 0.0;
+    for (int t = 0; t <= 160; t++) {
+        const double xi = 0.3 / box * pow(400.0, t / 160.0);
+        const double rmax = ewald_rmax(xi, tol, box), kmax = ewald_kmax(xi, tol, box);
+        const double nimg = pow(2.0 * ((int)(rmax / box) + 1) + 1.0, 3.0);
+        const double pairs = std::min(rho * 4.0 / 3.0 * M_PI * rmax * rmax * rmax, npos * nimg);
+        const double nk = 4.0 / 3.0 * M_PI * pow(kmax * kcell, 3.0);
+        const double cost = (double)nrows * npos * nimg + 20.0 * nrows * pairs +
+                            10.0 * ((double)npos + nrows) * nk;
+        if (t == 0 || cost < best_cost) {
+            best = xi;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
+void ewald_rows(int npos, const double* pos, const double* rdi, double box, double xi, double tol,
+                int nrows, const int* rows, const double* f, double* v, hipStream_t s) {
+    SDT_REQUIRE(npos > 0 && nrows > 0, SDT_EINVAL, "Ewald rows: npos and nrows must be positive");
+    for (int r = 0; r < nrows; r++)
+        SDT_REQUIRE(rows[r] >= 0 && rows[r] < npos, SDT_EINVAL,
+                    "Ewald rows: row index out of range");
+    if (xi <= 0.0) xi = ewald_rows_xi(npos, nrows, box, tol);
+    EwaldMatrixGpu ew;
+    ew.setup(box, xi, tol, 0, s);
+    const size_t dim = (size_t)3 * npos;
+    DevBuf<double> d_pos, d_rdi, d_f, d_sk, d_v;
+    DevBuf<int> d_rows;
+    d_pos.reserve(dim);
+    d_rdi.reserve(npos);
+    d_f.reserve(dim);
+    d_sk.reserve((size_t)std::max(1, ew.nk) * kSk);
+    d_v.reserve((size_t)3 * nrows);
+    d_rows.reserve(nrows);
+    SDT_HIP_CHECK(hipMemcpyAsync(d_pos.get(), pos, sizeof(double) * dim, hipMemcpyHostToDevice, s));
+    SDT_HIP_CHECK(hipMemcpyAsync(d_rdi.get(), rdi, sizeof(double) * npos, hipMemcpyHostToDevice, s));
+    SDT_HIP_CHECK(hipMemcpyAsync(d_f.get(), f, sizeof(double) * dim, hipMemcpyHostToDevice, s));
+    SDT_HIP_CHECK(hipMemcpyAsync(d_rows.get(), rows, sizeof(int) * nrows, hipMemcpyHostToDevice, s));
+    ew.rows_mul(npos, d_pos.get(), d_rdi.get(), nrows, d_rows.get(), d_f.get(), d_sk.get(),
+                d_v.get(), s);
+    SDT_HIP_CHECK(hipMemcpyAsync(v, d_v.get(), sizeof(double) * 3 * nrows, hipMemcpyDeviceToHost, s));
+    SDT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 void ewald_matrix(int npos, const double* pos, const double* rdi, double box, double xi,
                   double tol, int mode, int ldm, double* mat, hipStream_t s) {
     SDT_REQUIRE(npos > 0 && npos <= 30000, SDT_EINVAL, "dense Ewald: npos must be in [1, 30000]");

@@ -48,6 +48,9 @@ struct SpmeGpu {
     SpmeTuning tune[2];
     FFTPlan plans[2];
     static int width_class(int b) { return b >= 8 ? 1 : 0; }
+    // interp meshes per LDS pass: 0 the built-in choice (1), else 1, 2 or
+    // (FP32) 4
+    int interp_form = 0;
     // replaces the launch choices of a width class (re-plans the FFT order)
     void set_tuning(const SpmeTuning& t, int width);
     DevBuf<double> d_tw;   // K roots of unity (cos, sin) of -2*pi*j/K

@@ -1688,10 +1688,15 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
     const double a_eff = alpha / (box * box * box);
     const int RB = 4;
     const int R = porder - 1;
-    // meshes per LDS pass, strip width (tiles along x) and particle chunk
-    // (measured at K = 128, p = 6): FP32 one mesh per pass over 4-tile
-    // strips; FP64 two meshes (16-byte LDS vectors) over 2-tile strips
-    constexpr int MV = sizeof(T) == 8 ? 2 : 1;
+    // strip width (tiles along x) and particle chunk (measured at K = 128,
+    // p = 6): FP32 4-tile strips, FP64 2-tile strips
+    // interp_form > 0 overrides the meshes per pass (FP32 1, 2 or 4; FP64 1
+    // or 2); one is fastest at K = 128, p = 6 in both precisions (the kernel
+    // is bound by LDS reads, which wider vectors do not reduce)
+    const int MV = interp_form > 0 ? interp_form : 1;
+    SDT_REQUIRE(MV == 1 || MV == 2 || (MV == 4 && sizeof(T) == 4), SDT_EINVAL,
+                "interp: meshes per pass must be 1, 2 or (FP32) 4");
+    const int rlim = MV == 4 ? 16 : 26;
     int sxt = sizeof(T) == 8 ? 2 : 4, ch = sizeof(T) == 8 ? 64 : 128;
     const SpmeTuning& tn = tune[width_class(b)];
     if (tn.interp_sxt > 0) sxt = tn.interp_sxt;
@@ -1707,7 +1712,7 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
     auto rreg_of = [&](int sx) {
         return (int)ceil_div((size_t)(sx * ts + R) * (ts + R) * (ts + R), 256);
     };
-    while (rreg_of(sxt) > 26 && sxt > 1) sxt /= 2;
+    while (rreg_of(sxt) > rlim && sxt > 1) sxt /= 2;
     while (lds_of(sxt, ch) > 80 * 1024 && (sxt > 1 || ch > 32)) {
         if (sxt > 1) sxt /= 2;
         else ch /= 2;
@@ -1717,11 +1722,12 @@ void SpmeGpu<T>::interp(int b, double alpha, double beta, double* Y, hipStream_t
     const int nsx = (ntile + sxt - 1) / sxt;
     // region points per thread (registers of the prefetch)
     const int rreg = rreg_of(sxt);
-    SDT_REQUIRE(rreg <= 26, SDT_EINVAL, "interp: region too large for the prefetch registers");
+    SDT_REQUIRE(rreg <= rlim, SDT_EINVAL, "interp: region too large for the prefetch registers");
     dim3 grid(nsx * ntile * ntile, ceil_div(b, RB));
 #define SDT_INTERP_R(PP, RR)                                                                  \
-    if constexpr (sizeof(T) == 8) SDT_INTERP_M(PP, RR, 2);                                    \
-    else SDT_INTERP_M(PP, RR, 1);
+    if (MV == 1) SDT_INTERP_M(PP, RR, 1);                                                     \
+    else if (MV == 2) SDT_INTERP_M(PP, RR, 2);                                                \
+    else if constexpr (sizeof(T) == 4) SDT_INTERP_M(PP, 16, 4);
 #define SDT_INTERP_M(PP, RR, MM)                                                              \
     do {                                                                                      \
     allow_lds<T>((const void*)interp_tile_kernel<T, PP, RR, MM>, lds);                            \

@@ -248,3 +248,45 @@ def test_blocked_input_paths_match_oracle(sdt, orc, prec, tol):
     for mode, fn in (("full", g.mul_vector), ("recip", g.recip_mul_vector)):
         ref = o.mul_vector(f, mode=mode)
         assert orc.frob_rel_err(ref, fn(f)) < tol, mode
+
+
+@pytest.mark.parametrize("n,box,K,p,prec,b", [
+    (20000, 74.8, 128, 6, 1, 16),    # the workload's mesh and order
+    (3000, 30.0, 48, 8, 1, 9),       # p = 8
+    (1500, 28.0, 30, 6, 1, 12),      # tile side 6
+    (800, 20.0, 50, 5, 0, 8),        # FP64, tile side 5
+    (300, 20.0, 24, 4, 1, 3),        # narrow batch
+])
+def test_interp_meshes_per_pass_agree(sdt, orc, n, box, K, p, prec, b):
+    # the interp kernel stages 1, 2 or 4 meshes per LDS pass (vector reads);
+    # every width sums each stencil in the same order, so all agree with the
+    # oracle to the precision's tolerance and with each other to rounding
+    pos, rdi = uniform_system(n, box, 41)
+    g = sdt.MobSpme(n, rdi, box, 0.9, 5.0, K, p, prec)
+    assert g.init()
+    g.update(pos)
+    f = np.random.default_rng(42).standard_normal((3 * n, b))
+    o = orc.MobSpme(n, rdi, box, 0.9, 5.0, K, p)
+    assert o.init()
+    o.update(pos, rdi)
+    ref = o.mul_vector(f, mode="recip")
+    out = {}
+    for mv in ((1, 2, 4) if prec == 1 else (1, 2)):
+        g.set_interp_form(mv)
+        out[mv] = g.recip_mul_vector(f)
+        assert orc.frob_rel_err(ref, out[mv]) < (1e-5 if prec == 1 else 1e-10), mv
+    g.set_interp_form(0)
+    for mv in out:
+        assert orc.frob_rel_err(out[1], out[mv]) < (1e-6 if prec == 1 else 1e-14), mv
+
+
+def test_interp_meshes_per_pass_refused(sdt):
+    pos, rdi = uniform_system(200, 16.0, 43)
+    g = sdt.MobSpme(200, rdi, 16.0, 1.0, 4.0, 40, 6, sdt.FP64)
+    assert g.init()
+    g.update(pos)
+    g.set_interp_form(4)
+    with pytest.raises(Exception):
+        g.recip_mul_vector(np.ones((600, 8)))
+    with pytest.raises(Exception):
+        g.set_interp_form(3)
