@@ -35,10 +35,13 @@ int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh);
  * interpolation step of the reference's recip driver (src/mob/spme.cc:
  * 163-198). */
 int sdt_debug_spme_interp(sdt_mob_t mob, const double* mesh, double alpha, double* v);
-/* Meshes per LDS pass of the interp kernel, for tests and measurements:
- * 0 the built-in choice (1), else 1, 2 or 4 (4: FP32 only; an
- * error at apply time otherwise). */
-int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form);
+/* Launch forms of a mobility's kernels, for tests and measurements:
+ * interp_form, meshes per LDS pass of the interp kernel: 0 the built-in
+ * choice (1), else 1, 2 or 4 (4: FP32 only; an error at apply time
+ * otherwise); z_form, threads and pencils per workgroup of the z transform:
+ * 0 the built-in choice (FP32 (512, 16), FP64 (512, 8)), 1 .. 5 =
+ * (1024, 16), (1024, 16), (512, 16), (512, 8), (256, 8), (256, 4). */
+int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form);
 /* Mean device time (ms, HIP events on the handle's stream) of each stage of
  * one packed matvec with b right-hand sides over `reps` repetitions:
  * t[0] spread, t[1] forward x/y FFT, t[2] z FFT + influence + inverse z,
@@ -54,13 +57,15 @@ int sdt_bench_bd_steps(sdt_bd_t bd, int nsteps, double* ms_total);
 /* Per-stage device time of the BD driver, recorded with HIP events on the
  * handle's stream around every launch group while profiling is on.
  * Stages: 0 spread, 1 forward x/y FFT, 2 z FFT + influence + inverse z,
- * 3 inverse y/x FFT, 4 interp, 5 real space, 6 Krylov block products,
- * 7 steric (pair list + force), 8 mobility update. sdt_bd_set_profiling(1)
+ * 3 inverse y/x FFT, 4 interp, 5 real space (on a second stream beside
+ * stages 0-3 in wide full products), 6 Krylov block products, 7 steric
+ * (pair list + force), 8 mobility update, 9 the column-blocked input copy of
+ * wide full products (elsewhere timed with the spread). sdt_bd_set_profiling(1)
  * clears the sums; sdt_bd_profile returns sums (ms) and counts. */
 int sdt_bd_set_profiling(sdt_bd_t bd, int on);
 int sdt_bd_profile(sdt_bd_t bd, double* sum_ms, int64_t* counts, int nstages);
 /* right-hand sides processed per stage over the recorded launches (stages 0-5
- * record their launch width; the others 0) */
+ * and 9 record their launch width; the others 0) */
 int sdt_bd_profile_columns(sdt_bd_t bd, int64_t* columns, int nstages);
 /* Random sequential addition of n non-overlapping spheres of radius a in a
  * periodic box of side L (seeded); returns SDT_ECAP if max_attempts ran out

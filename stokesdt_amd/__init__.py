@@ -105,7 +105,7 @@ SIGNATURES = {
     "sdt_debug_band_sqrt": (_I, [_I, _I, _P, _I, _P]),
     "sdt_debug_spme_spread": (_I, [_P, _P, _P]),
     "sdt_debug_spme_interp": (_I, [_P, _P, _D, _P]),
-    "sdt_debug_spme_forms": (_I, [_P, _I]),
+    "sdt_debug_spme_forms": (_I, [_P, _I, _I]),
     "sdt_bench_stage_times": (_I, [_P, _I, _I, _P]),
     "sdt_bench_copy": (_I, [_I64, _I, ctypes.POINTER(_D)]),
     "sdt_bench_bd_steps": (_I, [_P, _I, ctypes.POINTER(_D)]),
@@ -163,7 +163,7 @@ SIGNATURES = {
 }
 
 STAGES = ["spread", "fft_xy_fwd", "fft_z_influence", "fft_xy_inv", "interp", "real_space",
-          "krylov_block", "steric", "mob_update"]
+          "krylov_block", "steric", "mob_update", "input_copy"]
 
 
 class SdtError(RuntimeError):
@@ -284,10 +284,14 @@ class MobSpme:
         _check(lib().sdt_debug_spme_interp(self.h, _ptr(m), alpha, _ptr(v)))
         return v
 
+    def set_forms(self, interp=0, z=0):
+        """Launch forms (testing and measurement): interp meshes per LDS pass
+        (0 built-in, 1, 2 or 4 (FP32)); z transform shape (0 built-in,
+        1 .. 5)."""
+        _check(lib().sdt_debug_spme_forms(self.h, int(interp), int(z)))
+
     def set_interp_form(self, form):
-        """Interp meshes per LDS pass: 0 built-in, 1, 2 or 4 (FP32) (testing
-        and measurement)."""
-        _check(lib().sdt_debug_spme_forms(self.h, int(form)))
+        self.set_forms(interp=form)
 
     def stage_times(self, b, reps=5):
         t = np.zeros(7)

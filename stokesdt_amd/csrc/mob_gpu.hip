@@ -35,9 +35,15 @@ MobSpmeGpu::MobSpmeGpu(int npos, const double* rdi, double box, double xi, doubl
     rdi_.assign(rdi, rdi + npos);
     SDT_REQUIRE(precision == kFP64 || precision == kFP32, SDT_EINVAL, "precision must be 0 or 1");
     SDT_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    SDT_HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    SDT_HIP_CHECK(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    SDT_HIP_CHECK(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
 }
 
 MobSpmeGpu::~MobSpmeGpu() {
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -142,6 +148,41 @@ void MobSpmeGpu::apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const
     if (rc > 0) d_xblk_.reserve(ceil_div(real_scratch_bytes<T>(npos_, b), sizeof(double)));
     T* xb = reinterpret_cast<T*>(d_xblk_.get());
     bool copied = false;
+    // wide full products (the Lanczos batches): the real space runs on the
+    // side stream while the transforms run here (it is bound by gather
+    // latency, they by VALU and LDS issue; started after the spread, whose
+    // VALU work it would slow); it writes Y = beta Y + alpha M_real X first
+    // and the interpolation then adds the reciprocal part. Not under graph
+    // capture (single-column tails stay on one stream).
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    SDT_HIP_CHECK(hipStreamIsCapturing(s, &cap));
+    if (mode == 0 && rc > 0 && b >= 8 && cap == hipStreamCaptureStatusNone) {
+        const bool share = spme->blocked_cols(b) == rc;
+        {
+            StageScope scope(timer_, 9, s, b);
+            real_block_copy<T>(npos_, b, X, xb, s);
+        }
+        // the reciprocal pipeline of SpmeGpu::apply, stage by stage, with the
+        // fork after the spread and the join before the interpolation
+        StageTimer* tm = spme->timer;
+        spme->reserve_rhs(b);
+        { StageScope sc(tm, 0, s, b); spme->spread(b, X, s, share ? xb : nullptr); }
+        SDT_HIP_CHECK(hipEventRecord(ev_fork_, s));
+        SDT_HIP_CHECK(hipStreamWaitEvent(side_, ev_fork_, 0));
+        {
+            StageScope scope(timer_, 5, side_, b);
+            real_matvec<T>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha, X,
+                           beta, Y, d_xblk_.get(), true, side_);
+            real_overlap(ovl_, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha, X, Y, side_);
+        }
+        SDT_HIP_CHECK(hipEventRecord(ev_join_, side_));
+        { StageScope sc(tm, 1, s, b); spme->fft_forward(b, s); }
+        { StageScope sc(tm, 2, s, b); spme->influence(b, s); }
+        { StageScope sc(tm, 3, s, b); spme->fft_backward(b, s); }
+        SDT_HIP_CHECK(hipStreamWaitEvent(s, ev_join_, 0));
+        { StageScope sc(tm, 4, s, b); spme->interp(b, alpha, 1.0, Y, s); }
+        return;
+    }
     if (mode == 0 || mode == 2) {
         const bool share = rc > 0 && spme->blocked_cols(b) == rc;
         spme->apply(b, alpha, X, beta, Y, s, share ? xb : nullptr, share);

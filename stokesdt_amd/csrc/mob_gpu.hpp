@@ -90,6 +90,10 @@ class MobSpmeGpu : public MobGpu {
     std::vector<double> rdi_;
     bool mono_ = true, inited_ = false, updated_ = false;
     hipStream_t stream_ = nullptr;
+    // the real space of wide launches runs on side_ beside the reciprocal
+    // pipeline (fork after the input copy, join before the interpolation)
+    hipStream_t side_ = nullptr;
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     std::unique_ptr<SpmeGpu<float>> spme32_;
     std::unique_ptr<SpmeGpu<double>> spme64_;
     PairListGpu pairs_;

@@ -135,7 +135,7 @@ namespace sdt {
 // FFT, 4 interp, 5 real space, 6 Krylov block products, 7 steric (pair list
 // + force), 8 mobility update (sort + pair list).
 struct StageTimer {
-    static constexpr int kStages = 9;
+    static constexpr int kStages = 10;
     bool on = false;
     double sum_ms[kStages] = {0};
     long long count[kStages] = {0};

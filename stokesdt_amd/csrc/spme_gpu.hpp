@@ -51,6 +51,10 @@ struct SpmeGpu {
     // interp meshes per LDS pass: 0 the built-in choice (1), else 1, 2 or
     // (FP32) 4
     int interp_form = 0;
+    // z transform launch shape: 0 the built-in choice (FP32 2, FP64 3),
+    // 1 .. 5 (threads, pencils) = (1024, 16), (1024, 16), (512, 16),
+    // (512, 8), (256, 8), (256, 4)
+    int z_form = 0;
     // replaces the launch choices of a width class (re-plans the FFT order)
     void set_tuning(const SpmeTuning& t, int width);
     DevBuf<double> d_tw;   // K roots of unity (cos, sin) of -2*pi*j/K

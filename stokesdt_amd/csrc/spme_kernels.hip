@@ -947,11 +947,10 @@ fft_plane_inv_kernel(FFTArgs fa, const cplx<T>* __restrict__ spec, T* __restrict
 // mesh round trips (flags 1, 2, 2) reduce to two outputs
 //   U = G (F f - k^2/6 F[a^2 f]),  W = -k^2/6 G F f,  G = lm2 (I - k k^T / k^2)
 // interpolated as interp(U) + a_i^2 interp(W).
-constexpr int kZThreads = 1024;
 constexpr int kZMaxI2 = 8;
 
-template <typename T, int R1, int R2>
-__global__ void __launch_bounds__(kZThreads)
+template <typename T, int R1, int R2, int NT>
+__global__ void __launch_bounds__(NT)
 fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ tw_g,
                        const double* __restrict__ bsp2, int lnp, int NC, double box, double xi) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -973,11 +972,11 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
     const int nvalid = min(NP, nj - j0);
     // the loads of a thread (ZB at a time) are in flight together
     constexpr int ZB = 8;
-    for (int i0 = threadIdx.x; i0 < NC * NP * K; i0 += ZB * kZThreads) {
+    for (int i0 = threadIdx.x; i0 < NC * NP * K; i0 += ZB * NT) {
         cplx<T> v[ZB];
 #pragma unroll
         for (int u = 0; u < ZB; u++) {
-            const int i = i0 + u * kZThreads;
+            const int i = i0 + u * NT;
             const int jl = i & (NP - 1);
             const int cz = i >> lnp;
             const int c = kd.div_k(cz), z = cz - c * K;
@@ -986,7 +985,7 @@ fft_z_influence_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __r
         }
 #pragma unroll
         for (int u = 0; u < ZB; u++) {
-            const int i = i0 + u * kZThreads;
+            const int i = i0 + u * NT;
             const int jl = i & (NP - 1);
             const int cz = i >> lnp;
             const int c = kd.div_k(cz), z = cz - c * K;
@@ -1609,34 +1608,45 @@ void SpmeGpu<T>::influence(int b, hipStream_t s) {
     const int Kh = K / 2 + 1;
     const int NC = poly ? 6 : 3;
     const int nj = K * Kh;
-    // pencils per workgroup: at most 16, one padded buffer of NC * NP pencils
-    // within ~110 KB (two workgroups per CU) and the butterfly bound
+    // threads per workgroup and at most 2^lnp pencils per component, one
+    // padded buffer of NC * 2^lnp pencils within ~110 KB and the butterfly
+    // bounds; z_form > 0 picks (threads, largest lnp) explicitly. Built-in
+    // (measured at K = 128, 100 columns): FP32 512 threads x 16 pencils,
+    // FP64 512 x 8 (three workgroups per CU instead of one)
+    static const int forms[][2] = {{1024, 4}, {1024, 4}, {512, 4}, {512, 3}, {256, 3}, {256, 2}};
+    SDT_REQUIRE(z_form >= 0 && z_form <= 5, SDT_EINVAL, "z transform: form must be in [0, 5]");
+    const int zf = z_form > 0 ? z_form : (sizeof(T) == 4 ? 2 : 3);
+    const int nt = forms[zf][0];
+    int lnp = forms[zf][1];
     const size_t budget = std::min<size_t>(max_lds_bytes(), 112 * 1024);
-    int lnp = 4;
     auto lds_of = [&](int l) {
         return (K + (size_t)NC * ((size_t)1 << l) * pencil_stride(K)) * sizeof(cplx<T>) +
                (size_t)K * sizeof(T);
     };
-    while (lnp > 0 && (lds_of(lnp) > budget || NC * (K << lnp) > 2 * kZMaxI2 * kZThreads)) lnp--;
-    SDT_REQUIRE(lds_of(lnp) <= max_lds_bytes() && NC * K <= 2 * kZMaxI2 * kZThreads, SDT_EINVAL,
+    while (lnp > 0 && (lds_of(lnp) > budget || NC * (K << lnp) > 2 * kZMaxI2 * nt)) lnp--;
+    SDT_REQUIRE(lds_of(lnp) <= max_lds_bytes() && NC * K <= 2 * kZMaxI2 * nt, SDT_EINVAL,
                 "z transform: mesh dimension too large");
     const size_t lds = lds_of(lnp);
     const unsigned nch = ceil_div(nj, (size_t)1 << lnp);
     const dim3 grid(nch * (unsigned)b);
-    // two-stage transforms need one stage-1 butterfly per thread
-    if (sizeof(T) == 4 && K == 128 && (NC << lnp) * 8 <= kZThreads) {
-        allow_lds<T>((const void*)fft_z_influence_kernel<T, 16, 8>, lds);
-        hipLaunchKernelGGL((fft_z_influence_kernel<T, 16, 8>), grid, dim3(kZThreads), lds, s, fa,
-                           d_spec.get(), d_tw.get(), d_bsp2.get(), lnp, NC, box, xi);
-    } else if (K == 64 && (NC << lnp) * 8 <= kZThreads) {
-        allow_lds<T>((const void*)fft_z_influence_kernel<T, 8, 8>, lds);
-        hipLaunchKernelGGL((fft_z_influence_kernel<T, 8, 8>), grid, dim3(kZThreads), lds, s, fa,
-                           d_spec.get(), d_tw.get(), d_bsp2.get(), lnp, NC, box, xi);
-    } else {
-        allow_lds<T>((const void*)fft_z_influence_kernel<T, 0, 0>, lds);
-        hipLaunchKernelGGL((fft_z_influence_kernel<T, 0, 0>), grid, dim3(kZThreads), lds, s, fa,
-                           d_spec.get(), d_tw.get(), d_bsp2.get(), lnp, NC, box, xi);
-    }
+#define SDT_Z_LAUNCH(R1, R2, NT)                                                              \
+    do {                                                                                      \
+        allow_lds<T>((const void*)fft_z_influence_kernel<T, R1, R2, NT>, lds);                \
+        hipLaunchKernelGGL((fft_z_influence_kernel<T, R1, R2, NT>), grid, dim3(NT), lds, s, fa, \
+                           d_spec.get(), d_tw.get(), d_bsp2.get(), lnp, NC, box, xi);         \
+    } while (0)
+#define SDT_Z_SHAPES(NT)                                                                      \
+    do {                                                                                      \
+        /* two-stage transforms need one stage-1 butterfly per thread */                      \
+        if (sizeof(T) == 4 && K == 128 && (NC << lnp) * 8 <= NT) SDT_Z_LAUNCH(16, 8, NT);     \
+        else if (K == 64 && (NC << lnp) * 8 <= NT) SDT_Z_LAUNCH(8, 8, NT);                    \
+        else SDT_Z_LAUNCH(0, 0, NT);                                                          \
+    } while (0)
+    if (nt == 1024) SDT_Z_SHAPES(1024);
+    else if (nt == 512) SDT_Z_SHAPES(512);
+    else SDT_Z_SHAPES(256);
+#undef SDT_Z_SHAPES
+#undef SDT_Z_LAUNCH
     SDT_HIP_CHECK(hipGetLastError());
 }
 

@@ -290,3 +290,43 @@ def test_interp_meshes_per_pass_refused(sdt):
         g.recip_mul_vector(np.ones((600, 8)))
     with pytest.raises(Exception):
         g.set_interp_form(3)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_z_transform_launch_shapes_agree(sdt, orc, prec):
+    # the z transform and influence run with 1024, 512 or 256 threads and
+    # 16, 8 or 4 pencils per workgroup; each pencil's arithmetic is the same
+    # in every shape, so the results are identical
+    n, box = 2000, 30.0
+    g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, 64, 6, False, 51, prec)
+    f = np.random.default_rng(52).standard_normal((3 * n, 5))
+    ref = None
+    for z in (1, 2, 3, 4, 5):
+        g.set_forms(0, z)
+        out = g.recip_mul_vector(f)
+        if ref is None:
+            ref = out
+            assert orc.frob_rel_err(o.mul_vector(f, mode="recip"), out) < (1e-10 if prec == 0 else 1e-5)
+        assert np.array_equal(out, ref), z
+    g.set_forms(0, 0)
+    with pytest.raises(Exception):
+        g.set_forms(0, 6)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-13), (1, 1e-6)])
+def test_wide_full_product_overlapped_streams(sdt, orc, prec, tol):
+    # wide full products run the real space on a second stream beside the
+    # spread and transforms, joined before the interpolation adds the
+    # reciprocal part: the sum of the two parts, deterministic from call to
+    # call, and equal to the oracle
+    n, box = 3000, 30.0
+    g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, 48, 6, False, 61, prec)
+    f = np.random.default_rng(62).standard_normal((3 * n, 24))
+    v0 = np.random.default_rng(63).standard_normal((3 * n, 24))
+    full = g.mul_vector(f, 1.5, 0.5, v0)
+    again = g.mul_vector(f, 1.5, 0.5, v0)
+    assert np.array_equal(full, again)
+    parts = 1.5 * (g.real_mul_vector(f) + g.recip_mul_vector(f)) + 0.5 * v0
+    assert orc.frob_rel_err(parts, full) < tol
+    ref = o.mul_vector(f, mode="full")
+    assert orc.frob_rel_err(1.5 * ref + 0.5 * v0, full) < (1e-10 if prec == 0 else 1e-5)
