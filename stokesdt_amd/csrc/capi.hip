@@ -781,9 +781,11 @@ int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh) {
     });
 }
 
-int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form) {
+int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_form) {
     SDT_CHECK_HANDLE(mob);
     return guard([&] {
+        SDT_REQUIRE(spread_form >= 0 && spread_form <= 6 && (spread_form & 3) != 3, SDT_EINVAL,
+                    "spread form: force reads 0, 1 or 2, plus 4 for row-tiled mesh stores");
         SDT_REQUIRE(interp_form == 0 || interp_form == 1 || interp_form == 2 || interp_form == 4,
                     SDT_EINVAL, "interp meshes per pass must be 0 (built-in), 1, 2 or 4");
         SDT_REQUIRE(z_form >= 0 && z_form <= 5, SDT_EINVAL, "z transform form must be in [0, 5]");
@@ -791,10 +793,12 @@ int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form) {
         if (m->spme32()) {
             m->spme32()->interp_form = interp_form;
             m->spme32()->z_form = z_form;
+            m->spme32()->spread_form = spread_form;
         }
         if (m->spme64()) {
             m->spme64()->interp_form = interp_form;
             m->spme64()->z_form = z_form;
+            m->spme64()->spread_form = spread_form;
         }
     });
 }

@@ -55,6 +55,10 @@ struct SpmeGpu {
     // 1 .. 5 (threads, pencils) = (1024, 16), (1024, 16), (512, 16),
     // (512, 8), (256, 8), (256, 4)
     int z_form = 0;
+    // spread launch form: bits 0-1, force reads of wide FP32 launches (0
+    // built-in = scalar loads, 1 staged in LDS, 2 scalar loads); bit 2, mesh
+    // stores as whole rows through LDS instead of per lane
+    int spread_form = 0;
     // replaces the launch choices of a width class (re-plans the FFT order)
     void set_tuning(const SpmeTuning& t, int width);
     DevBuf<double> d_tw;   // K roots of unity (cos, sin) of -2*pi*j/K

@@ -358,7 +358,7 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                  const double* __restrict__ frac, const int* __restrict__ base,
                  const double* __restrict__ a2, const double* __restrict__ X,
                  const T* __restrict__ Xb, int npos, int b, int RB, int NC, int cap,
-                 T* __restrict__ mesh) {
+                 T* __restrict__ mesh, int tiled_out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // record: wx[P] 0, wy[P] 0, wz[P] 0, base - block origin (3 ints in the
     // slots of bx by bz), pad, f[NQ] (16-byte aligned)
@@ -626,20 +626,65 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     }
     __syncthreads();
     accumulate(nst);
-    if (!wave_on) return;
+    // Mesh stores. Default: each lane stores its own point (a wave writes
+    // 16-byte pieces of 16 rows; the eight waves' pieces of a row reach HBM
+    // about 1.4x). tiled_out: the meshes leave through LDS (the stage is free
+    // now), QC at a time (up to 48 KB): each wave writes its 4 x 4 x 4
+    // sub-block into a 16-row x 32 tile per mesh, then every 32 threads store
+    // one whole x row (128 bytes for xlen = 32). Exactly the mesh bytes reach
+    // HBM, but the extra LDS round trip and barriers cost about 5 % of the
+    // kernel at K = 128 and 100 columns, so it is not the default.
+    constexpr int QC = (NQ * 512 * (int)sizeof(T) <= 49152) ? NQ : NQ / 2;
+    static_assert(NQ % QC == 0, "mesh groups must divide NQ");
+    T* tile = stage;
     const size_t K3 = (size_t)K * K * K;
-    const size_t pt = ((size_t)pz * K + py) * K + px;
-    if constexpr (BLK) {
-        constexpr int RBB = NQ / 3;
+    if (!tiled_out) {
+        if (!wave_on) return;
+        const size_t pt = ((size_t)pz * K + py) * K + px;
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
-            const int c = q / RBB, r = q % RBB;
-            if (r < rb) mesh[((size_t)(r0 + r) * 3 + c) * K3 + pt] = acc[q];
+            size_t mi;
+            bool on;
+            if constexpr (BLK) {
+                constexpr int RBB = NQ / 3;
+                const int c = q / RBB, r = q % RBB;
+                on = r < rb;
+                mi = (size_t)(r0 + r) * 3 + c;
+            } else {
+                on = q < nq;
+                mi = (size_t)r0 * NC + q;
+            }
+            if (on) mesh[mi * K3 + pt] = acc[q];
         }
-    } else {
+        return;
+    }
+    __syncthreads();   // every wave is done reading the stage
 #pragma unroll
-        for (int q = 0; q < NQ; q++)
-            if (q < nq) mesh[(size_t)(r0 * NC + q) * K3 + pt] = acc[q];
+    for (int q0 = 0; q0 < NQ; q0 += QC) {
+        if (wave_on) {
+#pragma unroll
+            for (int qq = 0; qq < QC; qq++) tile[(qq * 16 + lz * 4 + ly) * 32 + lx] = acc[q0 + qq];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < QC * 512; e += kPtThreads) {
+            const int x = e & 31, rw = (e >> 5) & 15, qq = e >> 9;
+            const int q = q0 + qq;
+            size_t mi;
+            bool on;
+            if constexpr (BLK) {
+                constexpr int RBB = NQ / 3;
+                const int c = q / RBB, r = q % RBB;
+                on = r < rb;
+                mi = (size_t)(r0 + r) * 3 + c;
+            } else {
+                on = q < nq;
+                mi = (size_t)r0 * NC + q;
+            }
+            if (on && x < xlen)
+                mesh[mi * K3 + ((size_t)(z0 + (rw >> 2)) * K + (y0 + (rw & 3))) * K + x0 + x] =
+                    tile[e];
+        }
+        __syncthreads();
     }
 }
 
@@ -1435,20 +1480,24 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
         const int NQ = pt_meshes<T>(b, NC);
         // FP32 reads the blocked forces through the scalar path (measured
         // faster at K = 128; FP64 keeps them staged in LDS)
-        const bool sgf = xb != nullptr && sizeof(T) == 4;
+        const bool sgf = xb != nullptr && sizeof(T) == 4 && (spread_form & 3) != 1;
         const int rec = ((3 * (porder + 1) + 3 + 3) & ~3) + (sgf ? 0 : NQ);
         // few meshes: a 48 KB stage leaves room for three workgroups per CU
         const int stage_bytes =
             tn.spread_stage_kb > 0 ? tn.spread_stage_kb * 1024 : (NQ <= 12 ? 48 * 1024 : 65536);
         const int cap = (int)(stage_bytes / (rec * sizeof(T) + 9 * sizeof(int)));
-        const size_t lds = (size_t)cap * (rec * sizeof(T) + 9 * sizeof(int));
+        // the output tiles reuse the stage's LDS: all meshes, or half of
+        // them above 48 KB (the allocation is at least that)
+        const size_t tile = (size_t)NQ * 512 * sizeof(T);
+        const size_t lds = std::max((size_t)cap * (rec * sizeof(T) + 9 * sizeof(int)),
+                                    tile <= 49152 ? tile : tile / 2);
         const int nsx = (K + 31) / 32;
         dim3 grid(nsx * (K / 4) * (K / 4), ceil_div(b, NQ / NC));
 #define SDT_SPREAD_PWS(PP, QQ, BB, WW, SS)                                                     \
     hipLaunchKernelGGL((spread_pt_kernel<T, PP, QQ, BB, WW, SS>), grid, dim3(kPtThreads), lds, s, \
                        K, ts, ntile, nsx, d_tile_start.get(), d_frac.get(), d_base.get(),      \
                        poly ? d_a2.get() : (double*)nullptr, X, xb, npos, b, NQ / NC, NC, cap, \
-                       d_real.get())
+                       d_real.get(), (spread_form & 4) ? 1 : 0)
 #define SDT_SPREAD_PW(PP, QQ, BB, WW)                                                          \
     do {                                                                                      \
         if constexpr (BB && sizeof(T) == 4) {                                                 \

@@ -330,3 +330,24 @@ def test_wide_full_product_overlapped_streams(sdt, orc, prec, tol):
     assert orc.frob_rel_err(parts, full) < tol
     ref = o.mul_vector(f, mode="full")
     assert orc.frob_rel_err(1.5 * ref + 0.5 * v0, full) < (1e-10 if prec == 0 else 1e-5)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_spread_forms_identical(sdt, orc, prec):
+    # the point-owner spread reads wide FP32 launches' forces through scalar
+    # loads or from LDS, and stores its meshes per lane or as whole rows
+    # through LDS: the sums are the same in every form
+    n, box = 3000, 30.0
+    g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, 48, 6, False, 81, prec)
+    out = {}
+    for b in (1, 20):
+        f = np.random.default_rng(82 + b).standard_normal((3 * n, b))
+        for form in (0, 1, 2, 4, 5, 6):
+            g.set_forms(0, 0, form)
+            out[(b, form)] = g.recip_mul_vector(f)
+        g.set_forms(0, 0, 0)
+        for form in (1, 2, 4, 5, 6):
+            assert np.array_equal(out[(b, 0)], out[(b, form)]), (b, form)
+        assert orc.frob_rel_err(o.mul_vector(f, mode="recip"), out[(b, 0)]) < (1e-10 if prec == 0 else 1e-5)
+    with pytest.raises(Exception):
+        g.set_forms(0, 0, 3)
