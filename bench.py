@@ -20,7 +20,8 @@ interval-start step and the mean of its other steps.
 Prints one JSON line: value = FP32-mixed steps/s; also the optimised FP64
 GPU path and the plain FP64 GPU path (the reference algorithm as structured
 on the CPU) on the same amortisation, the FP32-vs-FP64 errors, the error of M f against
-the dense Ewald sum on 2000 sampled particles, per-stage
+the dense Ewald sum on 2000 sampled particles, the polydisperse path at
+N = 100 000 (matvec stage times and FP32 steps/s), per-stage
 device times (HIP events on the launching stream), the roofline of the
 dominant kernel and the CPU baseline (the FP64 C++/OpenMP oracle timed on a
 bounded sample).
@@ -362,6 +363,48 @@ def run_gpu(S, pos, rdi, L, prec, steps, warmup):
                             matvec_ms=[round(float(x), 4) for x in tuned[2]]))
 
 
+POLY = dict(n=100_000, phi_max=0.3, amin=0.5, amax=1.5, K=180, p=6, seed=7)
+
+
+def run_poly(S):
+    """The polydisperse path at N = 100 000 (radii uniform in [0.5, 1.5],
+    sequential addition at the largest radius, phi 0.3 at that radius, mesh
+    180^3, spacing 0.93, whose half-spectrum planes fit LDS): stage times of the
+    matvec at 1 and 100 columns, FP32-mixed and FP64, and FP32 steps/s over
+    one mobility interval composed as (interval-start step) + 99 x (mean of
+    9 other steps)."""
+    c, q = CFG, POLY
+    n = q["n"]
+    L = (4.0 * np.pi * n * q["amax"] ** 3 / (3.0 * q["phi_max"])) ** (1.0 / 3.0)
+    pos = S.rsa_positions(n, L, q["amax"], q["seed"])
+    rdi = np.random.default_rng(q["seed"]).uniform(q["amin"], q["amax"], n)
+    out = dict(workload="N=%d radii U[%.1f, %.1f] L=%.2f mesh %d^3 p=%d rc=%.0f xi=%.3f "
+                        "block-Lanczos b=%d tol=%g mob-interval=%d"
+                        % (n, q["amin"], q["amax"], L, q["K"], q["p"], c["rmax"], c["xi"], c["b"],
+                           c["tol"], c["mob_interval"]))
+    names = ["spread", "fft_xy_fwd", "fft_z_influence", "fft_xy_inv", "interp", "real_space",
+             "whole"]
+    for prec, tag in ((S.FP32, "fp32_mixed"), (S.FP64, "fp64")):
+        m = S.MobSpme(n, rdi, L, c["xi"], c["rmax"], q["K"], q["p"], prec)
+        assert m.init()
+        m.update(pos)
+        out[tag + "_matvec_ms"] = {str(b): dict(zip(names, [round(float(x), 4) for x in
+                                                          m.stage_times(b, 3)]))
+                                   for b in (1, 100)}
+        del m
+    mi = c["mob_interval"]
+    bd = S.BDSim(pos, rdi, L, c["xi"], c["rmax"], q["K"], q["p"], S.FP32, c["max_iters"], c["b"],
+                 c["tol"], c["r0"], c["k0"], c["dt"], mi, c["seed"])
+    bd.init()
+    lat = step_window(bd, 10)
+    value, how = amortise(lat, 0, mi)
+    out["fp32_mixed_steps_per_s"] = round(value, 3)
+    out["interval_start_step_ms"] = round(lat[0] * 1000.0, 2)
+    out["other_step_ms_mean"] = round(float(np.mean(lat[1:])) * 1000.0, 3)
+    out["lanczos_iters"] = [int(x) for x in bd.last_iters()]
+    return out
+
+
 def run_plain(S, pos, rdi, L, steps):
     """The plain FP64 GPU path (the reference algorithm as structured on the
     CPU, on the device): library kernels are loaded on a small system first,
@@ -554,6 +597,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU paths")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU path")
     ap.add_argument("--no-err", action="store_true", help="skip the FP32-vs-FP64 error check")
+    ap.add_argument("--no-poly", action="store_true", help="skip the polydisperse N=100k leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-system ensemble")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
     ap.add_argument("--kernel-sample", action="store_true",
@@ -635,6 +679,9 @@ def main():
                      "(rocSOLVER Householder QR, rocBLAS products, Y every iteration)")
             res["speedup_vs_plain_fp64_gpu"] = round(value / rp["value"], 2)
             res["north_star_speedup_target"] = 10.0
+    if not args.no_poly:
+        note("polydisperse N=100k")
+        res["polydisperse"] = run_poly(S)
     if not args.no_ensemble:
         note("ensemble")
         ens = run_ensemble(S, args.steps, args.warmup)
