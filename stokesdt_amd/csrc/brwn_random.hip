@@ -81,7 +81,10 @@ BrwnRandomGpu::BrwnRandomGpu(int dim, int num_vecs, int max_nrhs, unsigned long 
     : dim_(dim), nv_(num_vecs), max_nrhs_(max_nrhs), seed_(seed) {}
 
 BrwnRandomGpu::~BrwnRandomGpu() {
-    if (h_) (void)rocblas_destroy_handle(h_);
+    {
+        CaptureLock lk(capture_mutex());
+        if (h_) (void)rocblas_destroy_handle(h_);
+    }
 }
 
 void BrwnRandomGpu::init() {
@@ -91,7 +94,10 @@ void BrwnRandomGpu::init() {
                 "the number of random vectors must be in [1, dim]");
     SDT_REQUIRE(nv_ <= kMaxVecs, SDT_ECAP, "at most 128 random vectors");
     SDT_REQUIRE(max_nrhs_ > 0, SDT_EINVAL, "max_nrhs must be positive");
-    if (!h_) SDT_RB_CHECK(rocblas_create_handle(&h_));
+    if (!h_) {
+        CaptureLock lk(capture_mutex());
+        SDT_RB_CHECK(rocblas_create_handle(&h_));
+    }
     const size_t blk = (size_t)dim_ * nv_;
     d_V_.reserve(blk);
     d_W_.reserve(blk);

@@ -257,14 +257,14 @@ int sdt_rpy_ewald_matrix(const double* pos, const double* rdi, int npos, double 
     if (xi <= 0.) xi = pow(10.0, 1.0 / 6.0) * sqrt(M_PI) / L;
     return guard([&] {
         hipStream_t s;
-        SDT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        SDT_HIP_CHECK(stream_create(&s));
         try {
             ewald_matrix(npos, pos, rdi, L, xi, tol, md, ldm, mat, s);
         } catch (...) {
-            (void)hipStreamDestroy(s);
+            (void)stream_destroy(s);
             throw;
         }
-        (void)hipStreamDestroy(s);
+        (void)stream_destroy(s);
     });
 }
 
@@ -280,14 +280,14 @@ int sdt_rpy_ewald_rows(const double* pos, const double* rdi, int npos, double L,
         const double x = xi > 0. ? xi : ewald_rows_xi(npos, nrows, L, tol);
         if (xi_used) *xi_used = x;
         hipStream_t s;
-        SDT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        SDT_HIP_CHECK(stream_create(&s));
         try {
             ewald_rows(npos, pos, rdi, L, x, tol, nrows, rows, f, v, s);
         } catch (...) {
-            (void)hipStreamDestroy(s);
+            (void)stream_destroy(s);
             throw;
         }
-        (void)hipStreamDestroy(s);
+        (void)stream_destroy(s);
     });
 }
 
@@ -299,14 +299,14 @@ int sdt_diffusion(const double* pos, int nframes, int npos, double dt, double* D
     *nlags = diffusion_lags(nframes);
     return guard([&] {
         hipStream_t s;
-        SDT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        SDT_HIP_CHECK(stream_create(&s));
         try {
             diffusion(pos, nframes, npos, dt, D, s);
         } catch (...) {
-            (void)hipStreamDestroy(s);
+            (void)stream_destroy(s);
             throw;
         }
-        (void)hipStreamDestroy(s);
+        (void)stream_destroy(s);
     });
 }
 
@@ -363,7 +363,7 @@ int sdt_pairlist_create(int npos, const double* rdi, double box_size, double cut
         p->pl.box = box_size;
         p->pl.cutoff = cutoff;
         p->pl.mode = mode;
-        hipError_t e = hipStreamCreateWithFlags(&p->s, hipStreamNonBlocking);
+        hipError_t e = stream_create(&p->s);
         if (e != hipSuccess) {
             delete p;
             throw Error{SDT_EHIP, "hipStreamCreate failed"};
@@ -399,17 +399,17 @@ int sdt_pairlist_get_pairs(sdt_pairlist_t pl, int32_t* rowptr, int32_t* colidx) 
     return guard([&] {
         SDT_REQUIRE(pl->built, SDT_ESTATE, "get_pairs called before build");
         SDT_REQUIRE(rowptr && (colidx || pl->pl.nnz == 0), SDT_EINVAL, "null output arrays");
-        SDT_HIP_CHECK(hipMemcpy(rowptr, pl->pl.d_rowptr.get(), sizeof(int) * (pl->pl.npos + 1),
+        SDT_HIP_CHECK(sync_memcpy(rowptr, pl->pl.d_rowptr.get(), sizeof(int) * (pl->pl.npos + 1),
                                 hipMemcpyDeviceToHost));
         if (pl->pl.nnz > 0)
-            SDT_HIP_CHECK(hipMemcpy(colidx, pl->pl.d_colidx.get(), sizeof(int) * pl->pl.nnz,
+            SDT_HIP_CHECK(sync_memcpy(colidx, pl->pl.d_colidx.get(), sizeof(int) * pl->pl.nnz,
                                     hipMemcpyDeviceToHost));
     });
 }
 int sdt_pairlist_destroy(sdt_pairlist_t pl) {
     if (!pl) return SDT_OK;
     return guard([&] {
-        if (pl->s) (void)hipStreamDestroy(pl->s);
+        if (pl->s) (void)stream_destroy(pl->s);
         delete pl;
     });
 }
@@ -539,7 +539,7 @@ int sdt_steric_create(int npos, const double* rdi, double box_size, double r0, d
     return guard([&] {
         StericGpu* f = new StericGpu(npos, rdi, box_size, r0, k0);
         hipStream_t s;
-        hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        hipError_t e = stream_create(&s);
         if (e != hipSuccess) {
             delete f;
             throw Error{SDT_EHIP, "hipStreamCreate failed"};
@@ -588,7 +588,7 @@ int sdt_force_steric(const double* pos, const double* rdi, int npos, double L, d
 int sdt_steric_destroy(sdt_force_t fo) {
     if (!fo) return SDT_OK;
     return guard([&] {
-        (void)hipStreamDestroy(fo->s);
+        (void)stream_destroy(fo->s);
         delete fo->f;
         delete fo;
     });
@@ -641,7 +641,7 @@ int sdt_gaussian(uint64_t seed, uint32_t stream, int64_t n, double* out) {
         DevBuf<double> d;
         d.reserve((size_t)n);
         gaussian_fill_device(seed, stream, n, d.get(), 0);
-        SDT_HIP_CHECK(hipMemcpy(out, d.get(), sizeof(double) * n, hipMemcpyDeviceToHost));
+        SDT_HIP_CHECK(sync_memcpy(out, d.get(), sizeof(double) * n, hipMemcpyDeviceToHost));
     });
 }
 int sdt_device_synchronize(void) {
@@ -725,11 +725,11 @@ int sdt_debug_gram_groups(int64_t n, int btot, int bs, const double* A, const do
         dA.reserve((size_t)n * btot);
         dB.reserve((size_t)n * btot);
         dG.reserve((size_t)ng * 256);
-        SDT_HIP_CHECK(hipMemcpy(dA.get(), A, sizeof(double) * n * btot, hipMemcpyHostToDevice));
-        SDT_HIP_CHECK(hipMemcpy(dB.get(), B, sizeof(double) * n * btot, hipMemcpyHostToDevice));
+        SDT_HIP_CHECK(sync_memcpy(dA.get(), A, sizeof(double) * n * btot, hipMemcpyHostToDevice));
+        SDT_HIP_CHECK(sync_memcpy(dB.get(), B, sizeof(double) * n * btot, hipMemcpyHostToDevice));
         gram_groups(n, btot, bs, dA.get(), dB.get(), dG.get(), part, 0);
         std::vector<double> h((size_t)ng * 256);
-        SDT_HIP_CHECK(hipMemcpy(h.data(), dG.get(), sizeof(double) * h.size(),
+        SDT_HIP_CHECK(sync_memcpy(h.data(), dG.get(), sizeof(double) * h.size(),
                                 hipMemcpyDeviceToHost));
         for (int g = 0; g < ng; g++)
             for (int i = 0; i < bs; i++)
@@ -1022,7 +1022,7 @@ int sdt_plain_mob_update(sdt_plain_mob_t mob, const double* pos) {
         SDT_REQUIRE(pos != nullptr, SDT_EINVAL, "pos is null");
         const size_t n = (size_t)mob->m->dim();
         mob->d_pos.reserve(n);
-        SDT_HIP_CHECK(hipMemcpy(mob->d_pos.get(), pos, sizeof(double) * n, hipMemcpyHostToDevice));
+        SDT_HIP_CHECK(sync_memcpy(mob->d_pos.get(), pos, sizeof(double) * n, hipMemcpyHostToDevice));
         mob->m->update(mob->d_pos.get());
     });
 }

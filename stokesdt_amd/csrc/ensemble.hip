@@ -27,7 +27,7 @@ EnsembleGpu::EnsembleGpu(const std::vector<EnsembleSystem>& sys, const double* p
                                    mob_interval, seed + s));
         off += (size_t)e.npos;
     }
-    SDT_HIP_CHECK(hipStreamCreateWithFlags(&s0_, hipStreamNonBlocking));
+    SDT_HIP_CHECK(stream_create(&s0_));
     SDT_HIP_CHECK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
     join_.resize(bd_.size(), nullptr);
     for (hipEvent_t& e : join_) SDT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -38,10 +38,11 @@ EnsembleGpu::~EnsembleGpu() {
     for (hipEvent_t e : join_)
         if (e) (void)hipEventDestroy(e);
     if (fork_) (void)hipEventDestroy(fork_);
-    if (s0_) (void)hipStreamDestroy(s0_);
+    if (s0_) (void)stream_destroy(s0_);
 }
 
 void EnsembleGpu::release_graph() {
+    CaptureLock lk(capture_mutex());
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     exec_ = nullptr;
@@ -108,6 +109,8 @@ void EnsembleGpu::tail_all() {
     }
     const long long blk = bd_[0]->block();
     if (graph_block_ != blk) {
+        // no other thread's allocation, free or synchronous copy meanwhile
+        CaptureLock lk(capture_mutex());
         release_graph();
         SDT_HIP_CHECK(hipStreamBeginCapture(s0_, hipStreamCaptureModeThreadLocal));
         try {

@@ -32,12 +32,15 @@ MobEwaldGpu::MobEwaldGpu(int npos, const double* rdi, double box, double tol, do
     rdi_.assign(rdi, rdi + npos);
     // the reference's default splitting (src/mob/mob_ewald.cc:51)
     if (!(xi_ > 0.0) && box > 0.0) xi_ = pow(10.0, 1.0 / 6.0) * sqrt(M_PI) / box;
-    SDT_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    SDT_HIP_CHECK(stream_create(&stream_));
 }
 
 MobEwaldGpu::~MobEwaldGpu() {
-    if (blas_) (void)rocblas_destroy_handle(blas_);
-    if (stream_) (void)hipStreamDestroy(stream_);
+    {
+        CaptureLock lk(capture_mutex());
+        if (blas_) (void)rocblas_destroy_handle(blas_);
+    }
+    if (stream_) (void)stream_destroy(stream_);
 }
 
 void MobEwaldGpu::init() {
@@ -58,7 +61,10 @@ void MobEwaldGpu::init() {
                        d_perm_.get());
     SDT_HIP_CHECK(hipGetLastError());
     ew_.setup(box_, xi_, tol_, 0, stream_);
-    SDT_EW_BLAS(rocblas_create_handle(&blas_));
+    {
+        CaptureLock lk(capture_mutex());
+        SDT_EW_BLAS(rocblas_create_handle(&blas_));
+    }
     SDT_EW_BLAS(rocblas_set_stream(blas_, stream_));
     inited_ = true;
 }
@@ -145,7 +151,10 @@ void MobEwaldGpu::get_matrix(int ld, double* mat, bool dev) {
 BrwnCholGpu::BrwnCholGpu(int dim) : dim_(dim) {}
 
 BrwnCholGpu::~BrwnCholGpu() {
-    if (blas_) (void)rocblas_destroy_handle(blas_);
+    {
+        CaptureLock lk(capture_mutex());
+        if (blas_) (void)rocblas_destroy_handle(blas_);
+    }
 }
 
 void BrwnCholGpu::init() {
@@ -154,7 +163,10 @@ void BrwnCholGpu::init() {
     SDT_REQUIRE(dim_ <= 90000, SDT_ECAP, "Cholesky displacements: at most 30000 particles");
     d_L_.reserve((size_t)dim_ * dim_);
     d_info_.reserve(1);
-    SDT_EW_BLAS(rocblas_create_handle(&blas_));
+    {
+        CaptureLock lk(capture_mutex());
+        SDT_EW_BLAS(rocblas_create_handle(&blas_));
+    }
     inited_ = true;
 }
 

@@ -34,8 +34,8 @@ MobSpmeGpu::MobSpmeGpu(int npos, const double* rdi, double box, double xi, doubl
     SDT_REQUIRE(npos > 0 && rdi != nullptr, SDT_EINVAL, "npos must be positive and rdi non-null");
     rdi_.assign(rdi, rdi + npos);
     SDT_REQUIRE(precision == kFP64 || precision == kFP32, SDT_EINVAL, "precision must be 0 or 1");
-    SDT_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    SDT_HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    SDT_HIP_CHECK(stream_create(&stream_));
+    SDT_HIP_CHECK(stream_create(&side_));
     SDT_HIP_CHECK(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
     SDT_HIP_CHECK(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
 }
@@ -43,8 +43,8 @@ MobSpmeGpu::MobSpmeGpu(int npos, const double* rdi, double box, double xi, doubl
 MobSpmeGpu::~MobSpmeGpu() {
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
     if (ev_join_) (void)hipEventDestroy(ev_join_);
-    if (side_) (void)hipStreamDestroy(side_);
-    if (stream_) (void)hipStreamDestroy(stream_);
+    if (side_) (void)stream_destroy(side_);
+    if (stream_) (void)stream_destroy(stream_);
 }
 
 void MobSpmeGpu::init() {
@@ -801,6 +801,7 @@ BDGpu::BDGpu(int npos, const double* pos, const double* rdi, double box, double 
 BDGpu::~BDGpu() { release_graph(); }
 
 void BDGpu::release_graph() {
+    CaptureLock lk(capture_mutex());
     if (tail_exec_) (void)hipGraphExecDestroy(tail_exec_);
     if (tail_graph_) (void)hipGraphDestroy(tail_graph_);
     tail_exec_ = nullptr;
@@ -843,7 +844,7 @@ void BDGpu::init() {
     d_f_.reserve((size_t)ldm_);
     d_cur_.reserve((size_t)ldm_);
     d_irhs_.reserve(1);
-    SDT_HIP_CHECK(hipMemcpy(d_pos_.get(), h_pos0_.data(), sizeof(double) * ldm_,
+    SDT_HIP_CHECK(sync_memcpy(d_pos_.get(), h_pos0_.data(), sizeof(double) * ldm_,
                             hipMemcpyHostToDevice));
 }
 
@@ -890,6 +891,8 @@ void BDGpu::tail_step() {
         return;
     }
     if (graph_block_ != d_block_) {
+        // no other thread's allocation, free or synchronous copy meanwhile
+        CaptureLock lk(capture_mutex());
         release_graph();
         SDT_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
@@ -927,7 +930,7 @@ void BDGpu::set_step(long long istep) {
 }
 
 void BDGpu::get_positions(double* pos) {
-    SDT_HIP_CHECK(hipMemcpy(pos, d_pos_.get(), sizeof(double) * ldm_, hipMemcpyDeviceToHost));
+    SDT_HIP_CHECK(sync_memcpy(pos, d_pos_.get(), sizeof(double) * ldm_, hipMemcpyDeviceToHost));
 }
 
 }  // namespace sdt

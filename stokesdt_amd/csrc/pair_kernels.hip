@@ -181,10 +181,10 @@ int PairListGpu::init(int npos_, const double* h_rdi, double box_, double cutoff
     nsidx = (int)(sidx.size() / 4);
     d_sidx.reserve(sidx.size());
     d_ssum.reserve(ssum.size());
-    SDT_HIP_CHECK(hipMemcpy(d_sidx.get(), sidx.data(), sizeof(int) * sidx.size(), hipMemcpyHostToDevice));
-    SDT_HIP_CHECK(hipMemcpy(d_ssum.get(), ssum.data(), sizeof(double) * ssum.size(), hipMemcpyHostToDevice));
+    SDT_HIP_CHECK(sync_memcpy(d_sidx.get(), sidx.data(), sizeof(int) * sidx.size(), hipMemcpyHostToDevice));
+    SDT_HIP_CHECK(sync_memcpy(d_ssum.get(), ssum.data(), sizeof(double) * ssum.size(), hipMemcpyHostToDevice));
     d_rdi.reserve(npos);
-    SDT_HIP_CHECK(hipMemcpy(d_rdi.get(), h_rdi, sizeof(double) * npos, hipMemcpyHostToDevice));
+    SDT_HIP_CHECK(sync_memcpy(d_rdi.get(), h_rdi, sizeof(double) * npos, hipMemcpyHostToDevice));
     d_pos0.reserve(3 * (size_t)npos);
     d_cidx.reserve(npos);
     d_cell_perm.reserve(npos);

@@ -264,15 +264,21 @@ PlainMobGpu::PlainMobGpu(int npos, const double* rdi, double box, double xi, dou
     : npos_(npos), K_(dim), porder_(porder), ldp_(0), box_(box), xi_(xi), rmax_(rmax) {
     SDT_REQUIRE(npos > 0 && rdi != nullptr, SDT_EINVAL, "npos must be positive and rdi non-null");
     rdi_.assign(rdi, rdi + npos);
-    SDT_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    SDT_HIP_CHECK(stream_create(&stream_));
 }
 
 PlainMobGpu::~PlainMobGpu() {
     if (plans_) {
-        (void)hipfftDestroy(fwd_);
-        (void)hipfftDestroy(bwd_);
+        {
+            CaptureLock lk(capture_mutex());
+            (void)hipfftDestroy(fwd_);
+        }
+        {
+            CaptureLock lk(capture_mutex());
+            (void)hipfftDestroy(bwd_);
+        }
     }
-    if (stream_) (void)hipStreamDestroy(stream_);
+    if (stream_) (void)stream_destroy(stream_);
 }
 
 void PlainMobGpu::init() {
@@ -297,19 +303,25 @@ void PlainMobGpu::init() {
     d_lm2_.reserve(nspec);
     d_rdi_.reserve(npos_);
     d_a2_.reserve(npos_);
-    SDT_HIP_CHECK(hipMemcpy(d_rdi_.get(), rdi_.data(), sizeof(double) * npos_,
+    SDT_HIP_CHECK(sync_memcpy(d_rdi_.get(), rdi_.data(), sizeof(double) * npos_,
                             hipMemcpyHostToDevice));
     hipLaunchKernelGGL(plain_square_kernel, dim3(ceil_div(npos_, 256)), dim3(256), 0, stream_,
                        npos_, d_rdi_.get(), d_a2_.get());
     std::vector<double> bsp;
     bspline_moduli(K_, porder_, bsp);
     d_bsp_.reserve(K_);
-    SDT_HIP_CHECK(hipMemcpy(d_bsp_.get(), bsp.data(), sizeof(double) * K_, hipMemcpyHostToDevice));
+    SDT_HIP_CHECK(sync_memcpy(d_bsp_.get(), bsp.data(), sizeof(double) * K_, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(plain_lm2_kernel, dim3(ceil_div(nspec, 256)), dim3(256), 0, stream_, K_,
                        box_, xi_, d_bsp_.get(), d_lm2_.get());
     SDT_HIP_CHECK(hipGetLastError());
-    SDT_PL_FFT(hipfftPlan3d(&fwd_, K_, K_, K_, HIPFFT_D2Z));
-    SDT_PL_FFT(hipfftPlan3d(&bwd_, K_, K_, K_, HIPFFT_Z2D));
+    {
+        CaptureLock lk(capture_mutex());
+        SDT_PL_FFT(hipfftPlan3d(&fwd_, K_, K_, K_, HIPFFT_D2Z));
+    }
+    {
+        CaptureLock lk(capture_mutex());
+        SDT_PL_FFT(hipfftPlan3d(&bwd_, K_, K_, K_, HIPFFT_Z2D));
+    }
     plans_ = true;
     SDT_PL_FFT(hipfftSetStream(fwd_, stream_));
     SDT_PL_FFT(hipfftSetStream(bwd_, stream_));
@@ -436,7 +448,10 @@ PlainLanczosGpu::PlainLanczosGpu(int dim, int max_iters, int max_nrhs, double to
     : dim_(dim), max_iters_(max_iters), max_nrhs_(max_nrhs), tol_(tol) {}
 
 PlainLanczosGpu::~PlainLanczosGpu() {
-    if (blas_) (void)rocblas_destroy_handle(blas_);
+    {
+        CaptureLock lk(capture_mutex());
+        if (blas_) (void)rocblas_destroy_handle(blas_);
+    }
 }
 
 void PlainLanczosGpu::init() {
@@ -444,7 +459,10 @@ void PlainLanczosGpu::init() {
     SDT_REQUIRE(dim_ > 0 && max_iters_ > 0 && max_nrhs_ > 0 && tol_ > 0.0, SDT_EINVAL,
                 "dimension, max_iters, max_nrhs and tol must be positive");
     if (max_nrhs_ > dim_ / 3) max_nrhs_ = dim_ / 3;
-    SDT_PL_BLAS(rocblas_create_handle(&blas_));
+    {
+        CaptureLock lk(capture_mutex());
+        SDT_PL_BLAS(rocblas_create_handle(&blas_));
+    }
     d_V_.reserve((size_t)dim_ * max_nrhs_ * (max_iters_ + 1));
     d_Yold_.reserve((size_t)dim_ * max_nrhs_ * 2);
     d_tau_.reserve(max_nrhs_);
@@ -618,7 +636,7 @@ void PlainBDGpu::init() {
     d_Z_.reserve((size_t)ldm_ * mob_interval_);
     d_D_.reserve((size_t)ldm_ * mob_interval_);
     d_f_.reserve((size_t)ldm_);
-    SDT_HIP_CHECK(hipMemcpy(d_pos_.get(), h_pos0_.data(), sizeof(double) * ldm_,
+    SDT_HIP_CHECK(sync_memcpy(d_pos_.get(), h_pos0_.data(), sizeof(double) * ldm_,
                             hipMemcpyHostToDevice));
 }
 
@@ -645,7 +663,7 @@ void PlainBDGpu::advance(int nsteps) {
 }
 
 void PlainBDGpu::get_positions(double* pos) {
-    SDT_HIP_CHECK(hipMemcpy(pos, d_pos_.get(), sizeof(double) * ldm_, hipMemcpyDeviceToHost));
+    SDT_HIP_CHECK(sync_memcpy(pos, d_pos_.get(), sizeof(double) * ldm_, hipMemcpyDeviceToHost));
 }
 
 }  // namespace sdt

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "../../include/stokesdt_amd.h"
@@ -37,6 +38,32 @@ struct Error {
 // status codes (SDT_OK, SDT_EINVAL, ...) come from the public header
 
 
+// ------------------------------------------- capture and device-wide calls --
+// Graph captures and the calls that can synchronise with every stream of the
+// device (allocation and freeing, synchronous copies, stream and graph
+// creation and destruction) exclude each other across threads. A capture is
+// thread-local, but such a call from another thread can still wait on the
+// capturing stream, which ends that capture in error. Recursive: a capture
+// section may allocate through the same thread's calls.
+inline std::recursive_mutex& capture_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+typedef std::lock_guard<std::recursive_mutex> CaptureLock;
+
+inline hipError_t sync_memcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    CaptureLock lk(capture_mutex());
+    return hipMemcpy(dst, src, bytes, kind);
+}
+inline hipError_t stream_create(hipStream_t* s) {
+    CaptureLock lk(capture_mutex());
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+inline hipError_t stream_destroy(hipStream_t s) {
+    CaptureLock lk(capture_mutex());
+    return hipStreamDestroy(s);
+}
+
 // ------------------------------------------------------ device buffer ------
 template <typename T>
 struct DevBuf {
@@ -47,13 +74,17 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) {
+            CaptureLock lk(capture_mutex());
+            (void)hipFree(ptr);
+        }
         ptr = nullptr;
         n = 0;
     }
     // grow-only allocation; contents are not preserved
     void reserve(size_t count) {
         if (count <= n && ptr) return;
+        CaptureLock lk(capture_mutex());
         release();
         if (count == 0) count = 1;
         hipError_t e = hipMalloc(&ptr, count * sizeof(T));
@@ -79,12 +110,16 @@ struct PinnedBuf {
     PinnedBuf& operator=(const PinnedBuf&) = delete;
     ~PinnedBuf() { release(); }
     void release() {
-        if (ptr) (void)hipHostFree(ptr);
+        if (ptr) {
+            CaptureLock lk(capture_mutex());
+            (void)hipHostFree(ptr);
+        }
         ptr = nullptr;
         n = 0;
     }
     void reserve(size_t count) {
         if (count <= n && ptr) return;
+        CaptureLock lk(capture_mutex());
         release();
         if (count == 0) count = 1;
         hipError_t e = hipHostMalloc((void**)&ptr, count * sizeof(T), hipHostMallocDefault);

@@ -97,6 +97,11 @@ void ServeGpu::work(ServeJob* j) {
                      p_.porder, p_.kind == kServeFP32 ? kFP32 : kFP64, p_.max_iters, p_.max_nrhs,
                      p_.tol, p_.r0, p_.k0, p_.dt, p_.mob_interval, j->seed);
             bd.init();
+            // step tails run eagerly: a graph capture in this thread would
+            // have to exclude every other job's allocations, frees and
+            // synchronous copies (they can synchronise with the capturing
+            // stream), and a graph saves about 2 % of a single system's tail
+            bd.set_graphs(false);
             bd.lanczos()->eval_threads = 2;   // jobs run side by side
             bd.set_cancel(&stop_);
             for (int k = 0; k < j->nsteps && !stop_; k++) {

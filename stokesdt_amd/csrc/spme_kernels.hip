@@ -1395,11 +1395,11 @@ void SpmeGpu<T>::create(int npos_, double box_, double xi_, int K_, int porder_,
         tw[2 * j + 1] = sin(-2.0 * M_PI * j / K);
     }
     d_tw.reserve(2 * K);
-    SDT_HIP_CHECK(hipMemcpy(d_tw.get(), tw.data(), sizeof(double) * 2 * K, hipMemcpyHostToDevice));
+    SDT_HIP_CHECK(sync_memcpy(d_tw.get(), tw.data(), sizeof(double) * 2 * K, hipMemcpyHostToDevice));
     std::vector<double> bs;
     bspline_moduli(K, porder, bs);
     d_bsp2.reserve(K);
-    SDT_HIP_CHECK(hipMemcpy(d_bsp2.get(), bs.data(), sizeof(double) * K, hipMemcpyHostToDevice));
+    SDT_HIP_CHECK(sync_memcpy(d_bsp2.get(), bs.data(), sizeof(double) * K, hipMemcpyHostToDevice));
     d_frac.reserve(3 * (size_t)npos);
     d_base.reserve(3 * (size_t)npos);
     d_frac_u.reserve(3 * (size_t)npos);
