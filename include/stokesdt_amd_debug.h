@@ -43,8 +43,11 @@ int sdt_debug_spme_interp(sdt_mob_t mob, const double* mesh, double alpha, doubl
  * (1024, 16), (1024, 16), (512, 16), (512, 8), (256, 8), (256, 4);
  * spread_form: the point-owner spread's force reads in wide FP32 launches
  * (0 the built-in choice = scalar loads, 1 staged in LDS, 2 scalar loads),
- * plus 4 to store the meshes as whole rows through LDS instead of per lane. */
-int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_form);
+ * plus 4 to store the meshes as whole rows through LDS instead of per lane;
+ * fft_form: 0 the fused x + y plane kernels where a plane fits LDS, 1 the
+ * separate x and y passes. */
+int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_form,
+                         int fft_form);
 /* Mean device time (ms, HIP events on the handle's stream) of each stage of
  * one packed matvec with b right-hand sides over `reps` repetitions:
  * t[0] spread, t[1] forward x/y FFT, t[2] z FFT + influence + inverse z,

@@ -105,7 +105,7 @@ SIGNATURES = {
     "sdt_debug_band_sqrt": (_I, [_I, _I, _P, _I, _P]),
     "sdt_debug_spme_spread": (_I, [_P, _P, _P]),
     "sdt_debug_spme_interp": (_I, [_P, _P, _D, _P]),
-    "sdt_debug_spme_forms": (_I, [_P, _I, _I, _I]),
+    "sdt_debug_spme_forms": (_I, [_P, _I, _I, _I, _I]),
     "sdt_bench_stage_times": (_I, [_P, _I, _I, _P]),
     "sdt_bench_copy": (_I, [_I64, _I, ctypes.POINTER(_D)]),
     "sdt_bench_bd_steps": (_I, [_P, _I, ctypes.POINTER(_D)]),
@@ -284,12 +284,13 @@ class MobSpme:
         _check(lib().sdt_debug_spme_interp(self.h, _ptr(m), alpha, _ptr(v)))
         return v
 
-    def set_forms(self, interp=0, z=0, spread=0):
+    def set_forms(self, interp=0, z=0, spread=0, fft=0):
         """Launch forms (testing and measurement): interp meshes per LDS pass
         (0 built-in, 1, 2 or 4 (FP32)); z transform shape (0 built-in,
         1 .. 5); spread form (force reads 0 built-in, 1 LDS, 2 scalar; plus 4
-        for row-tiled mesh stores)."""
-        _check(lib().sdt_debug_spme_forms(self.h, int(interp), int(z), int(spread)))
+        for row-tiled mesh stores); x / y transforms (0 fused planes where
+        they fit, 1 separate passes)."""
+        _check(lib().sdt_debug_spme_forms(self.h, int(interp), int(z), int(spread), int(fft)))
 
     def set_interp_form(self, form):
         self.set_forms(interp=form)

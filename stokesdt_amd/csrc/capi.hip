@@ -781,9 +781,11 @@ int sdt_debug_spme_spread(sdt_mob_t mob, const double* f, double* mesh) {
     });
 }
 
-int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_form) {
+int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_form,
+                         int fft_form) {
     SDT_CHECK_HANDLE(mob);
     return guard([&] {
+        SDT_REQUIRE(fft_form == 0 || fft_form == 1, SDT_EINVAL, "fft form must be 0 or 1");
         SDT_REQUIRE(spread_form >= 0 && spread_form <= 6 && (spread_form & 3) != 3, SDT_EINVAL,
                     "spread form: force reads 0, 1 or 2, plus 4 for row-tiled mesh stores");
         SDT_REQUIRE(interp_form == 0 || interp_form == 1 || interp_form == 2 || interp_form == 4,
@@ -794,11 +796,13 @@ int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_
             m->spme32()->interp_form = interp_form;
             m->spme32()->z_form = z_form;
             m->spme32()->spread_form = spread_form;
+            m->spme32()->fft_form = fft_form;
         }
         if (m->spme64()) {
             m->spme64()->interp_form = interp_form;
             m->spme64()->z_form = z_form;
             m->spme64()->spread_form = spread_form;
+            m->spme64()->fft_form = fft_form;
         }
     });
 }

@@ -59,6 +59,9 @@ struct SpmeGpu {
     // built-in = scalar loads, 1 staged in LDS, 2 scalar loads); bit 2, mesh
     // stores as whole rows through LDS instead of per lane
     int spread_form = 0;
+    // x / y transforms: 0 the fused plane kernels where a plane fits LDS,
+    // 1 the separate x and y passes
+    int fft_form = 0;
     // replaces the launch choices of a width class (re-plans the FFT order)
     void set_tuning(const SpmeTuning& t, int width);
     DevBuf<double> d_tw;   // K roots of unity (cos, sin) of -2*pi*j/K

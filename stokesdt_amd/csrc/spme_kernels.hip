@@ -1615,7 +1615,7 @@ void SpmeGpu<T>::fft_forward(int b, hipStream_t s) {
     const FFTArgs fa = fft_args(plans[width_class(b)]);
     const int Kh = K / 2 + 1;
     const long long nrows = (long long)nmesh(b) * K * K;
-    if (const int pv = plane_path<T>(K)) {
+    if (const int pv = fft_form == 1 ? 0 : plane_path<T>(K)) {
         const size_t lds = plane_lds_bytes<T>(K);
         const dim3 grid((unsigned)(nmesh(b) * K));
         if (sizeof(T) == 4 && K == 128) {
@@ -1704,7 +1704,7 @@ void SpmeGpu<T>::fft_backward(int b, hipStream_t s) {
     const FFTArgs fa = fft_args(plans[width_class(b)]);
     const int Kh = K / 2 + 1;
     const long long nrows = (long long)nmesh(b) * K * K;
-    if (const int pv = plane_path<T>(K)) {
+    if (const int pv = fft_form == 1 ? 0 : plane_path<T>(K)) {
         const size_t lds = plane_lds_bytes<T>(K);
         const dim3 grid((unsigned)(nmesh(b) * K));
         if (sizeof(T) == 4 && K == 128) {

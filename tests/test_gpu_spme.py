@@ -351,3 +351,24 @@ def test_spread_forms_identical(sdt, orc, prec):
         assert orc.frob_rel_err(o.mul_vector(f, mode="recip"), out[(b, 0)]) < (1e-10 if prec == 0 else 1e-5)
     with pytest.raises(Exception):
         g.set_forms(0, 0, 3)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fused_and_separate_xy_transforms_agree(sdt, orc, prec):
+    # the x / y transforms run as one fused plane kernel where a plane fits
+    # LDS and as separate x and y passes otherwise; forcing the separate
+    # passes on the same mesh checks one against the other (FP64: the same
+    # radix chain, identical; FP32 K = 128: the fused kernel's two-stage
+    # 16 x 8 transform against the generic chain, rounding only)
+    n, box = 2000, 30.0
+    g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, 128, 6, False, 91, prec)
+    f = np.random.default_rng(92).standard_normal((3 * n, 9))
+    g.set_forms(0, 0, 0, 1)
+    sep = g.recip_mul_vector(f)
+    g.set_forms(0, 0, 0, 0)
+    fused = g.recip_mul_vector(f)
+    if prec == 0:
+        assert np.array_equal(sep, fused)
+    else:
+        assert orc.frob_rel_err(fused, sep) < 1e-6
+    assert orc.frob_rel_err(o.mul_vector(f, mode="recip"), sep) < (1e-10 if prec == 0 else 1e-5)
