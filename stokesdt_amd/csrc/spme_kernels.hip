@@ -709,15 +709,27 @@ fft_x_r2c_kernel(FFTArgs fa, long long nrows, const T* __restrict__ real,
     cplx<T>* bufb = bufa + npen * PS;
     load_twiddles(tw, tw_g, K);
     const long long row0 = (long long)blockIdx.x * NR;
-    for (int i = threadIdx.x; i < npen * K; i += blockDim.x) {
-        const int pen = (int)fdiv(i, fa.K_m), x = i - pen * K;
-        const long long ra = row0 + 2 * pen;
-        T va = T(0), vb = T(0);
-        if (ra < nrows) {
-            va = real[ra * K + x];
-            vb = real[(ra + 1) * K + x];
+    // the loads of a thread (XB rows pairs at a time) are in flight together
+    constexpr int XB = 8;
+    for (int i0 = threadIdx.x; i0 < npen * K; i0 += XB * blockDim.x) {
+        T va[XB], vb[XB];
+#pragma unroll
+        for (int u = 0; u < XB; u++) {
+            const int i = i0 + u * (int)blockDim.x;
+            const int pen = (int)fdiv(i, fa.K_m), x = i - pen * K;
+            const long long ra = row0 + 2 * pen;
+            va[u] = vb[u] = T(0);
+            if (i < npen * K && ra < nrows) {
+                va[u] = real[ra * K + x];
+                vb[u] = real[(ra + 1) * K + x];
+            }
         }
-        bufa[pen * PS + lpad(x)] = cplx<T>{va, vb};
+#pragma unroll
+        for (int u = 0; u < XB; u++) {
+            const int i = i0 + u * (int)blockDim.x;
+            const int pen = (int)fdiv(i, fa.K_m), x = i - pen * K;
+            if (i < npen * K) bufa[pen * PS + lpad(x)] = cplx<T>{va[u], vb[u]};
+        }
     }
     __syncthreads();
     const cplx<T>* res = fft_pencils<T, -1>(bufa, bufb, npen, PS, fa, tw);
@@ -745,19 +757,37 @@ fft_x_c2r_kernel(FFTArgs fa, long long nrows, const cplx<T>* __restrict__ spec,
     cplx<T>* bufb = bufa + npen * PS;
     load_twiddles(tw, tw_g, K);
     const long long row0 = (long long)blockIdx.x * NR;
-    for (int i = threadIdx.x; i < npen * K; i += blockDim.x) {
-        const int pen = (int)fdiv(i, fa.K_m), k = i - pen * K;
-        const long long ra = row0 + 2 * pen;
-        cplx<T> z{T(0), T(0)};
-        if (ra < nrows) {
-            const int kk = (k <= K / 2) ? k : K - k;
-            cplx<T> A = spec[ra * Kh + kk];
-            cplx<T> B = spec[(ra + 1) * Kh + kk];
-            if (kk == 0 || 2 * kk == K) { A.y = T(0); B.y = T(0); }
-            if (k <= K / 2) z = cplx<T>{A.x - B.y, A.y + B.x};
-            else z = cplx<T>{A.x + B.y, B.x - A.y};
+    // the loads of a thread (XB at a time) are in flight together
+    constexpr int XB = 8;
+    for (int i0 = threadIdx.x; i0 < npen * K; i0 += XB * blockDim.x) {
+        cplx<T> A[XB], B[XB];
+#pragma unroll
+        for (int u = 0; u < XB; u++) {
+            const int i = i0 + u * (int)blockDim.x;
+            const int pen = (int)fdiv(i, fa.K_m), k = i - pen * K;
+            const long long ra = row0 + 2 * pen;
+            A[u] = B[u] = cplx<T>{T(0), T(0)};
+            if (i < npen * K && ra < nrows) {
+                const int kk = (k <= K / 2) ? k : K - k;
+                A[u] = spec[ra * Kh + kk];
+                B[u] = spec[(ra + 1) * Kh + kk];
+            }
         }
-        bufa[pen * PS + lpad(k)] = z;
+#pragma unroll
+        for (int u = 0; u < XB; u++) {
+            const int i = i0 + u * (int)blockDim.x;
+            const int pen = (int)fdiv(i, fa.K_m), k = i - pen * K;
+            const long long ra = row0 + 2 * pen;
+            cplx<T> z{T(0), T(0)};
+            if (i < npen * K && ra < nrows) {
+                const int kk = (k <= K / 2) ? k : K - k;
+                cplx<T> a = A[u], bb = B[u];
+                if (kk == 0 || 2 * kk == K) { a.y = T(0); bb.y = T(0); }
+                if (k <= K / 2) z = cplx<T>{a.x - bb.y, a.y + bb.x};
+                else z = cplx<T>{a.x + bb.y, bb.x - a.y};
+            }
+            if (i < npen * K) bufa[pen * PS + lpad(k)] = z;
+        }
     }
     __syncthreads();
     const cplx<T>* res = fft_pencils<T, +1>(bufa, bufb, npen, PS, fa, tw);
@@ -788,10 +818,23 @@ fft_y_kernel(FFTArgs fa, cplx<T>* __restrict__ spec, const double* __restrict__ 
     const int kx0 = (int)(blockIdx.x % nkb) * NP;
     const long long plane = blockIdx.x / nkb;    // mesh * K + z
     cplx<T>* base = spec + plane * K * Kh;
-    for (int i = threadIdx.x; i < NP * K; i += blockDim.x) {
-        const int y = i >> lnp, j = i & (NP - 1);
-        const int kx = kx0 + j;
-        bufa[j * PS + lpad(y)] = (kx < Kh) ? base[(size_t)y * Kh + kx] : cplx<T>{T(0), T(0)};
+    // the loads of a thread (YB at a time) are in flight together
+    constexpr int YB = 8;
+    for (int i0 = threadIdx.x; i0 < NP * K; i0 += YB * blockDim.x) {
+        cplx<T> v[YB];
+#pragma unroll
+        for (int u = 0; u < YB; u++) {
+            const int i = i0 + u * (int)blockDim.x;
+            const int y = i >> lnp, j = i & (NP - 1);
+            const int kx = kx0 + j;
+            v[u] = (i < NP * K && kx < Kh) ? base[(size_t)y * Kh + kx] : cplx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < YB; u++) {
+            const int i = i0 + u * (int)blockDim.x;
+            const int y = i >> lnp, j = i & (NP - 1);
+            if (i < NP * K) bufa[j * PS + lpad(y)] = v[u];
+        }
     }
     __syncthreads();
     const cplx<T>* res = fft_pencils<T, SIGN>(bufa, bufb, NP, PS, fa, tw);
