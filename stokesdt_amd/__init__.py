@@ -287,8 +287,9 @@ class MobSpme:
     def set_forms(self, interp=0, z=0, spread=0, fft=0):
         """Launch forms (testing and measurement): interp meshes per LDS pass
         (0 built-in, 1, 2 or 4 (FP32)); z transform shape (0 built-in,
-        1 .. 5); spread form (force reads 0 built-in, 1 LDS, 2 scalar; plus 4
-        for row-tiled mesh stores); x / y transforms (0 fused planes where
+        1 .. 5); spread form (0 built-in, 1 forces in LDS, 2 scalar force
+        reads, 3 matrix-core sums; plus 4 for row-tiled mesh stores, plus 8
+        for column blocks in turn); x / y transforms (0 fused planes where
         they fit, 1 separate passes)."""
         _check(lib().sdt_debug_spme_forms(self.h, int(interp), int(z), int(spread), int(fft)))
 

@@ -786,8 +786,9 @@ int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_
     SDT_CHECK_HANDLE(mob);
     return guard([&] {
         SDT_REQUIRE(fft_form == 0 || fft_form == 1, SDT_EINVAL, "fft form must be 0 or 1");
-        SDT_REQUIRE(spread_form >= 0 && spread_form <= 6 && (spread_form & 3) != 3, SDT_EINVAL,
-                    "spread form: force reads 0, 1 or 2, plus 4 for row-tiled mesh stores");
+        SDT_REQUIRE(spread_form >= 0 && spread_form <= 15, SDT_EINVAL,
+                    "spread form: 0 .. 3 (built-in, LDS, scalar, matrix cores), plus 4 for "
+                    "row-tiled mesh stores, plus 8 for column blocks in turn (with 3)");
         SDT_REQUIRE(interp_form == 0 || interp_form == 1 || interp_form == 2 || interp_form == 4,
                     SDT_EINVAL, "interp meshes per pass must be 0 (built-in), 1, 2 or 4");
         SDT_REQUIRE(z_form >= 0 && z_form <= 5, SDT_EINVAL, "z transform form must be in [0, 5]");

@@ -352,8 +352,27 @@ constexpr int kPtMaxTiles = 200;   // contributor bucket tiles of one block (hos
 // NQ values at a wave-uniform address, through the scalar data cache into
 // scalar registers, so the FMAs take them as scalar operands and LDS serves
 // only the per-lane weights.
-template <typename T, int P, int NQ, bool BLK, bool WRAP, bool SGF = false>
-__global__ void __launch_bounds__(kPtThreads) __attribute__((amdgpu_waves_per_eu(NQ <= 12 ? 6 : 1)))
+// MF (FP32, BLK, 48 meshes, staged forces): the sums run on the matrix
+// cores. A wave's sub-block is the product W F of its 64 points x listed
+// candidates weight matrix and the candidates' 48 forces, taken 4 candidates
+// at a time by v_mfma_f32_16x16x4f32: m-tile m = the 16 points of z plane m
+// (row i = 4 y + x), n-tile t = component t of the 16 columns. Each lane
+// forms its A entry (point l % 16, candidate l / 16) from the staged 1D
+// weights, so no lane is spent on points outside a stencil, and its result
+// registers hold 4 consecutive x points of one mesh (one 16-byte store).
+// MA (with MF): one workgroup per mesh block takes every 16-column block in
+// turn (grid y = 1). When all its candidates fit the stage, they are found
+// and their weights staged once; each further column block then only loads
+// the candidates' forces into the stage before its sums and stores. The
+// meshes leave through an LDS tile after the stage (16 columns x 4 rows x 32
+// points of one z plane and component at a time, rows padded to 132 floats
+// so that the waves' 16-byte pieces land in distinct banks): every store
+// instruction then writes whole 128-byte rows.
+constexpr int kMaTileStride = 132;                       // floats per column in the tile
+constexpr int kMaTileBytes = 16 * kMaTileStride * 4;     // 8448
+template <typename T, int P, int NQ, bool BLK, bool WRAP, bool SGF = false, bool MF = false,
+          bool MA = false>
+__global__ void __launch_bounds__(kPtThreads) __attribute__((amdgpu_waves_per_eu(NQ <= 12 ? 6 : (MA ? 4 : 1))))
 spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile_start,
                  const double* __restrict__ frac, const int* __restrict__ base,
                  const double* __restrict__ a2, const double* __restrict__ X,
@@ -367,11 +386,17 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     constexpr int REC = SGF ? OF : OF + NQ;
     static_assert(NQ % 4 == 0, "NQ must be a multiple of 4");
     static_assert(!SGF || BLK, "scalar force reads need the blocked copy");
+    static_assert(!MF || (BLK && !SGF && !WRAP && NQ == 48 && sizeof(T) == 4),
+                  "matrix-core sums: FP32, blocked staged forces, 48 meshes, no wrap");
+    static_assert(!MA || MF, "column blocks in turn need the matrix-core sums");
     const T* __restrict__ xbb = Xb + (size_t)byl_of(blockIdx.x + blockIdx.y * gridDim.x,
                                                     gridDim.x * gridDim.y, gridDim.x) * npos * NQ;
     T* stage = reinterpret_cast<T*>(smem_raw);
     int* s_list = reinterpret_cast<int*>(stage + (size_t)cap * REC);   // [8][cap]
     int* s_pid = s_list + 8 * cap;                                       // [cap]
+    // MA: output tile after the stage (16-byte aligned)
+    [[maybe_unused]] T* ma_tile =
+        reinterpret_cast<T*>(smem_raw + ((((size_t)cap * (REC * sizeof(T) + 9 * sizeof(int))) + 15) & ~(size_t)15));
     __shared__ int s_wave[kPtThreads / 64];
     __shared__ int s_off[kPtMaxTiles + 1];
     __shared__ int s_beg[kPtMaxTiles];
@@ -383,9 +408,9 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     const int yz = bxl / nsx;
     const int x0 = sxi * 32, y0 = (yz % K4) * 4, z0 = (yz / K4) * 4;
     const int xlen = min(32, K - x0);
-    const int r0 = byl * (NQ / NC);
-    const int rb = min(NQ / NC, b - r0);
-    const int nq = rb * NC;
+    int r0 = byl * (NQ / NC);
+    int rb = min(NQ / NC, b - r0);
+    int nq = rb * NC;
     const bool wave_on = 4 * wave < xlen;
     // this lane's point, relative to the block origin
     const int lx = 4 * wave + (lane & 3), ly = (lane >> 2) & 3, lz = lane >> 4;
@@ -393,6 +418,13 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     T acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) acc[q] = T(0);
+    // MF: result tiles [z plane m][component t]
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 accm[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int t = 0; t < 3; t++) accm[m][t] = f4{0.f, 0.f, 0.f, 0.f};
 
     // contributor base tiles (distinct, with wrap): bases in [lo, lo + len);
     // wave 0 reads the bucket bounds of 64 tiles at a time and scans the
@@ -468,7 +500,37 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
         // the list is written and read by this wave only (LDS accesses of a
         // wave complete in order)
         __builtin_amdgcn_wave_barrier();
-        if constexpr (!WRAP) {
+        if constexpr (MF) {
+            // A entry of lane l: point (x, y) = (l % 4, (l / 4) % 4) of the
+            // sub-block, candidate l / 16 of the step; B entry: force
+            // component t of column l % 16 of the same candidate. Past the
+            // list end both are 0.
+            const int kq = lane >> 4, jj = lane & 15;
+            const int ax = 4 * wave + (lane & 3), ay = (lane >> 2) & 3;
+            for (int k0 = 0; k0 < cnt; k0 += 4) {
+                const int k = k0 + kq;
+                T wxy = T(0), wz[4] = {T(0), T(0), T(0), T(0)}, bf[3] = {T(0), T(0), T(0)};
+                if (k < cnt) {
+                    const T* rec = stage + (size_t)list[k] * REC;
+                    const int* rl = reinterpret_cast<const int*>(rec + OB);
+                    const int dx = (int)min((unsigned)(ax - rl[0]), (unsigned)P);
+                    const int dy = (int)min((unsigned)(ay - rl[1]), (unsigned)P);
+                    wxy = rec[dx] * rec[OWY + dy];
+#pragma unroll
+                    for (int m = 0; m < 4; m++)
+                        wz[m] = rec[OWZ + (int)min((unsigned)(m - rl[2]), (unsigned)P)];
+#pragma unroll
+                    for (int t = 0; t < 3; t++) bf[t] = rec[OF + 16 * t + jj];
+                }
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    const T a = wxy * wz[m];
+#pragma unroll
+                    for (int t = 0; t < 3; t++)
+                        accm[m][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bf[t], accm[m][t], 0, 0, 0);
+                }
+            }
+        } else if constexpr (!WRAP) {
             // no stencil reaches both ends of the block: the offset of this
             // lane's point in a candidate's stencil is a plain difference
             // (out of the stencil: P, a zero weight)
@@ -511,6 +573,47 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     };
 
     int nst = 0;
+    bool staged_all = false;   // MA: the stage holds every candidate
+    [[maybe_unused]] const int ncb = MA ? (b + NQ / NC - 1) / (NQ / NC) : 1;
+    int cb = 0;
+    // MA: the next column block starts here (a jump back, not a loop, so
+    // that the other instances compile without one)
+next_column_block:
+    {
+    const int cbl = MA ? cb : byl;
+    if constexpr (MA) {
+        r0 = cb * (NQ / NC);
+        rb = min(NQ / NC, b - r0);
+        nq = rb * NC;
+        __syncthreads();   // every wave is done with the previous block's forces
+    }
+    if (staged_all) {
+        // the staged candidates' forces of this column block
+        constexpr int VW = 16 / sizeof(T);
+        constexpr int NV = NQ / VW;
+        constexpr int U = 4;
+        typedef T VT __attribute__((ext_vector_type(16 / sizeof(T))));
+        const VT* src = reinterpret_cast<const VT*>(Xb + (size_t)cbl * npos * NQ);
+        for (int e0 = threadIdx.x; e0 < nst * NV; e0 += kPtThreads * U) {
+            VT v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * kPtThreads;
+                v[u] = VT(0);
+                if (e < nst * NV) v[u] = src[(size_t)s_pid[e / NV] * NV + (e - (e / NV) * NV)];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * kPtThreads;
+                if (e < nst * NV)
+                    *reinterpret_cast<VT*>(stage + (size_t)(e / NV) * REC + OF + (e - (e / NV) * NV) * VW) = v[u];
+            }
+        }
+        __syncthreads();
+        accumulate(nst);
+    } else {
+    nst = 0;
+    bool flushed = false;
     for (int c0 = 0; c0 < ncand; c0 += kPtThreads) {
         const int ci = c0 + (int)threadIdx.x;
         int is = -1, b3[3] = {0, 0, 0};
@@ -538,6 +641,7 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                 accumulate(nst);
                 __syncthreads();
                 nst = 0;
+                flushed = true;
             }
             const int taken = min(cap - nst, tot - done);
             const int first = nst;
@@ -569,7 +673,7 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                 constexpr int NV = NQ / VW;
                 constexpr int U = 4;
                 typedef T VT __attribute__((ext_vector_type(16 / sizeof(T))));
-                const VT* src = reinterpret_cast<const VT*>(Xb + (size_t)byl * npos * NQ);
+                const VT* src = reinterpret_cast<const VT*>(Xb + (size_t)cbl * npos * NQ);
                 for (int e0 = threadIdx.x; e0 < taken * NV; e0 += kPtThreads * U) {
                     VT v[U];
 #pragma unroll
@@ -626,6 +730,9 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     }
     __syncthreads();
     accumulate(nst);
+    // the mesh tiles below reuse the stage
+    staged_all = MA && !flushed && !tiled_out;
+    }
     // Mesh stores. Default: each lane stores its own point (a wave writes
     // 16-byte pieces of 16 rows; the eight waves' pieces of a row reach HBM
     // about 1.4x). tiled_out: the meshes leave through LDS (the stage is free
@@ -638,6 +745,60 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     static_assert(NQ % QC == 0, "mesh groups must divide NQ");
     T* tile = stage;
     const size_t K3 = (size_t)K * K * K;
+    if constexpr (MF) {
+        // result register v of lane l holds row 4 (l / 16) + v of a tile:
+        // x = v, y = l / 16 of z plane m; column l % 16 of component t
+        if (MA && !tiled_out) {
+            // through the LDS tile: wave w writes x in [4w, 4w + 4) of every
+            // (column, row); thread e then stores 16 bytes of row e / 8
+            const int jj = lane & 15, sy = lane >> 4;
+            const int e = threadIdx.x;
+            const int sj = e >> 5, sr = (e >> 3) & 3, sx = (e & 7) * 4;
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    __syncthreads();   // the previous tile has been stored
+                    if (wave_on)
+                        *reinterpret_cast<f4*>(ma_tile + jj * kMaTileStride + sy * 32 + 4 * wave) =
+                            accm[m][t];
+                    __syncthreads();
+                    if (sj < rb && sx < xlen) {
+                        const f4 v =
+                            *reinterpret_cast<const f4*>(ma_tile + sj * kMaTileStride + sr * 32 + sx);
+                        *reinterpret_cast<f4*>(mesh + ((size_t)(r0 + sj) * 3 + t) * K3 +
+                                               ((size_t)(z0 + m) * K + (y0 + sr)) * K + x0 + sx) = v;
+                    }
+                }
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int t = 0; t < 3; t++) accm[m][t] = f4{0.f, 0.f, 0.f, 0.f};
+            if (++cb < ncb) goto next_column_block;
+            return;
+        }
+        if (!tiled_out) {
+            const int jj = lane & 15, sy = lane >> 4;
+            if (wave_on && jj < rb) {
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+#pragma unroll
+                    for (int t = 0; t < 3; t++) {
+                        T* dst = mesh + ((size_t)(r0 + jj) * 3 + t) * K3 +
+                                 ((size_t)(z0 + m) * K + (y0 + sy)) * K + x0 + 4 * wave;
+                        *reinterpret_cast<f4*>(dst) = accm[m][t];
+                    }
+            }
+            if constexpr (MA) {
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+#pragma unroll
+                    for (int t = 0; t < 3; t++) accm[m][t] = f4{0.f, 0.f, 0.f, 0.f};
+                if (++cb < ncb) goto next_column_block;
+            }
+            return;
+        }
+    }
     if (!tiled_out) {
         if (!wave_on) return;
         const size_t pt = ((size_t)pz * K + py) * K + px;
@@ -662,8 +823,23 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
 #pragma unroll
     for (int q0 = 0; q0 < NQ; q0 += QC) {
         if (wave_on) {
+            if constexpr (MF) {
+                const int jj = lane & 15, sy = lane >> 4;
 #pragma unroll
-            for (int qq = 0; qq < QC; qq++) tile[(qq * 16 + lz * 4 + ly) * 32 + lx] = acc[q0 + qq];
+                for (int m = 0; m < 4; m++)
+#pragma unroll
+                    for (int t = 0; t < 3; t++) {
+                        const int qq = 16 * t + jj - q0;
+                        if (qq >= 0 && qq < QC) {
+#pragma unroll
+                            for (int v = 0; v < 4; v++)
+                                tile[(qq * 16 + m * 4 + sy) * 32 + 4 * wave + v] = accm[m][t][v];
+                        }
+                    }
+            } else {
+#pragma unroll
+                for (int qq = 0; qq < QC; qq++) tile[(qq * 16 + lz * 4 + ly) * 32 + lx] = acc[q0 + qq];
+            }
         }
         __syncthreads();
         for (int e = threadIdx.x; e < QC * 512; e += kPtThreads) {
@@ -685,6 +861,14 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
                     tile[e];
         }
         __syncthreads();
+    }
+    if constexpr (MA) {
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int t = 0; t < 3; t++) accm[m][t] = f4{0.f, 0.f, 0.f, 0.f};
+        if (++cb < ncb) goto next_column_block;
+    }
     }
 }
 
@@ -1521,9 +1705,17 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
         const SpmeTuning& tn = tune[width_class(b)];
         constexpr int NQL = sizeof(T) == 4 ? 48 : 24;
         const int NQ = pt_meshes<T>(b, NC);
-        // FP32 reads the blocked forces through the scalar path (measured
-        // faster at K = 128; FP64 keeps them staged in LDS)
-        const bool sgf = xb != nullptr && sizeof(T) == 4 && (spread_form & 3) != 1;
+        // Wide FP32 launches on the blocked copy sum on the matrix cores, one
+        // workgroup per mesh block over every column block (K = 128, 100
+        // columns: 1.08 ms against 2.17 ms for the vector-ALU sums with
+        // scalar force reads, the same sums to the bit); where the stencil
+        // can wrap, FP32 reads the blocked forces through the scalar path and
+        // FP64 stages them in LDS. Forms override this for tests.
+        const int fr = spread_form & 3;
+        const bool mf_ok = xb != nullptr && sizeof(T) == 4 && NQ == NQL && K >= 32 + porder - 1;
+        const bool mf = mf_ok && (fr == 0 || fr == 3);
+        const bool ma = mf && (fr == 0 || (spread_form & 8) != 0);
+        const bool sgf = xb != nullptr && sizeof(T) == 4 && !mf && fr != 1;
         const int rec = ((3 * (porder + 1) + 3 + 3) & ~3) + (sgf ? 0 : NQ);
         // few meshes: a 48 KB stage leaves room for three workgroups per CU
         const int stage_bytes =
@@ -1532,22 +1724,30 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
         // the output tiles reuse the stage's LDS: all meshes, or half of
         // them above 48 KB (the allocation is at least that)
         const size_t tile = (size_t)NQ * 512 * sizeof(T);
-        const size_t lds = std::max((size_t)cap * (rec * sizeof(T) + 9 * sizeof(int)),
-                                    tile <= 49152 ? tile : tile / 2);
+        size_t lds = std::max((size_t)cap * (rec * sizeof(T) + 9 * sizeof(int)),
+                              tile <= 49152 ? tile : tile / 2);
+        if (ma && !(spread_form & 4))
+            lds = (((size_t)cap * (rec * sizeof(T) + 9 * sizeof(int)) + 15) & ~(size_t)15) +
+                  kMaTileBytes;
         const int nsx = (K + 31) / 32;
-        dim3 grid(nsx * (K / 4) * (K / 4), ceil_div(b, NQ / NC));
-#define SDT_SPREAD_PWS(PP, QQ, BB, WW, SS)                                                     \
-    hipLaunchKernelGGL((spread_pt_kernel<T, PP, QQ, BB, WW, SS>), grid, dim3(kPtThreads), lds, s, \
+        dim3 grid(nsx * (K / 4) * (K / 4), ma ? 1 : ceil_div(b, NQ / NC));
+#define SDT_SPREAD_PWS(PP, QQ, BB, WW, SS, MM, AA)                                             \
+    hipLaunchKernelGGL((spread_pt_kernel<T, PP, QQ, BB, WW, SS, MM, AA>), grid, dim3(kPtThreads), \
+                       lds, s,                                                                   \
                        K, ts, ntile, nsx, d_tile_start.get(), d_frac.get(), d_base.get(),      \
                        poly ? d_a2.get() : (double*)nullptr, X, xb, npos, b, NQ / NC, NC, cap, \
                        d_real.get(), (spread_form & 4) ? 1 : 0)
 #define SDT_SPREAD_PW(PP, QQ, BB, WW)                                                          \
     do {                                                                                      \
         if constexpr (BB && sizeof(T) == 4) {                                                 \
-            if (sgf) SDT_SPREAD_PWS(PP, QQ, BB, WW, true);                                     \
-            else SDT_SPREAD_PWS(PP, QQ, BB, WW, false);                                        \
+            if (sgf) SDT_SPREAD_PWS(PP, QQ, BB, WW, true, false, false);                       \
+            else if constexpr (QQ == 48 && !WW) {                                             \
+                if (ma) SDT_SPREAD_PWS(PP, QQ, BB, WW, false, true, true);                     \
+                else if (mf) SDT_SPREAD_PWS(PP, QQ, BB, WW, false, true, false);               \
+                else SDT_SPREAD_PWS(PP, QQ, BB, WW, false, false, false);                      \
+            } else SDT_SPREAD_PWS(PP, QQ, BB, WW, false, false, false);                        \
         } else {                                                                              \
-            SDT_SPREAD_PWS(PP, QQ, BB, WW, false);                                             \
+            SDT_SPREAD_PWS(PP, QQ, BB, WW, false, false, false);                               \
         }                                                                                     \
     } while (0)
 #define SDT_SPREAD_PT(PP, QQ, BB)                                                              \

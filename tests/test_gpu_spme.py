@@ -334,23 +334,32 @@ def test_wide_full_product_overlapped_streams(sdt, orc, prec, tol):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_spread_forms_identical(sdt, orc, prec):
-    # the point-owner spread reads wide FP32 launches' forces through scalar
-    # loads or from LDS, and stores its meshes per lane or as whole rows
-    # through LDS: the sums are the same in every form
+    # the point-owner spread sums wide FP32 launches on the vector ALUs
+    # (forces from scalar loads or LDS) or on the matrix cores (one
+    # workgroup per column block, or per mesh block over all column blocks),
+    # and stores its meshes per lane or as whole rows through LDS. The
+    # matrix cores take 4 candidates per step in list order with one
+    # rounding per product-add, so every form gives the same sums. The full
+    # product (b >= 8) spreads from the column-blocked copy, which the
+    # matrix-core forms need; K = 48 leaves a 16-point partial x block.
     n, box = 3000, 30.0
     g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, 48, 6, False, 81, prec)
-    out = {}
-    for b in (1, 20):
+    forms = (0, 1, 2, 3, 4, 5, 6, 7, 11, 15)
+    for b in (1, 20, 40):
         f = np.random.default_rng(82 + b).standard_normal((3 * n, b))
-        for form in (0, 1, 2, 4, 5, 6):
+        out = {}
+        for form in forms:
             g.set_forms(0, 0, form)
-            out[(b, form)] = g.recip_mul_vector(f)
+            out[form] = (g.recip_mul_vector(f), g.mul_vector(f))
         g.set_forms(0, 0, 0)
-        for form in (1, 2, 4, 5, 6):
-            assert np.array_equal(out[(b, 0)], out[(b, form)]), (b, form)
-        assert orc.frob_rel_err(o.mul_vector(f, mode="recip"), out[(b, 0)]) < (1e-10 if prec == 0 else 1e-5)
+        for form in forms[1:]:
+            for k in range(2):
+                assert np.array_equal(out[0][k], out[form][k]), (b, form, k)
+        tol = 1e-10 if prec == 0 else 1e-5
+        assert orc.frob_rel_err(o.mul_vector(f, mode="recip"), out[0][0]) < tol
+        assert orc.frob_rel_err(o.mul_vector(f), out[0][1]) < tol
     with pytest.raises(Exception):
-        g.set_forms(0, 0, 3)
+        g.set_forms(0, 0, 16)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
