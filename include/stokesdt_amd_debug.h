@@ -49,6 +49,10 @@ int sdt_debug_spme_interp(sdt_mob_t mob, const double* mesh, double alpha, doubl
  * gives the same sums;
  * fft_form: 0 the fused x + y plane kernels where a plane fits LDS, 1 the
  * separate x and y passes. */
+/* Right-hand sides per launch of an SPME mobility's mulvector (default 16;
+ * [1, 1024]): a wider launch runs every column block of a product in one
+ * spread and interpolation, as the block-Lanczos batches do. */
+int sdt_debug_mob_launch_width(sdt_mob_t mob, int width);
 int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_form,
                          int fft_form);
 /* Mean device time (ms, HIP events on the handle's stream) of each stage of

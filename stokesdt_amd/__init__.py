@@ -106,6 +106,7 @@ SIGNATURES = {
     "sdt_debug_spme_spread": (_I, [_P, _P, _P]),
     "sdt_debug_spme_interp": (_I, [_P, _P, _D, _P]),
     "sdt_debug_spme_forms": (_I, [_P, _I, _I, _I, _I]),
+    "sdt_debug_mob_launch_width": (_I, [_P, _I]),
     "sdt_bench_stage_times": (_I, [_P, _I, _I, _P]),
     "sdt_bench_copy": (_I, [_I64, _I, ctypes.POINTER(_D)]),
     "sdt_bench_bd_steps": (_I, [_P, _I, ctypes.POINTER(_D)]),
@@ -295,6 +296,10 @@ class MobSpme:
 
     def set_interp_form(self, form):
         self.set_forms(interp=form)
+
+    def set_launch_width(self, width):
+        """Right-hand sides per launch of mul_vector (default 16)."""
+        _check(lib().sdt_debug_mob_launch_width(self.h, int(width)))
 
     def stage_times(self, b, reps=5):
         t = np.zeros(7)

@@ -808,6 +808,14 @@ int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_
     });
 }
 
+int sdt_debug_mob_launch_width(sdt_mob_t mob, int width) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        SDT_REQUIRE(width >= 1 && width <= 1024, SDT_EINVAL, "launch width must be in [1, 1024]");
+        spme_of(mob)->set_launch_width(width);
+    });
+}
+
 int sdt_debug_spme_interp(sdt_mob_t mob, const double* mesh, double alpha, double* v) {
     SDT_CHECK_HANDLE(mob);
     return guard([&] {

@@ -224,7 +224,7 @@ void MobSpmeGpu::mul_vector_impl(int mode, int nrhs, double alpha, int ldf, cons
                 "leading dimensions must be at least 3*npos");
     SDT_REQUIRE(f != nullptr && v != nullptr, SDT_EINVAL, "null vector");
     hipStream_t s = stream_;
-    const int chunk = 16;
+    const int chunk = launch_width_;
     const int bmax = std::min(nrhs, chunk);
     d_X_.reserve((size_t)3 * npos_ * bmax);
     d_Y_.reserve((size_t)3 * npos_ * bmax);

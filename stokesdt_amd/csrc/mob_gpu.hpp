@@ -78,8 +78,11 @@ class MobSpmeGpu : public MobGpu {
     SpmeTuning tuning(int b) const;
     double xi() const { return xi_; }
     bool mono() const { return mono_; }
+    // right-hand sides per launch of mul_vector (bounds the mesh memory)
+    void set_launch_width(int w) { launch_width_ = w; }
 
   private:
+    int launch_width_ = 16;
     void mul_vector_impl(int mode, int nrhs, double alpha, int ldf, const double* f, double beta,
                          int ldv, double* v, bool dev, bool sync);
     template <typename T>

@@ -181,17 +181,21 @@ def test_init_and_state_errors(sdt):
 
 def test_large_system_fp32_vs_fp64(sdt, orc):
     # size-independent properties at the N = 100 000, phi = 0.3, K = 128,
-    # p = 6 shape: FP32 within 1e-5 of FP64, FP64 operator symmetric
+    # p = 6 shape: FP32 within 1e-5 of FP64, FP64 operator symmetric. The 40
+    # columns go through in one launch (as the block-Lanczos batches do): the
+    # FP32 spread sums three column blocks, the last one partial, on the
+    # matrix cores in one workgroup per mesh block.
     n = 100_000
     box = (4.0 * np.pi * n / (3.0 * 0.3)) ** (1.0 / 3.0)
     pos, rdi = uniform_system(n, box, 2024)
     rng = np.random.default_rng(1)
-    f = rng.standard_normal((3 * n, 2))
+    f = rng.standard_normal((3 * n, 40))
     outs = []
     for prec in (sdt.FP64, sdt.FP32):
         g = sdt.MobSpme(n, rdi, box, 0.896, 5.0, 128, 6, prec)
         assert g.init()
         g.update(pos)
+        g.set_launch_width(64)
         outs.append(g.mul_vector(f))
     assert orc.frob_rel_err(outs[0], outs[1]) < 1e-5
     x, y = f[:, 0], f[:, 1]
@@ -341,9 +345,12 @@ def test_spread_forms_identical(sdt, orc, prec):
     # matrix cores take 4 candidates per step in list order with one
     # rounding per product-add, so every form gives the same sums. The full
     # product (b >= 8) spreads from the column-blocked copy, which the
-    # matrix-core forms need; K = 48 leaves a 16-point partial x block.
+    # matrix-core forms need; each product is one launch (b = 20 and 40 are
+    # two and three column blocks, the last one partial); K = 48 leaves a
+    # 16-point partial x block.
     n, box = 3000, 30.0
     g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, 48, 6, False, 81, prec)
+    g.set_launch_width(64)
     forms = (0, 1, 2, 3, 4, 5, 6, 7, 11, 15)
     for b in (1, 20, 40):
         f = np.random.default_rng(82 + b).standard_normal((3 * n, b))
