@@ -13,6 +13,15 @@ __device__ __forceinline__ double pair_rr(double dx, double dy, double dz) {
     return fma(dx, dx, fma(dy, dy, dz * dz));
 }
 
+// remainder(d, box) for a position difference whose quotient d / box is not
+// within rounding of a half-integer (every pair within a cutoff below box / 2:
+// its min image is far from +-box / 2): n = rint(d / box) then equals the
+// quotient remainder() uses, and d - n box is exactly representable, so the
+// single fused multiply-add returns the same bits without a division
+__device__ __forceinline__ double min_image(double d, double box, double ibox) {
+    return fma(-rint(d * ibox), box, d);
+}
+
 template <typename T>
 __device__ __forceinline__ void ewald_scalars(T xi, T r, T aa, T ab, T* xa, T* ya) {
     const T xir = xi * r;

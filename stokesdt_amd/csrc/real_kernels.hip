@@ -52,6 +52,7 @@ real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restri
         const T self_a =
             (T)(1.0 / aa_d - xi / sqrt(M_PI) * (6.0 - 40.0 / 3.0 * xi * xi * aa_d * aa_d));
         const int kend = rowptr[i + 1];
+        const double ibox = 1.0 / box;
         for (int k = rowptr[i] + sub; k < kend; k += LPR) {
             const int j = colidx[k];
             XT xv[3][RC];
@@ -80,9 +81,11 @@ real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restri
                 bxx = byy = bzz = self_a;
                 bxy = bxz = byz = T(0);
             } else {
-                const double dx = remainder(pos[3 * j] - x1, box);
-                const double dy = remainder(pos[3 * j + 1] - y1, box);
-                const double dz = remainder(pos[3 * j + 2] - z1, box);
+                // listed pairs lie within the cutoff, so the min image equals
+                // remainder(., box) bit for bit
+                const double dx = min_image(pos[3 * j] - x1, box, ibox);
+                const double dy = min_image(pos[3 * j + 1] - y1, box, ibox);
+                const double dz = min_image(pos[3 * j + 2] - z1, box, ibox);
                 const double ab_d = rdi[j];
                 const double sab = aa_d + ab_d;
                 if (pair_rr(dx, dy, dz) < sab * sab) continue;   // overlap pass
