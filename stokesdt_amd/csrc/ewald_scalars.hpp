@@ -42,4 +42,28 @@ __device__ __forceinline__ void ewald_scalars(T xi, T r, T aa, T ab, T* xa, T* y
     *xa = T(0.75) * (b0 + b2 * s6) + *ya;
 }
 
+// The same scalars from r and its reciprocal ir = 1 / r, with products by ir
+// in place of the divisions by r, r^2 and r^3 (the matvec's gather loop,
+// which needs ir for the unit vector anyway); equal to ewald_scalars to a
+// few rounding errors.
+template <typename T>
+__device__ __forceinline__ void ewald_scalars_ir(T xi, T r, T ir, T aa, T ab, T* xa, T* ya) {
+    const T xir = xi * r;
+    const T xir2 = xir * xir;
+    const T ir2 = ir * ir;
+    const T ir3 = ir2 * ir;
+    const T erfcxir = erfc(xir);
+    const T expxir2 = xi * T(0.56418958354775628694807945156077) * exp(-xir2);  // xi/sqrt(pi)
+    const T er = erfcxir * ir;
+    const T a0 = T(2) * (T(-3) + xir2 * T(2)) * expxir2 + er;
+    const T b0 = T(2) * (T(1) + xir2 * T(-2)) * expxir2 + er;
+    const T a2 = T(4) * ir2 * (T(1) + xir2 * (T(14) + xir2 * (T(-20) + xir2 * T(4)))) * expxir2 +
+                 T(2) * ir3 * erfcxir;
+    const T b2 = T(-4) * ir2 * (T(3) + xir2 * (T(2) + xir2 * (T(-16) + xir2 * T(4)))) * expxir2 -
+                 T(6) * ir3 * erfcxir;
+    const T s6 = (aa * aa + ab * ab) * T(1.0 / 6.0);
+    *ya = T(0.75) * (a0 + a2 * s6);
+    *xa = T(0.75) * (b0 + b2 * s6) + *ya;
+}
+
 }  // namespace sdt

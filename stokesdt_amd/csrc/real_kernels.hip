@@ -93,9 +93,9 @@ real_matvec_kernel(int npos, const int* __restrict__ rowptr, const int* __restri
                 const T rr = rx * rx + ry * ry + rz * rz;
                 const T r = sqrt(rr);
                 const T ab = (T)ab_d;
-                T xa, ya;
-                ewald_scalars<T>(txi, r, aa, ab, &xa, &ya);
                 const T ir = T(1) / r;
+                T xa, ya;
+                ewald_scalars_ir<T>(txi, r, ir, aa, ab, &xa, &ya);
                 const T ex = rx * ir, ey = ry * ir, ez = rz * ir;
                 const T d = xa - ya;
                 bxx = d * ex * ex + ya; bxy = d * ex * ey; bxz = d * ex * ez;
@@ -219,9 +219,11 @@ void real_matvec(int npos, const int* d_rowptr, const int* d_colidx, const doubl
 __device__ __forceinline__ bool overlapping(int i, int j, const double* pos, const double* rdi,
                                             double box) {
     if (i == j) return false;
-    const double dx = remainder(pos[3 * j] - pos[3 * i], box);
-    const double dy = remainder(pos[3 * j + 1] - pos[3 * i + 1], box);
-    const double dz = remainder(pos[3 * j + 2] - pos[3 * i + 2], box);
+    // listed pairs: the same min image as the matvec's (remainder's bits)
+    const double ibox = 1.0 / box;
+    const double dx = min_image(pos[3 * j] - pos[3 * i], box, ibox);
+    const double dy = min_image(pos[3 * j + 1] - pos[3 * i + 1], box, ibox);
+    const double dz = min_image(pos[3 * j + 2] - pos[3 * i + 2], box, ibox);
     const double sab = rdi[i] + rdi[j];
     return pair_rr(dx, dy, dz) < sab * sab;
 }
@@ -304,11 +306,12 @@ real_overlap_kernel(int nrows, const int* __restrict__ rows,
     for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int r = 0; r < kOvRC; r++) acc[c][r] = 0.0;
+    const double ibox = 1.0 / box;
     for (int k = orp[i]; k < orp[i + 1]; k++) {
         const int j = ocol[k];
-        const double dx = remainder(pos[3 * j] - pos[3 * i], box);
-        const double dy = remainder(pos[3 * j + 1] - pos[3 * i + 1], box);
-        const double dz = remainder(pos[3 * j + 2] - pos[3 * i + 2], box);
+        const double dx = min_image(pos[3 * j] - pos[3 * i], box, ibox);
+        const double dy = min_image(pos[3 * j + 1] - pos[3 * i + 1], box, ibox);
+        const double dz = min_image(pos[3 * j + 2] - pos[3 * i + 2], box, ibox);
         const double ab = rdi[j];
         const double rr = dx * dx + dy * dy + dz * dz;
         const double rv = sqrt(rr);
