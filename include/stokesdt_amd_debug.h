@@ -53,6 +53,12 @@ int sdt_debug_spme_interp(sdt_mob_t mob, const double* mesh, double alpha, doubl
  * [1, 1024]): a wider launch runs every column block of a product in one
  * spread and interpolation, as the block-Lanczos batches do. */
 int sdt_debug_mob_launch_width(sdt_mob_t mob, int width);
+/* Launch choices of an initialised SPME mobility for every launch width
+ * (the autotuner's parameters; 0 = built-in): FFT radix order 0 .. 2,
+ * interp strip width 1, 2 or 4 tiles, interp particle chunk 32 .. 128,
+ * spread candidate stage 16 .. 64 KB. */
+int sdt_debug_spme_tuning(sdt_mob_t mob, int fft_order, int interp_sxt, int interp_ch,
+                          int spread_stage_kb);
 int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_form,
                          int fft_form);
 /* Mean device time (ms, HIP events on the handle's stream) of each stage of

@@ -107,6 +107,7 @@ SIGNATURES = {
     "sdt_debug_spme_interp": (_I, [_P, _P, _D, _P]),
     "sdt_debug_spme_forms": (_I, [_P, _I, _I, _I, _I]),
     "sdt_debug_mob_launch_width": (_I, [_P, _I]),
+    "sdt_debug_spme_tuning": (_I, [_P, _I, _I, _I, _I]),
     "sdt_bench_stage_times": (_I, [_P, _I, _I, _P]),
     "sdt_bench_copy": (_I, [_I64, _I, ctypes.POINTER(_D)]),
     "sdt_bench_bd_steps": (_I, [_P, _I, ctypes.POINTER(_D)]),
@@ -300,6 +301,12 @@ class MobSpme:
     def set_launch_width(self, width):
         """Right-hand sides per launch of mul_vector (default 16)."""
         _check(lib().sdt_debug_mob_launch_width(self.h, int(width)))
+
+    def set_tuning(self, fft_order=0, interp_sxt=0, interp_ch=0, spread_stage_kb=0):
+        """Launch choices for every width (0 = built-in): FFT radix order,
+        interp strip width and particle chunk, spread stage size in KB."""
+        _check(lib().sdt_debug_spme_tuning(self.h, int(fft_order), int(interp_sxt),
+                                           int(interp_ch), int(spread_stage_kb)))
 
     def stage_times(self, b, reps=5):
         t = np.zeros(7)

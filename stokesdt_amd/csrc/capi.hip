@@ -808,6 +808,21 @@ int sdt_debug_spme_forms(sdt_mob_t mob, int interp_form, int z_form, int spread_
     });
 }
 
+int sdt_debug_spme_tuning(sdt_mob_t mob, int fft_order, int interp_sxt, int interp_ch,
+                          int spread_stage_kb) {
+    SDT_CHECK_HANDLE(mob);
+    return guard([&] {
+        SpmeTuning t;
+        t.fft_order = fft_order;
+        t.interp_sxt = interp_sxt;
+        t.interp_ch = interp_ch;
+        t.spread_stage_kb = spread_stage_kb;
+        // single-column and Lanczos-batch launches alike
+        spme_of(mob)->set_tuning(t, 1);
+        spme_of(mob)->set_tuning(t, 100);
+    });
+}
+
 int sdt_debug_mob_launch_width(sdt_mob_t mob, int width) {
     SDT_CHECK_HANDLE(mob);
     return guard([&] {

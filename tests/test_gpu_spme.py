@@ -369,6 +369,27 @@ def test_spread_forms_identical(sdt, orc, prec):
         g.set_forms(0, 0, 16)
 
 
+def test_spread_stage_overflow_same_sums(sdt, orc):
+    # a 16 KB stage holds 50 candidates of the matrix-core spread, and a
+    # 32 x 4 x 4 block here has about 80: the blocks overflow it, and the
+    # workgroup then stages and sums each column block in chunks. The sums
+    # keep candidate order, so they equal the vector-ALU forms' (with their
+    # own chunking) to the bit, in one launch over three column blocks.
+    n, box = 3000, 30.0
+    g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, 48, 6, False, 83, sdt.FP32)
+    g.set_launch_width(64)
+    g.set_tuning(spread_stage_kb=16)
+    f = np.random.default_rng(84).standard_normal((3 * n, 40))
+    out = {}
+    for form in (0, 1, 2):
+        g.set_forms(0, 0, form)
+        out[form] = g.mul_vector(f)
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+    assert orc.frob_rel_err(o.mul_vector(f), out[0]) < 1e-5
+    with pytest.raises(Exception):
+        g.set_tuning(spread_stage_kb=8)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_fused_and_separate_xy_transforms_agree(sdt, orc, prec):
     # the x / y transforms run as one fused plane kernel where a plane fits
