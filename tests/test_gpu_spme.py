@@ -336,8 +336,8 @@ def test_wide_full_product_overlapped_streams(sdt, orc, prec, tol):
     assert orc.frob_rel_err(1.5 * ref + 0.5 * v0, full) < (1e-10 if prec == 0 else 1e-5)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_spread_forms_identical(sdt, orc, prec):
+@pytest.mark.parametrize("prec,K,p", [(0, 48, 6), (1, 48, 6), (1, 100, 6), (1, 120, 8), (1, 64, 4)])
+def test_spread_forms_identical(sdt, orc, prec, K, p):
     # the point-owner spread sums wide FP32 launches on the vector ALUs
     # (forces from scalar loads or LDS) or on the matrix cores (one
     # workgroup per column block, or per mesh block over all column blocks),
@@ -346,10 +346,11 @@ def test_spread_forms_identical(sdt, orc, prec):
     # rounding per product-add, so every form gives the same sums. The full
     # product (b >= 8) spreads from the column-blocked copy, which the
     # matrix-core forms need; each product is one launch (b = 20 and 40 are
-    # two and three column blocks, the last one partial); K = 48 leaves a
-    # 16-point partial x block.
+    # two and three column blocks, the last one partial); K = 48, 100 and
+    # 120 leave a 16-, a 4- (one active wave) and a 24-point partial x block,
+    # p = 4 and 8 the shortest and longest stencils.
     n, box = 3000, 30.0
-    g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, 48, 6, False, 81, prec)
+    g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, K, p, False, 81, prec)
     g.set_launch_width(64)
     forms = (0, 1, 2, 3, 4, 5, 6, 7, 11, 15)
     for b in (1, 20, 40):
