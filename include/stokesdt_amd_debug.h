@@ -41,12 +41,12 @@ int sdt_debug_spme_interp(sdt_mob_t mob, const double* mesh, double alpha, doubl
  * otherwise); z_form, threads and pencils per workgroup of the z transform:
  * 0 the built-in choice (FP32 (512, 16), FP64 (512, 8)), 1 .. 5 =
  * (1024, 16), (1024, 16), (512, 16), (512, 8), (256, 8), (256, 4);
- * spread_form: the point-owner spread's sums in wide FP32 launches (0 the
- * built-in choice = 3 + 8, 1 forces staged in LDS, 2 forces read by scalar
- * loads, 3 forces staged in LDS and summed by the matrix cores), plus 4 to
- * store the meshes as whole rows through the stage's LDS, plus 8 (with 3)
- * for one workgroup per mesh block over every column block; every form
- * gives the same sums;
+ * spread_form: the point-owner spread's sums in wide launches (0 the
+ * built-in choice: FP32 3 + 8, FP64 1 + 8; 1 forces staged in LDS, 2 forces
+ * read by scalar loads (FP32), 3 forces staged in LDS and summed by the
+ * matrix cores (FP32)), plus 4 to store the meshes as whole rows through
+ * the stage's LDS, plus 8 for one workgroup per mesh block over every
+ * column block (FP32 with 3); every form gives the same sums;
  * fft_form: 0 the fused x + y plane kernels where a plane fits LDS, 1 the
  * separate x and y passes. */
 /* Right-hand sides per launch of an SPME mobility's mulvector (default 16;

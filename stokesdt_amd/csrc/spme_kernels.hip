@@ -388,7 +388,7 @@ spread_pt_kernel(int K, int ts, int ntile, int nsx, const int* __restrict__ tile
     static_assert(!SGF || BLK, "scalar force reads need the blocked copy");
     static_assert(!MF || (BLK && !SGF && !WRAP && NQ == 48 && sizeof(T) == 4),
                   "matrix-core sums: FP32, blocked staged forces, 48 meshes, no wrap");
-    static_assert(!MA || MF, "column blocks in turn need the matrix-core sums");
+    static_assert(!MA || (BLK && !SGF && !WRAP), "column blocks in turn: blocked staged forces");
     const T* __restrict__ xbb = Xb + (size_t)byl_of(blockIdx.x + blockIdx.y * gridDim.x,
                                                     gridDim.x * gridDim.y, gridDim.x) * npos * NQ;
     T* stage = reinterpret_cast<T*>(smem_raw);
@@ -799,6 +799,24 @@ next_column_block:
             return;
         }
     }
+    if constexpr (MA && !MF) {
+        // vector-ALU sums (FP64): per-lane stores, then the next column block
+        if (!tiled_out) {
+            if (wave_on) {
+                const size_t pt = ((size_t)pz * K + py) * K + px;
+                constexpr int RBB = NQ / 3;
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const int c = q / RBB, r = q % RBB;
+                    if (r < rb) mesh[((size_t)(r0 + r) * 3 + c) * K3 + pt] = acc[q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; q++) acc[q] = T(0);
+            if (++cb < ncb) goto next_column_block;
+            return;
+        }
+    }
     if (!tiled_out) {
         if (!wave_on) return;
         const size_t pt = ((size_t)pz * K + py) * K + px;
@@ -863,10 +881,15 @@ next_column_block:
         __syncthreads();
     }
     if constexpr (MA) {
+        if constexpr (MF) {
 #pragma unroll
-        for (int m = 0; m < 4; m++)
+            for (int m = 0; m < 4; m++)
 #pragma unroll
-            for (int t = 0; t < 3; t++) accm[m][t] = f4{0.f, 0.f, 0.f, 0.f};
+                for (int t = 0; t < 3; t++) accm[m][t] = f4{0.f, 0.f, 0.f, 0.f};
+        } else {
+#pragma unroll
+            for (int q = 0; q < NQ; q++) acc[q] = T(0);
+        }
         if (++cb < ncb) goto next_column_block;
     }
     }
@@ -1708,13 +1731,17 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
         // Wide FP32 launches on the blocked copy sum on the matrix cores, one
         // workgroup per mesh block over every column block (K = 128, 100
         // columns: 1.08 ms against 2.17 ms for the vector-ALU sums with
-        // scalar force reads, the same sums to the bit); where the stencil
-        // can wrap, FP32 reads the blocked forces through the scalar path and
-        // FP64 stages them in LDS. Forms override this for tests.
+        // scalar force reads, the same sums to the bit); wide FP64 launches
+        // keep the vector-ALU sums over every column block in turn (2.85
+        // against 3.97 ms); where the stencil can wrap, FP32 reads the
+        // blocked forces through the scalar path and FP64 stages them in
+        // LDS per column block. Forms override this for tests.
         const int fr = spread_form & 3;
         const bool mf_ok = xb != nullptr && sizeof(T) == 4 && NQ == NQL && K >= 32 + porder - 1;
         const bool mf = mf_ok && (fr == 0 || fr == 3);
-        const bool ma = mf && (fr == 0 || (spread_form & 8) != 0);
+        const bool ma64 = sizeof(T) == 8 && xb != nullptr && NQ == NQL && K >= 32 + porder - 1 &&
+                          (fr == 0 || (spread_form & 8) != 0);
+        const bool ma = (mf && (fr == 0 || (spread_form & 8) != 0)) || ma64;
         const bool sgf = xb != nullptr && sizeof(T) == 4 && !mf && fr != 1;
         const int rec = ((3 * (porder + 1) + 3 + 3) & ~3) + (sgf ? 0 : NQ);
         // few meshes: a 48 KB stage leaves room for three workgroups per CU
@@ -1726,7 +1753,7 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
         const size_t tile = (size_t)NQ * 512 * sizeof(T);
         size_t lds = std::max((size_t)cap * (rec * sizeof(T) + 9 * sizeof(int)),
                               tile <= 49152 ? tile : tile / 2);
-        if (ma && !(spread_form & 4))
+        if (mf && ma && !(spread_form & 4))
             lds = (((size_t)cap * (rec * sizeof(T) + 9 * sizeof(int)) + 15) & ~(size_t)15) +
                   kMaTileBytes;
         const int nsx = (K + 31) / 32;
@@ -1746,6 +1773,9 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
                 else if (mf) SDT_SPREAD_PWS(PP, QQ, BB, WW, false, true, false);               \
                 else SDT_SPREAD_PWS(PP, QQ, BB, WW, false, false, false);                      \
             } else SDT_SPREAD_PWS(PP, QQ, BB, WW, false, false, false);                        \
+        } else if constexpr (BB && QQ == NQL && !WW) {                                      \
+            if (ma) SDT_SPREAD_PWS(PP, QQ, BB, WW, false, false, true);                        \
+            else SDT_SPREAD_PWS(PP, QQ, BB, WW, false, false, false);                          \
         } else {                                                                              \
             SDT_SPREAD_PWS(PP, QQ, BB, WW, false, false, false);                               \
         }                                                                                     \

@@ -341,7 +341,9 @@ def test_spread_forms_identical(sdt, orc, prec, K, p):
     # the point-owner spread sums wide FP32 launches on the vector ALUs
     # (forces from scalar loads or LDS) or on the matrix cores (one
     # workgroup per column block, or per mesh block over all column blocks),
-    # and stores its meshes per lane or as whole rows through LDS. The
+    # wide FP64 launches on the vector ALUs per column block or over all of
+    # them (form 1 + 8, the built-in one), and stores its meshes per lane or
+    # as whole rows through LDS. The
     # matrix cores take 4 candidates per step in list order with one
     # rounding per product-add, so every form gives the same sums. The full
     # product (b >= 8) spreads from the column-blocked copy, which the
@@ -352,7 +354,7 @@ def test_spread_forms_identical(sdt, orc, prec, K, p):
     n, box = 3000, 30.0
     g, o, _, _ = _pair(sdt, orc, n, box, 0.9, 5.0, K, p, False, 81, prec)
     g.set_launch_width(64)
-    forms = (0, 1, 2, 3, 4, 5, 6, 7, 11, 15)
+    forms = (0, 1, 2, 3, 4, 5, 6, 7, 9, 11, 15)
     for b in (1, 20, 40):
         f = np.random.default_rng(82 + b).standard_normal((3 * n, b))
         out = {}
