@@ -364,12 +364,12 @@ constexpr int kPtMaxTiles = 200;   // contributor bucket tiles of one block (hos
 // turn (grid y = 1). When all its candidates fit the stage, they are found
 // and their weights staged once; each further column block then only loads
 // the candidates' forces into the stage before its sums and stores. The
-// meshes leave through an LDS tile after the stage (16 columns x 4 rows x 32
+// meshes leave through LDS tiles after the stage (16 columns x 4 rows x 32
 // points of one z plane and component at a time, rows padded to 132 floats
-// so that the waves' 16-byte pieces land in distinct banks): every store
-// instruction then writes whole 128-byte rows.
+// so that the waves' 16-byte pieces land in distinct banks; two tiles used
+// in turn): every store instruction then writes whole 128-byte rows.
 constexpr int kMaTileStride = 132;                       // floats per column in the tile
-constexpr int kMaTileBytes = 16 * kMaTileStride * 4;     // 8448
+constexpr int kMaTileBytes = 2 * 16 * kMaTileStride * 4;     // two tiles, 16896
 template <typename T, int P, int NQ, bool BLK, bool WRAP, bool SGF = false, bool MF = false,
           bool MA = false>
 __global__ void __launch_bounds__(kPtThreads) __attribute__((amdgpu_waves_per_eu(NQ <= 12 ? 6 : (MA ? 4 : 1))))
@@ -749,8 +749,11 @@ next_column_block:
         // result register v of lane l holds row 4 (l / 16) + v of a tile:
         // x = v, y = l / 16 of z plane m; column l % 16 of component t
         if (MA && !tiled_out) {
-            // through the LDS tile: wave w writes x in [4w, 4w + 4) of every
-            // (column, row); thread e then stores 16 bytes of row e / 8
+            // through the LDS tiles: wave w writes x in [4w, 4w + 4) of every
+            // (column, row); thread e then stores 16 bytes of row e / 8. The
+            // rounds alternate between two tiles, so one barrier per round
+            // suffices: a thread writes a tile again only after the next
+            // round's barrier, which every thread reaches after its reads.
             const int jj = lane & 15, sy = lane >> 4;
             const int e = threadIdx.x;
             const int sj = e >> 5, sr = (e >> 3) & 3, sx = (e & 7) * 4;
@@ -758,14 +761,14 @@ next_column_block:
             for (int m = 0; m < 4; m++)
 #pragma unroll
                 for (int t = 0; t < 3; t++) {
-                    __syncthreads();   // the previous tile has been stored
+                    T* tb = ma_tile + ((m * 3 + t) & 1) * (16 * kMaTileStride);
                     if (wave_on)
-                        *reinterpret_cast<f4*>(ma_tile + jj * kMaTileStride + sy * 32 + 4 * wave) =
+                        *reinterpret_cast<f4*>(tb + jj * kMaTileStride + sy * 32 + 4 * wave) =
                             accm[m][t];
                     __syncthreads();
                     if (sj < rb && sx < xlen) {
                         const f4 v =
-                            *reinterpret_cast<const f4*>(ma_tile + sj * kMaTileStride + sr * 32 + sx);
+                            *reinterpret_cast<const f4*>(tb + sj * kMaTileStride + sr * 32 + sx);
                         *reinterpret_cast<f4*>(mesh + ((size_t)(r0 + sj) * 3 + t) * K3 +
                                                ((size_t)(z0 + m) * K + (y0 + sr)) * K + x0 + sx) = v;
                     }
@@ -1745,8 +1748,11 @@ void SpmeGpu<T>::spread(int b, const double* X, hipStream_t s, const T* xb) {
         const bool sgf = xb != nullptr && sizeof(T) == 4 && !mf && fr != 1;
         const int rec = ((3 * (porder + 1) + 3 + 3) & ~3) + (sgf ? 0 : NQ);
         // few meshes: a 48 KB stage leaves room for three workgroups per CU
+        // 60 KB for the matrix-core form over every column block, whose two
+        // output tiles (16.5 KB) also fit two workgroups per CU
         const int stage_bytes =
-            tn.spread_stage_kb > 0 ? tn.spread_stage_kb * 1024 : (NQ <= 12 ? 48 * 1024 : 65536);
+            tn.spread_stage_kb > 0 ? tn.spread_stage_kb * 1024
+                                   : (NQ <= 12 ? 48 * 1024 : (mf && ma ? 60 * 1024 : 65536));
         const int cap = (int)(stage_bytes / (rec * sizeof(T) + 9 * sizeof(int)));
         // the output tiles reuse the stage's LDS: all meshes, or half of
         // them above 48 KB (the allocation is at least that)
