@@ -89,6 +89,7 @@ void MobEwaldGpu::apply_packed(int mode, int b, double alpha, const double* X, d
     SDT_REQUIRE(updated_, SDT_ESTATE, "mulvector called before update");
     SDT_REQUIRE(mode == 0, SDT_EINVAL, "the dense Ewald mobility has no real/recip split");
     const int n = 3 * npos_;
+    if (fill_y) fill_y(stream());
     // row-major n x b blocks are column-major b x n: Y^T = alpha X^T M + beta Y^T
     SDT_EW_BLAS(rocblas_set_pointer_mode(blas_, rocblas_pointer_mode_host));
     SDT_EW_BLAS(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, b, n, n,

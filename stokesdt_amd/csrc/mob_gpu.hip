@@ -156,7 +156,12 @@ void MobSpmeGpu::apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const
     // capture (single-column tails stay on one stream).
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     SDT_HIP_CHECK(hipStreamIsCapturing(s, &cap));
-    if (mode == 0 && rc > 0 && b >= 8 && cap == hipStreamCaptureStatusNone) {
+    const bool two_streams = mode == 0 && rc > 0 && b >= 8 && cap == hipStreamCaptureStatusNone;
+    if (fill_y && !two_streams) {
+        StageScope scope(timer_, 6, s);
+        fill_y(s);
+    }
+    if (two_streams) {
         const bool share = spme->blocked_cols(b) == rc;
         {
             StageScope scope(timer_, 9, s, b);
@@ -169,6 +174,10 @@ void MobSpmeGpu::apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const
         { StageScope sc(tm, 0, s, b); spme->spread(b, X, s, share ? xb : nullptr); }
         SDT_HIP_CHECK(hipEventRecord(ev_fork_, s));
         SDT_HIP_CHECK(hipStreamWaitEvent(side_, ev_fork_, 0));
+        if (fill_y) {
+            StageScope scope(timer_, 6, side_);
+            fill_y(side_);
+        }
         {
             StageScope scope(timer_, 5, side_, b);
             real_matvec<T>(npos_, rp, ci, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha, X,
@@ -528,11 +537,15 @@ void BrwnLanczosGpu::lanczos_batch(MobGpu* mob, int btot, int gbase, const doubl
         for (int gi = 0; gi < ng; gi++) hm[gi] = grp[gi].done ? 0 : 1;
         SDT_HIP_CHECK(hipMemcpyAsync(d_mask_.get(), hm, sizeof(int) * ng, hipMemcpyHostToDevice, s));
         double beta = 0.0;
+        // W = V_{i-1} B_{i-1}^T: its coefficients here, the block product
+        // inside the matvec's fill of W (beside the transforms of a wide SPME
+        // product) or here before a column subset is taken
+        auto fill_w = [&](hipStream_t hs) {
+            tsmm_groups1(n, btot, bs, V + (size_t)(iter - 1) * blk, d_S_.get(), 1.0, 0.0, W, hs);
+        };
         if (iter > 0) {
-            // W = V_{i-1} B_{i-1}^T
             StageScope sc(timer, 6, s);
             lz_coef(ng, bs, btot, d_mask_.get(), nullptr, prev, nullptr, d_S_.get(), 1, s);
-            tsmm_groups1(n, btot, bs, V + (size_t)(iter - 1) * blk, d_S_.get(), 1.0, 0.0, W, s);
             beta = -1.0;
         }
         // W = M V_i - V_{i-1} B^T, one application for the columns of
@@ -543,8 +556,17 @@ void BrwnLanczosGpu::lanczos_batch(MobGpu* mob, int btot, int gbase, const doubl
             if (!g.done)
                 for (int c = 0; c < g.b; c++) act.push_back(g.c0 + c);
         if ((int)act.size() == btot || 4 * (int)act.size() > 3 * btot) {
+            struct FillReset {
+                MobGpu* m;
+                ~FillReset() { m->fill_y = nullptr; }
+            } reset{mob};
+            if (iter > 0) mob->fill_y = fill_w;
             mob->apply_packed(0, btot, 1.0, Vi, beta, W);
         } else {
+            if (iter > 0) {
+                StageScope sc(timer, 6, s);
+                fill_w(s);
+            }
             const int na = (int)act.size();
             d_cols_.reserve(na);
             d_Xa_.reserve((size_t)n * na);

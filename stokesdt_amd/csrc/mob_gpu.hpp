@@ -2,6 +2,7 @@
 // BD step driver (BDSimBox::Advance) behind the C ABI.
 #pragma once
 #include <atomic>
+#include <functional>
 #include <memory>
 #include "block_gpu.hpp"
 #include "host_linalg.hpp"
@@ -38,6 +39,12 @@ class MobGpu {
     virtual hipStream_t stream() const = 0;
     virtual const int* perm() const = 0;
     virtual bool updated() const = 0;
+    // Optional work that writes Y's initial contents (the beta Y of the next
+    // apply_packed), enqueued by apply_packed on the stream it is given
+    // before Y is read: a wide SPME product runs it on its second stream
+    // beside the transforms, other products first on stream(). Set by the
+    // caller around one apply_packed.
+    std::function<void(hipStream_t)> fill_y;
 };
 
 // MobSpme (src/mob/mob_spme.h:21-125) with device-resident state.
