@@ -149,11 +149,13 @@ void MobSpmeGpu::apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const
     T* xb = reinterpret_cast<T*>(d_xblk_.get());
     bool copied = false;
     // wide full products (the Lanczos batches): the real space runs on the
-    // side stream while the transforms run here (it is bound by gather
-    // latency, they by VALU and LDS issue; started after the spread, whose
-    // VALU work it would slow); it writes Y = beta Y + alpha M_real X first
-    // and the interpolation then adds the reciprocal part. Not under graph
-    // capture (single-column tails stay on one stream).
+    // side stream while the z and inverse transforms run here (it is bound
+    // by gather latency; started after the forward transform, which streams
+    // HBM at its fastest and slowed most beside it: K = 128, 100 columns,
+    // 7.5 against 7.7 ms when it started after the spread); it writes
+    // Y = beta Y + alpha M_real X first and the interpolation then adds the
+    // reciprocal part. Not under graph capture (single-column tails stay on
+    // one stream).
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     SDT_HIP_CHECK(hipStreamIsCapturing(s, &cap));
     const bool two_streams = mode == 0 && rc > 0 && b >= 8 && cap == hipStreamCaptureStatusNone;
@@ -168,10 +170,12 @@ void MobSpmeGpu::apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const
             real_block_copy<T>(npos_, b, X, xb, s);
         }
         // the reciprocal pipeline of SpmeGpu::apply, stage by stage, with the
-        // fork after the spread and the join before the interpolation
+        // fork after the forward transform and the join before the
+        // interpolation
         StageTimer* tm = spme->timer;
         spme->reserve_rhs(b);
         { StageScope sc(tm, 0, s, b); spme->spread(b, X, s, share ? xb : nullptr); }
+        { StageScope sc(tm, 1, s, b); spme->fft_forward(b, s); }
         SDT_HIP_CHECK(hipEventRecord(ev_fork_, s));
         SDT_HIP_CHECK(hipStreamWaitEvent(side_, ev_fork_, 0));
         if (fill_y) {
@@ -185,7 +189,6 @@ void MobSpmeGpu::apply_as(SpmeGpu<T>* spme, int mode, int b, double alpha, const
             real_overlap(ovl_, d_pos_s_.get(), d_rdi_s_.get(), box_, xi_, b, alpha, X, Y, side_);
         }
         SDT_HIP_CHECK(hipEventRecord(ev_join_, side_));
-        { StageScope sc(tm, 1, s, b); spme->fft_forward(b, s); }
         { StageScope sc(tm, 2, s, b); spme->influence(b, s); }
         { StageScope sc(tm, 3, s, b); spme->fft_backward(b, s); }
         SDT_HIP_CHECK(hipStreamWaitEvent(s, ev_join_, 0));
