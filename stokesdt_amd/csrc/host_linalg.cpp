@@ -59,48 +59,141 @@ void BandSqrt::compute(int m, int kd, const double* T, int kb, double* S) {
     compute_band(m, kd, Tb.data(), ldt, kb, S);
 }
 
+namespace {
+
+typedef BandSqrt::Rot Rot;
+
+// rows (p, p+1) of the m x KB row-major X by [[c, sg s], [-sg s, c]] for each
+// recorded rotation, in order or in reverse (sg = +-1: the products by -s
+// equal those of the written-out forms, sign changes being exact)
+template <int KB>
+void replay_fixed(const Rot* r, size_t n, bool reverse, double sg, double* X, int kb) {
+    const int nk = KB ? KB : kb;   // KB = 0: the width at run time
+    for (size_t t = 0; t < n; t++) {
+        const Rot& g = r[reverse ? n - 1 - t : t];
+        const double c = g.c, s = sg * g.s;
+        double* xp = X + (size_t)g.p * nk;
+        double* xq = xp + nk;
+        for (int k = 0; k < nk; k++) {
+            const double a = xp[k], b = xq[k];
+            xp[k] = c * a + s * b;
+            xq[k] = -s * a + c * b;
+        }
+    }
+}
+
+void replay(const std::vector<Rot>& r, bool reverse, double sg, double* X, int kb) {
+    switch (kb) {
+        case 4: replay_fixed<4>(r.data(), r.size(), reverse, sg, X, kb); break;
+        case 8: replay_fixed<8>(r.data(), r.size(), reverse, sg, X, kb); break;
+        case 16: replay_fixed<16>(r.data(), r.size(), reverse, sg, X, kb); break;
+        default: replay_fixed<0>(r.data(), r.size(), reverse, sg, X, kb); break;
+    }
+}
+
+// implicit QL with Wilkinson-type shifts on the tridiagonal (d, e), the
+// rotations recorded. Each rotation R^T = [[c, -s], [s, c]] is also applied
+// to rows (i, i+1) of X as it is made (X <- Z^T X): the row updates do not
+// feed the sweep's chain of square roots and divisions and run beside it.
+template <int KB>
+void ql_sweeps(int m, double* d, double* e, std::vector<Rot>& rot, double* X, int kb) {
+    const int nk = KB ? KB : kb;
+    const double eps = 2.220446049250313e-16;
+    for (int l = 0; l < m; l++) {
+        int iter = 0, mm;
+        do {
+            for (mm = l; mm < m - 1; mm++) {
+                double dd = fabs(d[mm]) + fabs(d[mm + 1]);
+                if (fabs(e[mm]) <= eps * dd) break;
+            }
+            if (mm != l) {
+                if (++iter > 60) break;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = sqrt(g * g + 1.0);
+                g = d[mm] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+                double s = 1.0, c = 1.0, p = 0.0;
+                bool early = false;
+                for (int i = mm - 1; i >= l; i--) {
+                    double f = s * e[i], bb = c * e[i];
+                    r = sqrt(f * f + g * g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[mm] = 0.0;
+                        early = true;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * bb;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - bb;
+                    rot.push_back(Rot{i, c, s});
+                    double* xi = X + (size_t)i * nk;
+                    double* xj = xi + nk;
+                    for (int k = 0; k < nk; k++) {
+                        const double xa = xi[k], xb = xj[k];
+                        xi[k] = c * xa - s * xb;
+                        xj[k] = s * xa + c * xb;
+                    }
+                }
+                if (early) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[mm] = 0.0;
+            }
+        } while (mm != l);
+    }
+}
+
+}  // namespace
+
 void BandSqrt::compute_band(int m, int kd, const double* Tb, int ldt, int kb, double* S) {
-    // band storage of half-width w (the bandwidth kd plus room for the
-    // bulge chased during the reduction): element (i, j), |i - j| <= w, at
-    // A[i * ld + j - i + w]
+    // lower band storage of half-width w (the bandwidth kd plus room for the
+    // bulge chased during the reduction): A(i, j), 0 <= i - j <= w, at
+    // A[i * ld + i - j]; the rotations keep A symmetric, so the upper half is
+    // never formed
     const int w = kd + 2;
-    const int ld = 2 * w + 1;
+    const int ld = w + 1;
     A_.assign((size_t)m * ld, 0.0);
     double* A = A_.data();
-    auto at = [&](int i, int j) -> double& { return A[(size_t)i * ld + (j - i + w)]; };
+    auto at = [&](int i, int j) -> double& { return A[(size_t)i * ld + (i - j)]; };
     for (int i = 0; i < m; i++)
-        for (int j = std::max(0, i - kd); j <= std::min(m - 1, i + kd); j++)
-            at(i, j) = Tb[(size_t)i * ldt + (j - i + kd)];
+        for (int j = std::max(0, i - kd); j <= i; j++) at(i, j) = Tb[(size_t)i * ldt + (j - i + kd)];
     band_rot_.clear();
     ql_rot_.clear();
     auto rotate = [&](int p, double c, double s) {
         // A <- G A G^T with G = [[c, s], [-s, c]] on rows/cols (p, p+1)
         const int q = p + 1;
-        const int lo = std::max(0, q - w), hi = std::min(m - 1, p + w);
-        // rows p and q (columns lo..hi lie within both bands)
-        double* rp = &at(p, lo);
-        double* rq = &at(q, lo);
-        for (int l = 0; l <= hi - lo; l++) {
-            double ap = rp[l], aq = rq[l];
-            rp[l] = c * ap + s * aq;
-            rq[l] = -s * ap + c * aq;
+        double* rp = A + (size_t)p * ld;   // rp[p - l] = A(p, l)
+        double* rq = A + (size_t)q * ld;   // rq[q - l] = A(q, l)
+        // rows p and q left of the 2x2 block
+        for (int l = std::max(0, q - w); l < p; l++) {
+            const double ap = rp[p - l], aq = rq[q - l];
+            rp[p - l] = c * ap + s * aq;
+            rq[q - l] = -s * ap + c * aq;
         }
-        // columns p and q, by symmetry, outside the 2x2 block
-        for (int l = lo; l <= hi; l++) {
-            if (l == p || l == q) continue;
-            at(l, p) = at(p, l);
-            at(l, q) = at(q, l);
+        // columns p and q below it (rows l, entries A(l, p) and A(l, q))
+        const int l1 = std::min(m - 1, p + w);
+        for (int l = q + 1; l <= l1; l++) {
+            double* rl = A + (size_t)l * ld;
+            const double ap = rl[l - p], aq = rl[l - q];
+            rl[l - p] = c * ap + s * aq;
+            rl[l - q] = -s * ap + c * aq;
         }
-        // the 2x2 block: rows were rotated, now the columns
-        double app = at(p, p), apq = at(p, q), aqp = at(q, p), aqq = at(q, q);
-        at(p, p) = c * app + s * apq;
-        at(p, q) = -s * app + c * apq;
-        at(q, p) = c * aqp + s * aqq;
-        at(q, q) = -s * aqp + c * aqq;
+        // the 2x2 block: its rows, then its columns
+        const double app = rp[0], apq = rq[1], aqq = rq[0];
+        const double x1 = c * app + s * apq, x2 = c * apq + s * aqq;
+        const double x3 = -s * app + c * apq, x4 = -s * apq + c * aqq;
+        rp[0] = c * x1 + s * x2;
+        rq[1] = c * x3 + s * x4;
+        rq[0] = -s * x3 + c * x4;
         band_rot_.push_back(Rot{p, c, s});
     };
     auto zero_with = [&](int p, int col) {
-        // zero A[p+1][col] against A[p][col]
+        // zero A(p+1, col) against A(p, col), col < p
         const double a = at(p, col), b = at(p + 1, col);
         if (b == 0.0) return;
         const double r = sqrt(a * a + b * b);
@@ -125,103 +218,28 @@ void BandSqrt::compute_band(int m, int kd, const double* Tb, int ldt, int kb, do
     e_.assign(m, 0.0);
     for (int i = 0; i < m; i++) d_[i] = at(i, i);
     for (int i = 0; i + 1 < m; i++) e_[i] = at(i + 1, i);
-    // implicit QL with Wilkinson-type shifts, rotations recorded
-    {
-        double* d = d_.data();
-        double* e = e_.data();
-        const double eps = 2.220446049250313e-16;
-        for (int l = 0; l < m; l++) {
-            int iter = 0, mm;
-            do {
-                for (mm = l; mm < m - 1; mm++) {
-                    double dd = fabs(d[mm]) + fabs(d[mm + 1]);
-                    if (fabs(e[mm]) <= eps * dd) break;
-                }
-                if (mm != l) {
-                    if (++iter > 60) break;
-                    double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                    double r = sqrt(g * g + 1.0);
-                    g = d[mm] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
-                    double s = 1.0, c = 1.0, p = 0.0;
-                    bool early = false;
-                    for (int i = mm - 1; i >= l; i--) {
-                        double f = s * e[i], bb = c * e[i];
-                        r = sqrt(f * f + g * g);
-                        e[i + 1] = r;
-                        if (r == 0.0) {
-                            d[i + 1] -= p;
-                            e[mm] = 0.0;
-                            early = true;
-                            break;
-                        }
-                        s = f / r;
-                        c = g / r;
-                        g = d[i + 1] - p;
-                        r = (d[i] - g) * s + 2.0 * c * bb;
-                        p = s * r;
-                        d[i + 1] = g + p;
-                        g = c * r - bb;
-                        ql_rot_.push_back(Rot{i, c, s});
-                    }
-                    if (early) continue;
-                    d[l] -= p;
-                    e[l] = g;
-                    e[mm] = 0.0;
-                }
-            } while (mm != l);
-        }
-    }
     // X = E_1 (m x kb, row-major)
     X_.assign((size_t)m * kb, 0.0);
     double* X = X_.data();
     for (int c = 0; c < kb && c < m; c++) X[(size_t)c * kb + c] = 1.0;
-    // X <- Q_band^T X : apply each band rotation G (rows p, p+1) in order
-    for (const Rot& r : band_rot_) {
-        double* xp = X + (size_t)r.p * kb;
-        double* xq = xp + kb;
-        for (int c = 0; c < kb; c++) {
-            double a = xp[c], b = xq[c];
-            xp[c] = r.c * a + r.s * b;
-            xq[c] = -r.s * a + r.c * b;
-        }
-    }
-    // X <- Z^T X : QL rotation R^T = [[c, -s], [s, c]] on rows (i, i+1)
-    for (const Rot& r : ql_rot_) {
-        double* xi = X + (size_t)r.p * kb;
-        double* xj = xi + kb;
-        for (int c = 0; c < kb; c++) {
-            double a = xi[c], b = xj[c];
-            xi[c] = r.c * a - r.s * b;
-            xj[c] = r.s * a + r.c * b;
-        }
+    // X <- Q_band^T X : each band rotation G (rows p, p+1) in order
+    replay(band_rot_, false, 1.0, X, kb);
+    // X <- Z^T X inside the QL sweeps
+    switch (kb) {
+        case 4: ql_sweeps<4>(m, d_.data(), e_.data(), ql_rot_, X, kb); break;
+        case 8: ql_sweeps<8>(m, d_.data(), e_.data(), ql_rot_, X, kb); break;
+        case 16: ql_sweeps<16>(m, d_.data(), e_.data(), ql_rot_, X, kb); break;
+        default: ql_sweeps<0>(m, d_.data(), e_.data(), ql_rot_, X, kb); break;
     }
     // g(Lambda) = sqrt
     for (int i = 0; i < m; i++) {
         double g = sqrt(std::max(d_[i], 0.0));
         for (int c = 0; c < kb; c++) X[(size_t)i * kb + c] *= g;
     }
-    // X <- Z X : replay QL rotations in reverse applying R
-    for (size_t t = ql_rot_.size(); t-- > 0;) {
-        const Rot& r = ql_rot_[t];
-        double* xi = X + (size_t)r.p * kb;
-        double* xj = xi + kb;
-        for (int c = 0; c < kb; c++) {
-            double a = xi[c], b = xj[c];
-            xi[c] = r.c * a + r.s * b;
-            xj[c] = -r.s * a + r.c * b;
-        }
-    }
-    // X <- Q_band X : replay band rotations in reverse applying G^T
-    for (size_t t = band_rot_.size(); t-- > 0;) {
-        const Rot& r = band_rot_[t];
-        double* xp = X + (size_t)r.p * kb;
-        double* xq = xp + kb;
-        for (int c = 0; c < kb; c++) {
-            double a = xp[c], b = xq[c];
-            xp[c] = r.c * a - r.s * b;
-            xq[c] = r.s * a + r.c * b;
-        }
-    }
+    // X <- Z X : the QL rotations in reverse, applying R
+    replay(ql_rot_, true, 1.0, X, kb);
+    // X <- Q_band X : the band rotations in reverse, applying G^T
+    replay(band_rot_, true, -1.0, X, kb);
     memcpy(S, X, sizeof(double) * (size_t)m * kb);
 }
 

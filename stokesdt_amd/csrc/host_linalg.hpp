@@ -18,6 +18,7 @@ void mul_upper(int b, const double* A, const double* B, double* C);
 // tridiagonal, with every rotation applied to the thin m x kb matrix only,
 // so the cost is O(m^2 (kd + kb)) instead of O(m^3). Eigenvalues below zero
 // (rounding) are clamped to zero.
+// The lower triangle of T's band is read (T is taken to be symmetric).
 class BandSqrt {
   public:
     void compute(int m, int kd, const double* T, int kb, double* S);
@@ -25,8 +26,10 @@ class BandSqrt {
     // Tb[i * ldt + j - i + kd] (rows 0 .. m-1; entries outside T unused)
     void compute_band(int m, int kd, const double* Tb, int ldt, int kb, double* S);
 
-  private:
+    // a Givens rotation on rows (p, p+1)
     struct Rot { int p; double c, s; };
+
+  private:
     std::vector<double> A_;
     std::vector<Rot> band_rot_, ql_rot_;
     std::vector<double> d_, e_, X_;
