@@ -37,7 +37,10 @@ def test_version(sdt):
 
 def test_band_sqrt_host_routine(sdt):
     rng = np.random.default_rng(3)
-    for m, kd, kb in [(8, 2, 2), (64, 8, 8), (200, 8, 8), (50, 1, 1), (33, 5, 3), (16, 16, 16)]:
+    # widths 4, 8 and 16 take fixed-width loops, the others the general one;
+    # (440, 8, 8) is the size of a converging 8-column Lanczos group
+    for m, kd, kb in [(8, 2, 2), (64, 8, 8), (200, 8, 8), (50, 1, 1), (33, 5, 3), (16, 16, 16),
+                      (120, 4, 4), (90, 6, 4), (440, 8, 8), (70, 3, 5)]:
         A = rng.standard_normal((m, m))
         B = A @ A.T / m + 2 * np.eye(m)
         mask = np.abs(np.subtract.outer(np.arange(m), np.arange(m))) <= kd
