@@ -49,6 +49,22 @@ def test_band_sqrt_host_routine(sdt):
         ref = (V * np.sqrt(w)) @ V.T[:, :kb]
         S = sdt.band_sqrt(T, kd, kb)
         assert np.abs(S - ref).max() <= 1e-13 * np.abs(ref).max()
+    # a band matrix that splits: equal diagonal blocks with no coupling give
+    # every eigenvalue twice and zero off-diagonals in the middle (the QL
+    # sweeps deflate there); T^{1/2} then acts on each block alone
+    for m2, kd, kb in [(40, 4, 4), (60, 8, 8), (25, 3, 2)]:
+        A = rng.standard_normal((m2, m2))
+        B = A @ A.T / m2 + np.eye(m2)
+        mask = np.abs(np.subtract.outer(np.arange(m2), np.arange(m2))) <= kd
+        B = np.where(mask, B, 0.0)
+        T = np.zeros((2 * m2, 2 * m2))
+        T[:m2, :m2] = B
+        T[m2:, m2:] = B
+        w, V = np.linalg.eigh(B)
+        ref = np.zeros((2 * m2, kb))
+        ref[:m2] = ((V * np.sqrt(w)) @ V.T)[:, :kb]
+        S = sdt.band_sqrt(T, kd, kb)
+        assert np.abs(S - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
 def test_rpy_spme_argument_errors(sdt):
