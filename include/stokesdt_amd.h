@@ -280,8 +280,8 @@ int sdt_bd_autotune(sdt_bd_t bd, const char* tune_cache, int* choices, double* m
  * cutoff rmax[s] and mesh dim[s]; porder, precision, Lanczos, steric, dt and
  * mob_interval are shared; system s draws its Gaussian blocks from seed + s.
  * Each mobility interval's updates and Lanczos solves run side by side (one
- * HIP stream per system); every other step is one HIP graph launch holding
- * every system's step tail. init with autotune != 0 picks each system's SPME
+ * HIP stream per system); every other step is four HIP graph launches
+ * (lanes of systems) holding every system's step tail. init with autotune != 0 picks each system's SPME
  * launch choices (FFT radix order, interp strip and chunk, spread stage) by
  * timing them on the system, for single-column and Lanczos-batch launches;
  * tune_cache (may be NULL) names a text file of earlier choices.

@@ -899,7 +899,8 @@ class PlainBDSim:
 class Ensemble:
     """Independent BD systems of uneven size, box and mesh advanced together:
     each mobility interval's updates and Lanczos solves side by side, every
-    other step one HIP graph launch over all systems' step tails.
+    other step four HIP graph launches (lanes of systems) over all systems'
+    step tails.
     systems: list of dicts with keys pos (flat AoS), rdi, box, xi, rmax, dim."""
 
     def __init__(self, systems, porder, precision, max_iters, max_nrhs, tol, steric_r0,

@@ -1,11 +1,13 @@
 // Ensemble step: every mobility interval, each system's update and Lanczos
 // solve run concurrently (host threads, one HIP stream per system); every
 // other step, the tails of all systems (forces, M f, position update) are
-// one HIP graph launch: the systems' tails are captured as parallel branches
-// forked from and joined to one origin stream, and the graph is replayed for
-// the rest of the interval.
+// HIP graph launches: the systems' tails are captured as parallel branches
+// forked from and joined to an origin stream, one graph per lane of systems,
+// and the lanes' graphs are replayed side by side for the rest of the
+// interval.
 #include <algorithm>
 #include <chrono>
+#include <numeric>
 #include <thread>
 #include "ensemble.hpp"
 
@@ -27,8 +29,26 @@ EnsembleGpu::EnsembleGpu(const std::vector<EnsembleSystem>& sys, const double* p
                                    mob_interval, seed + s));
         off += (size_t)e.npos;
     }
-    SDT_HIP_CHECK(stream_create(&s0_));
-    SDT_HIP_CHECK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    // systems dealt to the lanes largest first, in snake order, so that the
+    // lanes hold similar numbers of systems and of particles
+    const int nl = std::min<int>(kGraphLanes, (int)sys.size());
+    std::vector<int> order(sys.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return sys[a].npos > sys[b].npos; });
+    lane_.assign(nl, std::vector<int>());
+    for (size_t k = 0; k < order.size(); k++) {
+        const int r = (int)(k % (size_t)(2 * nl));
+        lane_[r < nl ? r : 2 * nl - 1 - r].push_back(order[k]);
+    }
+    s0_.assign(nl, nullptr);
+    fork_.assign(nl, nullptr);
+    graph_.assign(nl, nullptr);
+    exec_.assign(nl, nullptr);
+    for (int l = 0; l < nl; l++) {
+        SDT_HIP_CHECK(stream_create(&s0_[l]));
+        SDT_HIP_CHECK(hipEventCreateWithFlags(&fork_[l], hipEventDisableTiming));
+    }
     join_.resize(bd_.size(), nullptr);
     for (hipEvent_t& e : join_) SDT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
@@ -37,16 +57,22 @@ EnsembleGpu::~EnsembleGpu() {
     release_graph();
     for (hipEvent_t e : join_)
         if (e) (void)hipEventDestroy(e);
-    if (fork_) (void)hipEventDestroy(fork_);
-    if (s0_) (void)stream_destroy(s0_);
+    for (hipEvent_t e : fork_)
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t st : s0_)
+        if (st) (void)stream_destroy(st);
 }
 
 void EnsembleGpu::release_graph() {
     CaptureLock lk(capture_mutex());
-    if (exec_) (void)hipGraphExecDestroy(exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
-    exec_ = nullptr;
-    graph_ = nullptr;
+    for (hipGraphExec_t& x : exec_) {
+        if (x) (void)hipGraphExecDestroy(x);
+        x = nullptr;
+    }
+    for (hipGraph_t& g : graph_) {
+        if (g) (void)hipGraphDestroy(g);
+        g = nullptr;
+    }
     graph_block_ = -1;
 }
 
@@ -99,7 +125,6 @@ void EnsembleGpu::init(bool tune, const std::string& cache_path) {
 }
 
 void EnsembleGpu::tail_all() {
-    const int n = (int)bd_.size();
     bool warm = true;
     for (auto& b : bd_) warm = warm && b->warm_block() == b->block();
     if (!use_graph_ || !warm) {
@@ -112,27 +137,37 @@ void EnsembleGpu::tail_all() {
         // no other thread's allocation, free or synchronous copy meanwhile
         CaptureLock lk(capture_mutex());
         release_graph();
-        SDT_HIP_CHECK(hipStreamBeginCapture(s0_, hipStreamCaptureModeThreadLocal));
-        try {
-            SDT_HIP_CHECK(hipEventRecord(fork_, s0_));
-            for (int s = 0; s < n; s++) {
-                hipStream_t ss = bd_[s]->stream();
-                SDT_HIP_CHECK(hipStreamWaitEvent(ss, fork_, 0));
-                bd_[s]->enqueue_tail(ss);
-                SDT_HIP_CHECK(hipEventRecord(join_[s], ss));
-                SDT_HIP_CHECK(hipStreamWaitEvent(s0_, join_[s], 0));
+        for (size_t l = 0; l < lane_.size(); l++) {
+            SDT_HIP_CHECK(hipStreamBeginCapture(s0_[l], hipStreamCaptureModeThreadLocal));
+            try {
+                SDT_HIP_CHECK(hipEventRecord(fork_[l], s0_[l]));
+                for (int s : lane_[l]) {
+                    hipStream_t ss = bd_[s]->stream();
+                    SDT_HIP_CHECK(hipStreamWaitEvent(ss, fork_[l], 0));
+                    bd_[s]->enqueue_tail(ss);
+                    SDT_HIP_CHECK(hipEventRecord(join_[s], ss));
+                    SDT_HIP_CHECK(hipStreamWaitEvent(s0_[l], join_[s], 0));
+                }
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s0_[l], &g);
+                if (g) (void)hipGraphDestroy(g);
+                release_graph();
+                throw;
             }
-        } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(s0_, &g);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
+            SDT_HIP_CHECK(hipStreamEndCapture(s0_[l], &graph_[l]));
+            SDT_HIP_CHECK(hipGraphInstantiate(&exec_[l], graph_[l], nullptr, nullptr, 0));
         }
-        SDT_HIP_CHECK(hipStreamEndCapture(s0_, &graph_));
-        SDT_HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
         graph_block_ = blk;
     }
-    SDT_HIP_CHECK(hipGraphLaunch(exec_, s0_));
+    // the lanes' graphs are submitted from one host thread each
+    const int nl = (int)lane_.size();
+    std::vector<hipError_t> st(nl, hipSuccess);
+    std::vector<std::thread> th;
+    for (int l = 1; l < nl; l++) th.emplace_back([&, l] { st[l] = hipGraphLaunch(exec_[l], s0_[l]); });
+    st[0] = hipGraphLaunch(exec_[0], s0_[0]);
+    for (auto& x : th) x.join();
+    for (int l = 0; l < nl; l++) SDT_HIP_CHECK(st[l]);
     graph_launches_++;
 }
 
@@ -147,7 +182,7 @@ void EnsembleGpu::advance(int nsteps, double* step_ms) {
         });
         tail_all();
         for (auto& b : bd_) b->end_step();
-        SDT_HIP_CHECK(hipStreamSynchronize(s0_));
+        for (hipStream_t st : s0_) SDT_HIP_CHECK(hipStreamSynchronize(st));
         for (auto& b : bd_) SDT_HIP_CHECK(hipStreamSynchronize(b->stream()));
         if (step_ms)
             step_ms[k] = std::chrono::duration<double, std::milli>(

@@ -43,11 +43,17 @@ class EnsembleGpu {
     void release_graph();
     std::vector<std::unique_ptr<BDGpu>> bd_;
     std::vector<TuneResult> tune_;   // per system: narrow (2 s) and wide (2 s + 1)
-    hipStream_t s0_ = nullptr;       // origin stream of the step graph
-    std::vector<hipEvent_t> join_;
-    hipEvent_t fork_ = nullptr;
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t exec_ = nullptr;
+    // The step graph is split into lanes: each lane's systems are one graph
+    // on the lane's own origin stream, and the lanes' graphs are launched
+    // side by side (the branches of one graph are dispatched one after
+    // another, and the small systems' tails are bound by that dispatch).
+    static constexpr int kGraphLanes = 4;
+    std::vector<std::vector<int>> lane_;   // the systems of each lane
+    std::vector<hipStream_t> s0_;          // origin stream of each lane's graph
+    std::vector<hipEvent_t> join_;         // per system
+    std::vector<hipEvent_t> fork_;         // per lane
+    std::vector<hipGraph_t> graph_;
+    std::vector<hipGraphExec_t> exec_;
     long long graph_block_ = -1;
     long long graph_launches_ = 0;
     bool use_graph_ = true, inited_ = false;

@@ -3,7 +3,7 @@
 - A BD run whose step tails are replayed HIP graphs equals the eager run bit
   for bit (the graph holds the same launches).
 - Every member of an ensemble (uneven particle counts, boxes and meshes; the
-  interval work side by side, the tails of all systems in one graph launch)
+  interval work side by side, the tails of all systems in four lane graphs)
   equals the same system run alone (BDSimBox::Advance,
   src/simbox/simbox_bd.cc:296-366) bit for bit, and an FP64 member matches the
   oracle's step.
