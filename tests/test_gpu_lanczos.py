@@ -95,6 +95,27 @@ def test_lanczos_matches_oracle(sdt, orc, maxnrhs):
     assert np.abs(bl.iters() - oit).max() <= 3
 
 
+@pytest.mark.parametrize("tol", [1e-2, 1e-3])
+@pytest.mark.parametrize("maxnrhs", [4, 8])
+def test_lanczos_early_stop_same_iterate_as_oracle(sdt, orc, tol, maxnrhs):
+    # Stopped within 5-10 iterations the basis is still orthonormal and the
+    # oracle's iterate is stable to rounding (1e-14 under a 1e-15
+    # perturbation of z), while consecutive iterates differ by 8e-4 to 9e-3:
+    # the same iteration count and 1e-10 agreement pin that the returned Y_i
+    # is the iterate the convergence test accepted (the first i with
+    # ||Y_i - Y_{i-1}||_F / ||Z||_F < tol, src/brwn/brwn_lanczos.cc:198-206).
+    n, box = 300, 20.0
+    g, o = _mob(sdt, orc, n, box, 32, 6, sdt.FP64, 4)
+    z = orc.gaussian(7, 0, 3 * n * 20).reshape(20, 3 * n).T.copy()
+    bl = sdt.BrwnLanczos(3 * n, 100, maxnrhs, tol)
+    assert bl.init()
+    y, ok = bl.compute(g, z)
+    oy, oit = orc.lanczos(o, z, 100, maxnrhs, tol)
+    assert ok
+    assert list(bl.iters()) == list(oit)
+    assert orc.frob_rel_err(oy, y) < 1e-10
+
+
 def test_lanczos_many_groups_and_batches(sdt, orc):
     # 130 right-hand sides in groups of 8: two lockstep batches (128 + 2
     # columns), a narrower last group in each

@@ -316,6 +316,33 @@ def test_lanczos_matches_eigen_sqrt(orc, ewald1000):
     assert it1[0] > 0 and it3[0] > 0
 
 
+def test_lanczos_iterate_rounding_sensitivity(orc):
+    # The reference's block Lanczos (src/brwn/brwn_lanczos.cc:36-221) has no
+    # reorthogonalisation. Stopped early (tol 1e-2, 8 iterations at N = 2000)
+    # its iterate is stable to rounding; once the basis has lost
+    # orthogonality (tol 1e-6, 48 iterations) a 1e-15 relative perturbation
+    # of z moves the returned iterate by a fraction of the tolerance
+    # (3e-7 to 9e-7 here; the threaded sums of the host products alone, run
+    # twice on the same z, do the same and can stop an iteration apart).
+    # Two correct implementations with different rounding (QR method,
+    # summation order) agree to that level only.
+    n, box = 2000, 30.0
+    pos, rdi = uniform_system(n, box, 5)
+    o = orc.MobSpme(n, rdi, box, 0.9, 5.0, 64, 6)
+    assert o.init()
+    o.update(pos, rdi)
+    z = orc.gaussian(8, 1, 3 * n * 8).reshape(8, 3 * n).T.copy()
+    z2 = z * (1.0 + 1e-15 * np.random.default_rng(0).standard_normal(z.shape))
+    moved = {}
+    for tol in (1e-2, 1e-6):
+        y, it = orc.lanczos(o, z, 100, 8, tol)
+        y2, it2 = orc.lanczos(o, z2, 100, 8, tol)
+        assert abs(int(it[0]) - int(it2[0])) <= (0 if tol == 1e-2 else 1)
+        moved[tol] = orc.frob_rel_err(y, y2)
+    assert moved[1e-2] < 1e-12
+    assert 1e-8 < moved[1e-6] < 1e-5
+
+
 def test_lanczos_non_convergence_reported(orc):
     rng = np.random.default_rng(22)
     A = rng.standard_normal((90, 90))
