@@ -75,6 +75,29 @@ def test_ensemble_eager_equals_graphs(sdt):
         assert np.array_equal(a, b)
 
 
+def test_ensemble_lanes_equal_eager(sdt):
+    # nine systems over the four graph lanes (dealt largest first in snake
+    # order: lanes of 3, 2, 2 and 2 systems), two mobility intervals, so each
+    # lane's graph is captured twice: graph replays equal eager tails and the
+    # same systems run alone, bit for bit
+    spec = [(200 + 60 * k, 0.1 + 0.025 * k, (24, 30, 32)[k % 3]) for k in range(9)]
+    systems = _systems(sdt, spec, seed=31)
+    res = []
+    for graphs in (True, False):
+        ens = sdt.Ensemble(systems, 6, sdt.FP32, 100, 4, 1e-6, 2.0, 125.0, 1e-3, 4, 50)
+        ens.init(autotune=False)
+        ens.set_graphs(graphs)
+        ens.advance(9)
+        res.append([ens.positions(s) for s in range(9)])
+    for s, sy in enumerate(systems):
+        assert np.array_equal(res[0][s], res[1][s]), s
+        bd = sdt.BDSim(sy["pos"], sy["rdi"], sy["box"], sy["xi"], sy["rmax"], sy["dim"], 6,
+                       sdt.FP32, 100, 4, 1e-6, 2.0, 125.0, 1e-3, 4, 50 + s)
+        bd.init()
+        bd.advance(9)
+        assert np.array_equal(res[0][s], bd.positions()), s
+
+
 def test_ensemble_fp64_member_matches_oracle(sdt, orc):
     systems = _systems(sdt, SYS[:2], seed=21)
     ens = sdt.Ensemble(systems, 6, sdt.FP64, 100, 4, 1e-7, 2.0, 125.0, 1e-3, 5, 300)
